@@ -1,0 +1,607 @@
+// C ABI of libedhip (see include/edhip.h).  Every entry catches all C++
+// exceptions and turns them into a status code plus a per-thread message.
+#include <algorithm>
+#include <cstring>
+
+#include "edh_ctx.h"
+
+struct edh_model {
+  std::unique_ptr<edh::Model> m;
+};
+struct edh_ctx {
+  edh::Ctx c;
+};
+
+namespace {
+thread_local std::string g_err;
+}
+
+#define API_BEGIN try {
+#define API_END                                   \
+  return EDH_OK;                                  \
+  }                                               \
+  catch (const edh::Error& e) {                   \
+    g_err = e.what();                             \
+    return e.code;                                \
+  }                                               \
+  catch (const std::bad_alloc&) {                 \
+    g_err = "out of host memory";                 \
+    return EDH_ERR_NOMEM;                         \
+  }                                               \
+  catch (const std::exception& e) {               \
+    g_err = e.what();                             \
+    return EDH_ERR_INTERNAL;                      \
+  }                                               \
+  catch (...) {                                   \
+    g_err = "unknown error";                      \
+    return EDH_ERR_INTERNAL;                      \
+  }
+
+namespace edh {
+
+Ctx::~Ctx() {
+  dev.release();
+  stage_a.release();
+  stage_b.release();
+  partial.release();
+  scal.release();
+  for (auto& b : lz) b.release();
+  for (auto& b : krylov) b.release();
+  evecs.release();
+  small.release();
+  gtab.release();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+void require_ctx(edh_ctx* ctx) { EDH_CHECK(ctx != nullptr, EDH_ERR_ARG, "null context"); }
+void use_device(Ctx& c) { EDH_HIP(hipSetDevice(c.device)); }
+void require_sector(Ctx& c) { EDH_CHECK(c.has_sector, EDH_ERR_STATE, "no sector set (call edh_set_sector)"); }
+
+// Copies a vector argument to the device when it is a host pointer.
+const double* stage_in(Ctx& c, DevBuf& b, const double* p, int64_t n, int mem) {
+  if (mem == EDH_MEM_DEVICE) return p;
+  EDH_CHECK(mem == EDH_MEM_HOST, EDH_ERR_ARG, "invalid memory kind");
+  b.ensure((size_t)n * sizeof(double));
+  if (n) EDH_HIP(hipMemcpyAsync(b.p, p, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  return b.as<double>();
+}
+
+int popc_parity(int32_t x) { return __builtin_popcount((unsigned)x) & 1; }
+
+// Index/sign maps for c (annihilate) or c^+ on orbital orb of a vector in
+// sector (nup, ndn): target index (u', d') reads source (mu[u'], md[d']) times
+// su[u'] sd[d'] (-1 index = zero).  The sign counts occupied orbitals with a
+// smaller index (FermionicModel.h:45-68); for a down orbital that includes
+// every up electron.
+void adag_maps(int Ns, int nup, int ndn, int orb, bool annihilate, int& nup2, int& ndn2, std::vector<int32_t>& mu,
+               std::vector<int32_t>& md, std::vector<int8_t>& su, std::vector<int8_t>& sd) {
+  const int spin = orb >= Ns ? 1 : 0;
+  const int site = orb - spin * Ns;
+  const int pb = Ns - 1 - site;
+  nup2 = nup + (spin == 0 ? (annihilate ? -1 : 1) : 0);
+  ndn2 = ndn + (spin == 1 ? (annihilate ? -1 : 1) : 0);
+  std::vector<int32_t> tu = spin_basis(Ns, nup2), td = spin_basis(Ns, ndn2);
+  const int nsrc = spin == 0 ? nup : ndn;
+  std::vector<int32_t> src = spin_basis(Ns, nsrc);
+  std::vector<int32_t> rank(1u << Ns, -1);
+  for (size_t r = 0; r < src.size(); ++r) rank[src[r]] = (int32_t)r;
+  auto build = [&](const std::vector<int32_t>& tgt, std::vector<int32_t>& mp, std::vector<int8_t>& sg, int extra) {
+    mp.assign(tgt.size(), -1);
+    sg.assign(tgt.size(), 1);
+    for (size_t r = 0; r < tgt.size(); ++r) {
+      const int32_t b = tgt[r];
+      const bool has = (b >> pb) & 1;
+      if (annihilate == has) continue;  // c needs the orbital empty afterwards, c^+ occupied
+      const int32_t s = b ^ (1 << pb);
+      mp[r] = rank[s];
+      sg[r] = (int8_t)(((popc_parity(s >> (pb + 1)) + extra) & 1) ? -1 : 1);
+    }
+  };
+  auto identity = [](size_t D, std::vector<int32_t>& mp, std::vector<int8_t>& sg) {
+    mp.resize(D);
+    sg.assign(D, 1);
+    for (size_t r = 0; r < D; ++r) mp[r] = (int32_t)r;
+  };
+  if (spin == 0) {
+    build(tu, mu, su, 0);
+    identity(td.size(), md, sd);
+  } else {
+    identity(tu.size(), mu, su);
+    build(td, md, sd, nup & 1);
+  }
+}
+
+}  // namespace
+}  // namespace edh
+
+using namespace edh;
+
+extern "C" {
+
+const char* edh_last_error(void) { return g_err.c_str(); }
+int edh_version(void) { return 100; }
+
+int edh_model_hubbard(edh_model** out, int Ns, const double* t, const double* U, const double* J, const double* xmu,
+                      const double* Hmag) {
+  API_BEGIN
+  EDH_CHECK(out && t && U && xmu, EDH_ERR_ARG, "null argument");
+  edh_model* m = new edh_model();
+  m->m.reset(build_hubbard(Ns, t, U, J, xmu, Hmag));
+  *out = m;
+  API_END
+}
+
+int edh_model_siam(edh_model** out, int Ns, int ml, const double* Epsk, const double* Vk, const double* H0,
+                   const double* Ut, double mu) {
+  API_BEGIN
+  EDH_CHECK(out && Vk && H0 && Ut && (Epsk || Ns == ml), EDH_ERR_ARG, "null argument");
+  edh_model* m = new edh_model();
+  m->m.reset(build_siam(Ns, ml, Epsk, Vk, H0, Ut, mu));
+  *out = m;
+  API_END
+}
+
+int edh_model_free(edh_model* m) {
+  delete m;
+  return EDH_OK;
+}
+
+int edh_model_sites(const edh_model* m, int* Ns, int* ml) {
+  API_BEGIN
+  EDH_CHECK(m, EDH_ERR_ARG, "null model");
+  if (Ns) *Ns = m->m->Ns;
+  if (ml) *ml = m->m->ml;
+  API_END
+}
+
+int edh_model_transitions(const edh_model* m, int which, int* count, int* ints, double* vals) {
+  API_BEGIN
+  EDH_CHECK(m && count, EDH_ERR_ARG, "null argument");
+  const std::vector<Transition>& L = which == 0 ? m->m->T : m->m->V;
+  *count = (int)L.size();
+  if (ints && vals)
+    for (size_t q = 0; q < L.size(); ++q) {
+      const Transition& t = L[q];
+      int* o = ints + 7 * q;
+      o[0] = t.kind;
+      o[1] = t.i;
+      o[2] = t.j;
+      o[3] = t.k;
+      o[4] = t.l;
+      o[5] = t.s;
+      o[6] = t.sp;
+      vals[q] = t.value;
+    }
+  API_END
+}
+
+int edh_model_diagonal(const edh_model* m, long long state, double* out) {
+  API_BEGIN
+  EDH_CHECK(m && out, EDH_ERR_ARG, "null argument");
+  *out = m->m->diagonal_reference(state);
+  API_END
+}
+
+int edh_sector_csr(const edh_model* m, int nup, int ndn, int which, int64_t* nrows, int64_t* nnz, int* rowp, int* col,
+                   double* val) {
+  API_BEGIN
+  EDH_CHECK(m && nrows && nnz, EDH_ERR_ARG, "null argument");
+  SectorTables st;
+  build_sector(*m->m, nup, ndn, st, false);
+  const SpinTables& sp = which == 0 ? st.up : st.dn;
+  *nrows = sp.D;
+  *nnz = (int64_t)sp.col.size();
+  if (rowp) std::copy(sp.rowp.begin(), sp.rowp.end(), rowp);
+  if (col) std::copy(sp.col.begin(), sp.col.end(), col);
+  if (val) std::copy(sp.val.begin(), sp.val.end(), val);
+  API_END
+}
+
+int edh_sector_diagonal(const edh_model* m, int nup, int ndn, double* out) {
+  API_BEGIN
+  EDH_CHECK(m && out, EDH_ERR_ARG, "null argument");
+  SectorTables st;
+  build_sector(*m->m, nup, ndn, st, false);
+  for (int64_t u = 0; u < st.up.D; ++u)
+    for (int64_t d = 0; d < st.dn.D; ++d) out[u * st.dn.D + d] = host_diag(*m->m, st, u, d);
+  API_END
+}
+
+int edh_create(edh_ctx** out, int device) {
+  API_BEGIN
+  EDH_CHECK(out, EDH_ERR_ARG, "null argument");
+  int n = 0;
+  EDH_HIP(hipGetDeviceCount(&n));
+  EDH_CHECK(device >= 0 && device < n, EDH_ERR_ARG, "invalid device index");
+  std::unique_ptr<edh_ctx> c(new edh_ctx());
+  c->c.device = device;
+  EDH_HIP(hipSetDevice(device));
+  EDH_HIP(hipStreamCreate(&c->c.stream));
+  *out = c.release();
+  API_END
+}
+
+int edh_destroy(edh_ctx* ctx) {
+  API_BEGIN
+  if (ctx) {
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+  }
+  API_END
+}
+
+int edh_set_model(edh_ctx* ctx, const edh_model* m) {
+  API_BEGIN
+  require_ctx(ctx);
+  EDH_CHECK(m, EDH_ERR_ARG, "null model");
+  Ctx& c = ctx->c;
+  c.model.reset(new Model(*m->m));
+  c.has_sector = false;
+  API_END
+}
+
+int edh_set_sector(edh_ctx* ctx, int nup, int ndn, int64_t* dim) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  EDH_CHECK(c.model, EDH_ERR_STATE, "no model set");
+  use_device(c);
+  c.has_sector = false;
+  build_sector(*c.model, nup, ndn, c.tabs, false);
+  upload_sector(c.tabs, c.model->Ns, c.dev, c.stream);
+  c.has_sector = true;
+  if (dim) *dim = c.tabs.N;
+  API_END
+}
+
+int edh_sector_info(edh_ctx* ctx, int64_t* dup, int64_t* ddn, int64_t* tbytes) {
+  API_BEGIN
+  require_ctx(ctx);
+  require_sector(ctx->c);
+  if (dup) *dup = ctx->c.tabs.up.D;
+  if (ddn) *ddn = ctx->c.tabs.dn.D;
+  if (tbytes) *tbytes = ctx->c.tabs.table_bytes;
+  API_END
+}
+
+int edh_alloc(edh_ctx* ctx, size_t bytes, void** dptr) {
+  API_BEGIN
+  require_ctx(ctx);
+  EDH_CHECK(dptr, EDH_ERR_ARG, "null argument");
+  use_device(ctx->c);
+  hipError_t e = hipMalloc(dptr, bytes);
+  EDH_CHECK(e == hipSuccess, EDH_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  API_END
+}
+
+int edh_free(edh_ctx* ctx, void* dptr) {
+  API_BEGIN
+  require_ctx(ctx);
+  use_device(ctx->c);
+  if (dptr) EDH_HIP(hipFree(dptr));
+  API_END
+}
+
+int edh_memcpy(edh_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  use_device(c);
+  hipMemcpyKind k = kind == EDH_COPY_H2D ? hipMemcpyHostToDevice
+                                         : (kind == EDH_COPY_D2H ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  EDH_HIP(hipMemcpyAsync(dst, src, bytes, k, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int edh_synchronize(edh_ctx* ctx) {
+  API_BEGIN
+  require_ctx(ctx);
+  use_device(ctx->c);
+  EDH_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+int edh_av(edh_ctx* ctx, const double* v, double* w, int64_t n, int clear, int mem) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(n == c.tabs.N, EDH_ERR_ARG, "vector size does not match the sector dimension");
+  EDH_CHECK(v && w, EDH_ERR_ARG, "null vector");
+  const double* dv = stage_in(c, c.stage_a, v, n, mem);
+  double* dw = w;
+  if (mem == EDH_MEM_HOST) {
+    c.stage_b.ensure((size_t)n * sizeof(double));
+    dw = c.stage_b.as<double>();
+    if (!clear) EDH_HIP(hipMemcpyAsync(dw, w, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+  }
+  EDH_CHECK(dv != dw, EDH_ERR_ARG, "v and w must not alias");
+  HvArgs a{};
+  a.v = dv;
+  a.w = dw;
+  a.wprev = dw;
+  a.vscale = 1.0;
+  a.wscale = 1.0;
+  a.mode = clear ? 0 : 1;
+  launch_hv(c.dev, a, c.stream);
+  if (mem == EDH_MEM_HOST) {
+    EDH_HIP(hipMemcpyAsync(w, dw, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    EDH_HIP(hipStreamSynchronize(c.stream));
+  }
+  API_END
+}
+
+int edh_av_time(edh_ctx* ctx, const double* v, double* w, int clear, int warmup, int reps, double* avg_ms) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(v && w && avg_ms && reps > 0, EDH_ERR_ARG, "bad argument");
+  HvArgs a{};
+  a.v = v;
+  a.w = w;
+  a.wprev = w;
+  a.vscale = 1.0;
+  a.wscale = 1.0;
+  a.mode = clear ? 0 : 1;
+  for (int i = 0; i < warmup; ++i) launch_hv(c.dev, a, c.stream);
+  hipEvent_t e0, e1;
+  EDH_HIP(hipEventCreate(&e0));
+  EDH_HIP(hipEventCreate(&e1));
+  EDH_HIP(hipEventRecord(e0, c.stream));
+  for (int i = 0; i < reps; ++i) launch_hv(c.dev, a, c.stream);
+  EDH_HIP(hipEventRecord(e1, c.stream));
+  EDH_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  EDH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / reps;
+  API_END
+}
+
+int edh_vv(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* out, int mem) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  use_device(c);
+  EDH_CHECK(out, EDH_ERR_ARG, "null output");
+  const double* da = stage_in(c, c.stage_a, a, n, mem);
+  const double* db = (a == b) ? da : stage_in(c, c.stage_b, b, n, mem);
+  c.partial.ensure((size_t)kMaxPartials * kMaxBasis * sizeof(double));
+  c.scal.ensure(64 * sizeof(double));
+  double* res = c.scal.as<double>();
+  dev_dot(da, db, n, c.partial.as<double>(), res, c.stream);
+  EDH_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+}  // extern "C"
+
+namespace edh {
+namespace {
+
+// Runs the recurrence with v0 already in lz[0]; returns steps; fills host alpha/beta.
+int lanczos_device(Ctx& c, int nl, double* alpha, double* beta, double* step_ms) {
+  const int64_t n = c.tabs.N;
+  hipStream_t s = c.stream;
+  const int nblk = std::max(hv_num_blocks(c.dev), kMaxPartials);
+  c.partial.ensure((size_t)std::max(nblk, kMaxPartials * kMaxBasis) * sizeof(double));
+  c.scal.ensure((size_t)(8 + 2 * nl + 2) * sizeof(double));
+  double* sc = c.scal.as<double>();
+  int* st = (int*)(sc + 4);
+  double* dalpha = sc + 8;
+  double* dbeta = dalpha + nl;
+  EDH_HIP(hipMemsetAsync(dalpha, 0, (size_t)(2 * nl + 1) * sizeof(double), s));
+  dev_lanczos_init(sc, st, s);
+  double* cur = c.lz[0].as<double>();
+  double* prev = c.lz[1].as<double>();
+  double* out = c.lz[2].as<double>();
+  double* partial = c.partial.as<double>();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (step_ms) {
+    EDH_HIP(hipEventCreate(&e0));
+    EDH_HIP(hipEventCreate(&e1));
+    EDH_HIP(hipEventRecord(e0, s));
+  }
+  for (int k = 1; k <= nl; ++k) {
+    HvArgs a{};
+    a.v = cur;
+    a.w = out;
+    a.wprev = prev;
+    a.mode = k > 1 ? 1 : 0;
+    a.dscale = sc;
+    a.dot_partial = partial;
+    a.done = st + 1;
+    launch_hv(c.dev, a, s);
+    dev_lanczos_alpha(partial, hv_num_blocks(c.dev), dalpha, k, sc, st, s);
+    dev_lanczos_passB(out, cur, sc + 2, n, partial, st + 1, dbeta, k, nl, sc, st, s);
+    double* t = prev;
+    prev = cur;
+    cur = out;
+    out = t;
+  }
+  if (step_ms) {
+    EDH_HIP(hipEventRecord(e1, s));
+    EDH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    EDH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *step_ms = (double)ms / nl;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  int host_st[2];
+  EDH_HIP(hipMemcpyAsync(alpha, dalpha, nl * sizeof(double), hipMemcpyDeviceToHost, s));
+  EDH_HIP(hipMemcpyAsync(beta, dbeta, (nl + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+  EDH_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, s));
+  EDH_HIP(hipStreamSynchronize(s));
+  return host_st[0];
+}
+
+}  // namespace
+}  // namespace edh
+
+extern "C" {
+
+int edh_lanczos(edh_ctx* ctx, const double* v0, int64_t n, int nl, double* alpha, double* beta, int* nlanc, int mem) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(n == c.tabs.N, EDH_ERR_ARG, "vector size does not match the sector dimension");
+  EDH_CHECK(nl >= 1 && alpha && beta && nlanc && v0, EDH_ERR_ARG, "bad argument");
+  for (auto& b : c.lz) b.ensure((size_t)n * sizeof(double));
+  EDH_HIP(hipMemcpyAsync(c.lz[0].p, v0, (size_t)n * sizeof(double),
+                         mem == EDH_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c.stream));
+  *nlanc = lanczos_device(c, nl, alpha, beta, nullptr);
+  API_END
+}
+
+int edh_lanczos_time(edh_ctx* ctx, const double* v0, int nl, double* alpha, double* beta, int* nlanc,
+                     double* avg_step_ms) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(nl >= 1 && alpha && beta && nlanc && v0 && avg_step_ms, EDH_ERR_ARG, "bad argument");
+  const int64_t n = c.tabs.N;
+  for (auto& b : c.lz) b.ensure((size_t)n * sizeof(double));
+  EDH_HIP(hipMemcpyAsync(c.lz[0].p, v0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  *nlanc = lanczos_device(c, nl, alpha, beta, avg_step_ms);
+  API_END
+}
+
+int edh_adag(edh_ctx* ctx, int nup, int ndn, int orb, int annihilate, const double* in, double* out, int64_t* out_n,
+             int mem) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  EDH_CHECK(c.model, EDH_ERR_STATE, "no model set");
+  use_device(c);
+  const int Ns = c.model->Ns;
+  EDH_CHECK(orb >= 0 && orb < 2 * Ns, EDH_ERR_ARG, "orbital index out of range");
+  EDH_CHECK(nup >= 0 && nup <= Ns && ndn >= 0 && ndn <= Ns, EDH_ERR_ARG, "sector out of range");
+  int nup2, ndn2;
+  const int spin = orb >= Ns ? 1 : 0;
+  nup2 = nup + (spin == 0 ? (annihilate ? -1 : 1) : 0);
+  ndn2 = ndn + (spin == 1 ? (annihilate ? -1 : 1) : 0);
+  Binomial C(Ns);
+  if (nup2 < 0 || nup2 > Ns || ndn2 < 0 || ndn2 > Ns) {
+    if (out_n) *out_n = 0;
+    return EDH_OK;
+  }
+  const int64_t Du2 = C(Ns, nup2), Dd2 = C(Ns, ndn2), Dd = C(Ns, ndn), N = C(Ns, nup) * Dd;
+  if (out_n) *out_n = Du2 * Dd2;
+  if (!out) return EDH_OK;
+  EDH_CHECK(in, EDH_ERR_ARG, "null input");
+  std::vector<int32_t> mu, md;
+  std::vector<int8_t> su, sd;
+  adag_maps(Ns, nup, ndn, orb, annihilate != 0, nup2, ndn2, mu, md, su, sd);
+  const size_t bytes = (mu.size() + md.size()) * 4 + su.size() + sd.size() + 64;
+  c.gtab.ensure(bytes);
+  char* g = c.gtab.as<char>();
+  int32_t* dmu = (int32_t*)g;
+  int32_t* dmd = dmu + mu.size();
+  int8_t* dsu = (int8_t*)(dmd + md.size());
+  int8_t* dsd = dsu + su.size();
+  EDH_HIP(hipMemcpyAsync(dmu, mu.data(), mu.size() * 4, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(dmd, md.data(), md.size() * 4, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(dsu, su.data(), su.size(), hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(dsd, sd.data(), sd.size(), hipMemcpyHostToDevice, c.stream));
+  const double* din = stage_in(c, c.stage_a, in, N, mem);
+  double* dout = out;
+  if (mem == EDH_MEM_HOST) {
+    c.stage_b.ensure((size_t)Du2 * Dd2 * sizeof(double));
+    dout = c.stage_b.as<double>();
+  }
+  dev_separable_gather(din, Dd, dmu, dmd, dsu, dsd, dout, Du2, Dd2, c.stream);
+  if (mem == EDH_MEM_HOST)
+    EDH_HIP(hipMemcpyAsync(out, dout, (size_t)Du2 * Dd2 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int edh_local_op(edh_ctx* ctx, int nup, int ndn, int op, int norb, const int* orbs, const double* in, double* out,
+                 int mem) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  EDH_CHECK(c.model, EDH_ERR_STATE, "no model set");
+  use_device(c);
+  const int Ns = c.model->Ns;
+  EDH_CHECK(op == EDH_OP_SZ || op == EDH_OP_N, EDH_ERR_ARG, "unknown operator");
+  EDH_CHECK(norb >= 0 && (norb == 0 || orbs) && in && out, EDH_ERR_ARG, "bad argument");
+  for (int q = 0; q < norb; ++q) EDH_CHECK(orbs[q] >= 0 && orbs[q] < Ns, EDH_ERR_ARG, "orbital out of range");
+  std::vector<int32_t> bu = spin_basis(Ns, nup), bd = spin_basis(Ns, ndn);
+  std::vector<double> fu(bu.size(), 0.0), fd(bd.size(), 0.0);
+  const double cu = op == EDH_OP_SZ ? 0.5 : 1.0, cd = op == EDH_OP_SZ ? -0.5 : 1.0;
+  for (size_t r = 0; r < bu.size(); ++r)
+    for (int q = 0; q < norb; ++q) fu[r] += cu * ((bu[r] >> (Ns - 1 - orbs[q])) & 1);
+  for (size_t r = 0; r < bd.size(); ++r)
+    for (int q = 0; q < norb; ++q) fd[r] += cd * ((bd[r] >> (Ns - 1 - orbs[q])) & 1);
+  c.gtab.ensure((fu.size() + fd.size()) * sizeof(double));
+  double* dfu = c.gtab.as<double>();
+  double* dfd = dfu + fu.size();
+  EDH_HIP(hipMemcpyAsync(dfu, fu.data(), fu.size() * 8, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(dfd, fd.data(), fd.size() * 8, hipMemcpyHostToDevice, c.stream));
+  const int64_t N = (int64_t)bu.size() * bd.size();
+  const double* din = stage_in(c, c.stage_a, in, N, mem);
+  double* dout = out;
+  if (mem == EDH_MEM_HOST) {
+    c.stage_b.ensure((size_t)N * sizeof(double));
+    dout = c.stage_b.as<double>();
+  }
+  dev_separable_scale(din, dfu, dfd, dout, (int64_t)bu.size(), (int64_t)bd.size(), c.stream);
+  if (mem == EDH_MEM_HOST) EDH_HIP(hipMemcpyAsync(out, dout, (size_t)N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int edh_cf_eval(edh_ctx* ctx, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign, double expb,
+                const double* alpha, const double* beta, double* out_re, double* out_im) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  use_device(c);
+  EDH_CHECK(nz >= 0 && nlanc >= 1 && zre && zim && alpha && beta && out_re && out_im, EDH_ERR_ARG, "bad argument");
+  const size_t need = (size_t)(4 * nz + 2 * nlanc + 2) * sizeof(double);
+  c.small.ensure(need);
+  double* dz = c.small.as<double>();
+  double* dzi = dz + nz;
+  double* da = dzi + nz;
+  double* db = da + nlanc;
+  double* dr = db + nlanc + 1;
+  double* di = dr + nz;
+  EDH_HIP(hipMemcpyAsync(dz, zre, nz * 8, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(dzi, zim, nz * 8, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(da, alpha, nlanc * 8, hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipMemcpyAsync(db, beta, (nlanc + 1) * 8, hipMemcpyHostToDevice, c.stream));
+  dev_cf_eval(nz, dz, dzi, ev, nlanc, isign, expb, da, db, dr, di, c.stream);
+  EDH_HIP(hipMemcpyAsync(out_re, dr, nz * 8, hipMemcpyDeviceToHost, c.stream));
+  EDH_HIP(hipMemcpyAsync(out_im, di, nz * 8, hipMemcpyDeviceToHost, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int edh_eigsh(edh_ctx* ctx, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals, double* evecs,
+              int* nconv, int* n_av) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(evals && nconv && n_av && nev >= 1 && ncv >= 2, EDH_ERR_ARG, "bad argument");
+  eigsh_run(c, nev, ncv, tol, maxitr, seed, evals, evecs, nconv, n_av);
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// Context object behind the edh_ctx handle.
+#pragma once
+
+#include "edh_internal.h"
+#include "edh_kernels.h"
+
+namespace edh {
+
+// Growable device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    if (p) EDH_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Model> model;
+  SectorTables tabs;
+  DevSector dev;
+  bool has_sector = false;
+  DevBuf stage_a, stage_b;  // host-pointer staging
+  DevBuf partial;           // reduction partials
+  DevBuf scal;              // device scalars
+  DevBuf lz[3];             // Lanczos work vectors
+  DevBuf krylov[2];         // thick-restart bases (double-buffered)
+  DevBuf evecs;             // eigenvectors of the last eigsh (resident)
+  DevBuf small;             // small device arrays (coefficients, tables)
+  DevBuf gtab;              // gather tables for a_adag / operators
+  int n_evecs = 0;
+  ~Ctx();
+};
+
+void eigsh_run(Ctx& c, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals, double* evecs,
+               int* nconv, int* n_av);
+
+// Diagonal element of state (u, d) of the current sector from the host tables.
+double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d);
+
+}  // namespace edh

@@ -1,0 +1,192 @@
+// Internal declarations shared by the edhip library sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/edhip.h"
+
+namespace edh {
+
+typedef long long state_t;
+
+// Error carrying a status code across the C ABI.
+struct Error : public std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define EDH_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      throw ::edh::Error(EDH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define EDH_CHECK(cond, code, msg)                 \
+  do {                                             \
+    if (!(cond)) throw ::edh::Error((code), (msg)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Binomials and the per-spin lexicographic basis (reference Combination.h:23-50,
+// SzSymmetry.h:191-216).  Site s of a spin channel lives at bit (Ns-1-s).
+// ---------------------------------------------------------------------------
+struct Binomial {
+  int n = 0;
+  std::vector<int64_t> t;
+  explicit Binomial(int n_ = 0);
+  int64_t operator()(int a, int b) const {
+    if (a < 0 || b < 0 || b > a || a > n) return 0;
+    return t[a * (n + 1) + b];
+  }
+};
+
+std::vector<int32_t> spin_basis(int Ns, int k);
+
+// ---------------------------------------------------------------------------
+// Model description (host side).
+// ---------------------------------------------------------------------------
+struct Transition {
+  int kind;  // 0: c^+_{j s} c_{i s};  1: c^+_{i s} c^+_{j s'} c_{l s'} c_{k s}
+  int i, j, k, l, s, sp;
+  double value;
+};
+
+struct Model {
+  int kind = 0;  // EDH_MODEL_HUBBARD / EDH_MODEL_SIAM
+  int Ns = 0;
+  int ml = 0;         // interacting (impurity) orbitals: Ns for Hubbard
+  int hop_sign = -1;  // sign rule of kind-0 transitions: -s1 s2 (Hubbard) or +s1 s2 (SIAM)
+  std::vector<Transition> T, V;
+  // Diagonal decomposition: sum_s e[s][i] n_is + sum_s sum_{i!=j} S[s][i][j] n_is n_js
+  //                        + sum_{i,j} C[i][j] n_i(up) n_j(dn); site indices.
+  std::vector<double> e[2], S[2], C;
+  // Original parameters, kept for reference-order diagonal evaluation.
+  std::vector<double> t, U, J, xmu, Hmag;
+  std::vector<double> Epsk, Vk, H0, Ut;
+  double mu = 0.0;
+
+  double uten(int s1, int s2, int i, int j, int k, int l) const {
+    int m = ml;
+    return Ut[((((s1 * 2 + s2) * m + i) * m + j) * m + k) * m + l];
+  }
+  int valid(const Transition& tr, state_t nst) const;
+  double apply(const Transition& tr, state_t nst, state_t& k, int& sign) const;
+  double diagonal_reference(state_t st) const;  // model.diagonal() in the reference's order
+};
+
+Model* build_hubbard(int Ns, const double* t, const double* U, const double* J, const double* xmu,
+                     const double* Hmag);
+Model* build_siam(int Ns, int ml, const double* Epsk, const double* Vk, const double* H0, const double* Ut,
+                  double mu);
+
+// ---------------------------------------------------------------------------
+// Per-sector tables (host copies) and their device mirror.
+// ---------------------------------------------------------------------------
+enum CrossMode { CROSS_NONE = 0, CROSS_POPC = 1, CROSS_IMP = 2, CROSS_GENERAL = 3 };
+
+// Packed hopping entry: column in the low 22 bits, value-table index in the high 10.
+constexpr int kColBits = 22;
+constexpr uint32_t kColMask = (1u << kColBits) - 1u;
+constexpr int kMaxValues = 1 << (32 - kColBits);
+constexpr int kSlice = 64;  // sliced-ELL slice height (one wavefront)
+
+struct SpinTables {
+  int nel = 0;
+  int64_t D = 0;
+  std::vector<int32_t> bits;    // basis patterns
+  std::vector<double> ediag;    // single-spin diagonal part per basis state
+  std::vector<int32_t> rowp;    // CSR row pointer (source-state rows)
+  std::vector<int32_t> col;     // CSR columns (reference CRSMatrix semantics)
+  std::vector<double> val;      // CSR values
+  std::vector<uint32_t> packed; // packed CSR entries
+  std::vector<int32_t> sl_ptr, sl_len;  // sliced ELL
+  std::vector<uint32_t> ell;
+  std::vector<int32_t> rank;    // rank by bit pattern (2^Ns), -1 if not in the basis
+};
+
+struct KanEntry {
+  int32_t new_a, new_b;  // impurity bit patterns after the move
+  double coef;           // value * sign
+};
+
+struct SectorTables {
+  int nup = 0, ndn = 0;
+  int64_t N = 0;
+  SpinTables up, dn;
+  std::vector<double> values;  // distinct hopping values (index 0 = 0.0)
+  int imp_ml = 0;  // impurity (interacting) orbitals: their bits sit above nb = Ns - imp_ml bath bits
+  int cross_mode = CROSS_NONE;
+  double cross_u = 0.0;
+  std::vector<double> cross_imp;    // (2^ml)^2 table
+  std::vector<double> cross_rowtab; // Dup x Ns (GENERAL: R_u[bitpos])
+  // Kanamori / general V terms on impurity bits
+  int kan_ml = 0;
+  std::vector<int32_t> kan_ptr;  // (2^ml)^2 + 1
+  std::vector<KanEntry> kan;
+  int64_t table_bytes = 0;  // per-spin hopping table bytes (algorithmic T)
+};
+
+void build_sector(const Model& m, int nup, int ndn, SectorTables& st, bool need_rank);
+
+// Device mirror of one sector.
+struct DevSpin {
+  int64_t D = 0;
+  int32_t* bits = nullptr;
+  double* ediag = nullptr;
+  int32_t* rowp = nullptr;
+  uint32_t* packed = nullptr;
+  int32_t* sl_ptr = nullptr;
+  int32_t* sl_len = nullptr;
+  uint32_t* ell = nullptr;
+  int32_t* rank = nullptr;
+};
+
+struct DevSector {
+  int Ns = 0, nup = 0, ndn = 0;
+  int64_t N = 0;
+  DevSpin up, dn;
+  double* values = nullptr;
+  int imp_ml = 0;
+  int cross_mode = 0;
+  double cross_u = 0.0;
+  double* cross_imp = nullptr;
+  double* cross_rowtab = nullptr;
+  int kan_ml = 0;
+  int32_t* kan_ptr = nullptr;
+  int32_t* kan_a = nullptr;
+  int32_t* kan_b = nullptr;
+  double* kan_c = nullptr;
+  int32_t* row_order = nullptr;  // processing order of up rows
+  int64_t table_bytes = 0;
+  std::vector<void*> allocs;
+  void release();
+};
+
+void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s);
+
+// H*v launch parameters.
+struct HvArgs {
+  const double* v;      // gathered operand (scaled by vscale on load)
+  double* w;            // output
+  const double* wprev;  // optional additive input (scaled by wscale), may alias w
+  double vscale;        // multiplies every gathered v value
+  double wscale;        // multiplies wprev
+  int mode;             // 0: w = Hv ; 1: w = Hv + wscale*wprev
+  double* dot_partial;  // optional: per-block partial sums of v*w (after scale)
+  const double* dscale = nullptr;  // optional device scalars {vscale, wscale} overriding the values above
+  const int* done = nullptr;       // optional device stop flag: the launch does nothing when set
+};
+
+void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant = 0);
+int hv_num_blocks(const DevSector& ds);
+
+}  // namespace edh

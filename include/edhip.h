@@ -1,0 +1,162 @@
+/*
+ * edhip -- C ABI of the MI355X-native spin-resolved exact-diagonalisation
+ * engine (libedhip.so).  Plain pointers and sizes only; every entry returns
+ * an int status (EDH_OK == 0, negative on error) and never lets a C++
+ * exception cross the boundary.  The message of the last failure is
+ * available from edh_last_error() (per calling thread).
+ *
+ * Each entry names the reference (Q-solvers/EDLib) interface it replaces.
+ * Memory kinds: EDH_MEM_HOST means the vector arguments are host pointers
+ * (staged through device buffers owned by the context), EDH_MEM_DEVICE means
+ * they are device pointers on the context's GPU and the call stays on the
+ * context's stream (asynchronous unless stated otherwise).
+ */
+#ifndef EDHIP_H
+#define EDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDH_OK 0
+#define EDH_ERR_ARG -1
+#define EDH_ERR_HIP -2
+#define EDH_ERR_STATE -3
+#define EDH_ERR_LIMIT -4
+#define EDH_ERR_NOMEM -5
+#define EDH_ERR_NOCONV -6
+#define EDH_ERR_INTERNAL -9
+
+#define EDH_MODEL_HUBBARD 0
+#define EDH_MODEL_SIAM 1
+
+#define EDH_MEM_HOST 0
+#define EDH_MEM_DEVICE 1
+
+#define EDH_COPY_H2D 0
+#define EDH_COPY_D2H 1
+#define EDH_COPY_D2D 2
+
+#define EDH_OP_SZ 0 /* SzOperator::action, ChiLoc.h:57-61 */
+#define EDH_OP_N 1  /* NOperator::action,  ChiLoc.h:85-89 */
+
+typedef struct edh_ctx edh_ctx;
+typedef struct edh_model edh_model;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* edh_last_error(void);
+int edh_version(void);
+
+/* ---- Models (host side; no GPU needed) -------------------------------- */
+
+/* HubbardModel<double>(params) constructor, HubbardModel.h:48-77.
+ * t: Ns*Ns hopping/values (H_hop = -sum t_ij c^+_j c_i); U, xmu, Hmag: Ns;
+ * J: Ns*Ns exchange (may be NULL); Hmag may be NULL. */
+int edh_model_hubbard(edh_model** out, int Ns, const double* t, const double* U, const double* J,
+                      const double* xmu, const double* Hmag);
+
+/* SingleImpurityAndersonModel<double>(params) constructor,
+ * SingleImpurityAndersonModel.h:195-275.  Epsk: (Ns-ml)*2 (Bath/Epsk),
+ * Vk: ml*(Ns-ml)*2 (Bath/Vk_m), H0: ml*ml*2 (H0_m), Ut: 2*2*ml^4
+ * (interaction/values), mu (mu). */
+int edh_model_siam(edh_model** out, int Ns, int ml, const double* Epsk, const double* Vk, const double* H0,
+                   const double* Ut, double mu);
+int edh_model_free(edh_model* m);
+int edh_model_sites(const edh_model* m, int* Ns, int* interacting_orbitals);
+
+/* T_states() (which=0) / V_states() (which=1) as (kind,i,j,k,l,s,sp) + value;
+ * pass NULL arrays to query *count.  HubbardModel.h:141-147. */
+int edh_model_transitions(const edh_model* m, int which, int* count, int* ints, double* vals);
+
+/* Model::diagonal(state) evaluated in the reference's term order,
+ * HubbardModel.h:116-132 / SingleImpurityAndersonModel.h:282-306. */
+int edh_model_diagonal(const edh_model* m, long long state, double* out);
+
+/* Host-side table introspection for one (nup, ndn) sector, no GPU needed:
+ * which = 0 up hopping, 1 down hopping (SpinResolvedStorage::fill_spin,
+ * SpinResolvedStorage.h:622-638).  rowp has nrows+1 entries; pass NULL
+ * arrays to query sizes. */
+int edh_sector_csr(const edh_model* m, int nup, int ndn, int which, int64_t* nrows, int64_t* nnz, int* rowp,
+                   int* col, double* val);
+/* Diagonal of every state of the sector as the device evaluates it
+ * (decomposed form), for checks against Model::diagonal. */
+int edh_sector_diagonal(const edh_model* m, int nup, int ndn, double* out);
+
+/* ---- Context ------------------------------------------------------------ */
+
+int edh_create(edh_ctx** out, int device);
+int edh_destroy(edh_ctx* ctx);
+
+/* Copies the model into the context (Hamiltonian(p) constructor, Hamiltonian.h:40-42). */
+int edh_set_model(edh_ctx* ctx, const edh_model* m);
+
+/* SpinResolvedStorage::reset + fill for sector (nup, ndn)
+ * (SpinResolvedStorage.h:120-155,172-250): builds the per-spin tables and
+ * uploads them.  *dim receives the sector dimension (ntot()). */
+int edh_set_sector(edh_ctx* ctx, int nup, int ndn, int64_t* dim);
+/* Per-spin dimensions and algorithmic table bytes of the current sector. */
+int edh_sector_info(edh_ctx* ctx, int64_t* dim_up, int64_t* dim_dn, int64_t* table_bytes);
+
+/* ---- Device memory helpers ------------------------------------------------ */
+int edh_alloc(edh_ctx* ctx, size_t bytes, void** dptr);
+int edh_free(edh_ctx* ctx, void* dptr);
+int edh_memcpy(edh_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int edh_synchronize(edh_ctx* ctx);
+
+/* ---- Hot path -------------------------------------------------------------- */
+
+/* SpinResolvedStorage::av(v, w, n, clear) (SpinResolvedStorage.h:58-115):
+ * w = H v + (clear ? 0 : w).  With EDH_MEM_HOST the call is synchronous. */
+int edh_av(edh_ctx* ctx, const double* v, double* w, int64_t n, int clear, int mem);
+
+/* Times `reps` back-to-back av launches on device vectors with HIP events on
+ * the context stream after `warmup` untimed launches; *avg_ms = mean time. */
+int edh_av_time(edh_ctx* ctx, const double* v, double* w, int clear, int warmup, int reps, double* avg_ms);
+
+/* SpinResolvedStorage::vv (SpinResolvedStorage.h:390-400): *out = <a, b>. */
+int edh_vv(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* out, int mem);
+
+/* Lanczos::lanczos (Lanczos.h:36-77) on the current sector from the
+ * normalised start vector v0 (length n): alpha[nl], beta[nl+1] (beta[0]
+ * unused), *nlanc = number of steps taken (breaks when |beta| < 1e-10).
+ * The whole recurrence stays on the device (fused H*v + dot, fused axpy +
+ * norm); alpha/beta are host arrays. */
+int edh_lanczos(edh_ctx* ctx, const double* v0, int64_t n, int nl, double* alpha, double* beta, int* nlanc,
+                int mem);
+/* Same, timing the recurrence only (v0 on the device); *avg_step_ms. */
+int edh_lanczos_time(edh_ctx* ctx, const double* v0, int nl, double* alpha, double* beta, int* nlanc,
+                     double* avg_step_ms);
+
+/* SpinResolvedStorage::a_adag (SpinResolvedStorage.h:293-381): applies c
+ * (annihilate=1) or c^+ to orbital orb_spin = site + spin*Ns of a vector in
+ * sector (nup, ndn); out has the size of the resulting sector (*out_n). */
+int edh_adag(edh_ctx* ctx, int nup, int ndn, int orb_spin, int annihilate, const double* in, double* out,
+             int64_t* out_n, int mem);
+
+/* ChiLoc::operation (ChiLoc.h:474-530): out[s] = (sum_o action_o(s)) in[s]
+ * over the orbitals orbs[0..norb) in sector (nup, ndn); op = EDH_OP_SZ/N. */
+int edh_local_op(edh_ctx* ctx, int nup, int ndn, int op, int norb, const int* orbs, const double* in,
+                 double* out, int mem);
+
+/* Lanczos::get_frac_point (Lanczos.h:201-224) at nz complex energies, on the
+ * device: out = ev*expb*det[1]/det[0] with the reference's overflow rescaling. */
+int edh_cf_eval(edh_ctx* ctx, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign,
+                double expb, const double* alpha, const double* beta, double* out_re, double* out_im);
+
+/* Storage::diag (Storage.h:43-146) replacement: lowest `nev` eigenpairs of
+ * the current sector by thick-restart Lanczos with full reorthogonalisation
+ * on the device (which="SA"); ncv Krylov vectors, convergence when the Ritz
+ * residual <= tol*max(|theta|, eps^(2/3)), at most maxitr restarts.  evals
+ * (host, nev) ascending; evecs (host, n*nev column-major) may be NULL.
+ * Special cases of Storage::diag: n == 1 -> (diag, [1]). */
+int edh_eigsh(edh_ctx* ctx, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals,
+              double* evecs, int* nconv, int* n_av);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDHIP_H */

@@ -1,0 +1,236 @@
+"""GPU parity of the HIP path against the CPU oracle (calls go through the C ABI).
+
+Tolerances: H*v and Lanczos coefficients are FP64 with a different summation
+order than the reference, so they are compared to 1e-12 relative to the
+vector's largest entry; integer work (a_adag index maps and signs) is
+bit-exact; eigenvalues to 1e-10 relative; G(iw) / chi to 1e-8 absolute
+(the reference's own test pins 1e-10 / 1e-9, which is also checked).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import edlib_amd as ea
+from oracle import edoracle as eo
+from tests import models
+from tests.fixtures import hubbard_arrays, load_dat, load_input
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = ea.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(1.0, np.max(np.abs(b)))
+
+
+_rng = np.random.default_rng(7)
+CASES = [
+    ("hubbard_random_J", "hubbard", models.random_hubbard_params(5, _rng)),
+    ("hubbard_ring8", "hubbard", dict(Ns=8, t=models.ring_hopping(8), U=np.full(8, 4.0), xmu=np.full(8, 2.0))),
+    ("hubbard_nonuniformU", "hubbard", models.random_hubbard_params(7, np.random.default_rng(1), with_J=False)),
+    ("siam_kanamori_3orb", "siam", models.siam_params(7, 3, np.random.default_rng(3), h0_offdiag=0.2)),
+    ("siam_1orb", "siam", models.siam_params(8, 1, np.random.default_rng(4))),
+]
+
+
+@pytest.mark.parametrize("name,kind,p", CASES, ids=[c[0] for c in CASES])
+def test_av_parity(ctx, name, kind, p):
+    om, pm = models.oracle_model(kind, p), models.product_model(kind, p)
+    ctx.set_model(pm)
+    Ns = p["Ns"]
+    rng = np.random.default_rng(42)
+    for sec in [(Ns // 2, Ns // 2), (Ns // 2 + 1, Ns // 2 - 1), (1, Ns - 1), (0, Ns), (Ns, Ns), (2, 1)]:
+        n = ctx.set_sector(*sec)
+        srs = eo.SRS(om, *sec)
+        assert n == srs.n
+        v = rng.uniform(-1, 1, n)
+        w0 = rng.uniform(-1, 1, n)
+        assert _rel(ctx.av(v), srs.av(v)) < 1e-12
+        assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False)) < 1e-12
+
+
+def test_av_parity_12site_torus(ctx):
+    # 3x4 torus with t, t' (the 853,776-dimensional half-filled sector)
+    p = dict(Ns=12, t=models.torus_hopping(3, 4, 1.0, -0.3), U=np.full(12, 8.0), xmu=np.full(12, 4.0))
+    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
+    ctx.set_model(pm)
+    n = ctx.set_sector(6, 6)
+    assert n == 853776
+    srs = eo.SRS(om, 6, 6)
+    rng = np.random.default_rng(20261202 + 2)
+    v = rng.uniform(-1, 1, n)
+    w0 = rng.uniform(-1, 1, n)
+    assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False, omp=True)) < 1e-12
+
+
+@pytest.mark.parametrize("name,kind,p", CASES[1:2] + CASES[3:4], ids=["hubbard_ring8", "siam_kanamori"])
+def test_lanczos_coefficients(ctx, name, kind, p):
+    om, pm = models.oracle_model(kind, p), models.product_model(kind, p)
+    ctx.set_model(pm)
+    Ns = p["Ns"]
+    sec = (Ns // 2, Ns // 2)
+    n = ctx.set_sector(*sec)
+    srs = eo.SRS(om, *sec)
+    v = np.random.default_rng(5).uniform(-1, 1, n)
+    v /= np.linalg.norm(v)
+    nl_o, a_o, b_o = srs.lanczos(v, 40)
+    nl_g, a_g, b_g = ctx.lanczos(v, 40)
+    assert nl_o == nl_g
+    k = min(20, nl_o)
+    assert np.max(np.abs(a_o[:k] - a_g[:k])) < 1e-10
+    assert np.max(np.abs(b_o[1:k] - b_g[1:k])) < 1e-10
+    # the continued fraction built from the full recurrence agrees in G
+    z = 1j * eo.matsubara_points(10.0, 64)
+    g_o = eo.frac_points(z, 1.0, nl_o, 1, 1.0, a_o, b_o)
+    g_g = eo.frac_points(z, 1.0, nl_g, 1, 1.0, a_g, b_g)
+    assert np.max(np.abs(g_o - g_g)) < 1e-8
+
+
+def test_lanczos_breakdown_small_sector(ctx):
+    p = dict(Ns=4, t=models.ring_hopping(4), U=np.full(4, 4.0), xmu=np.full(4, 2.0))
+    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
+    ctx.set_model(pm)
+    n = ctx.set_sector(1, 1)
+    srs = eo.SRS(om, 1, 1)
+    v = np.ones(n) / math.sqrt(n)
+    nl_o, a_o, b_o = srs.lanczos(v, 100)
+    nl_g, a_g, b_g = ctx.lanczos(v, 100)
+    assert nl_o == nl_g < 100  # |beta| < 1e-10 stops both at the same step
+    assert np.allclose(a_o[:nl_o], a_g[:nl_g], atol=1e-10)
+
+
+@pytest.mark.parametrize("name,kind,p", CASES[1:2] + CASES[3:4], ids=["hubbard_ring8", "siam_kanamori"])
+def test_adag_bit_exact(ctx, name, kind, p):
+    om, pm = models.oracle_model(kind, p), models.product_model(kind, p)
+    ctx.set_model(pm)
+    Ns = p["Ns"]
+    rng = np.random.default_rng(9)
+    for sec in [(Ns // 2, Ns // 2), (Ns // 2 - 1, Ns // 2 + 1), (Ns, 0), (0, 0)]:
+        n = ea.binom(Ns, sec[0]) * ea.binom(Ns, sec[1])
+        v = rng.uniform(-1, 1, n)
+        for orb in range(2 * Ns):
+            for ann in (False, True):
+                nxt_o, out_o = eo.a_adag(om, sec, orb, ann, v)
+                nxt_g, out_g = ctx.adag(sec, orb, ann, v)
+                assert nxt_o == nxt_g
+                if out_o is None:
+                    assert out_g is None
+                else:
+                    assert np.array_equal(out_o, out_g)
+
+
+def test_local_operator(ctx):
+    p = dict(Ns=6, t=models.ring_hopping(6), U=np.full(6, 4.0), xmu=np.full(6, 2.0))
+    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
+    ctx.set_model(pm)
+    sec = (3, 2)
+    n = ea.binom(6, 3) * ea.binom(6, 2)
+    v = np.random.default_rng(1).uniform(-1, 1, n)
+    nu, nd = eo.occupations(om, sec, 6)
+    for orbs in ([0], [2], [1, 4], [3, 3]):
+        sz = sum(0.5 * (nu[:, o].astype(float) - nd[:, o]) for o in orbs) * v
+        nn = sum((nu[:, o].astype(float) + nd[:, o]) for o in orbs) * v
+        assert np.array_equal(ctx.local_op(sec, ea.EDH_OP_SZ, orbs, v), sz)
+        assert np.array_equal(ctx.local_op(sec, ea.EDH_OP_N, orbs, v), nn)
+
+
+def test_continued_fraction(ctx):
+    rng = np.random.default_rng(3)
+    for nl in (1, 2, 3, 17, 150):
+        a = rng.normal(size=nl)
+        b = np.concatenate([[0.0], np.abs(rng.normal(size=nl))])
+        for isign in (1, -1):
+            z = 1j * eo.matsubara_points(20.0, 512) + rng.normal() * isign
+            ref = eo.frac_points(z, 0.7, nl, isign, 0.3, a, b)
+            got = ctx.cf_eval(z, 0.7, nl, isign, 0.3, a, b)
+            assert np.max(np.abs(ref - got)) < 1e-12
+    # overflow rescaling path: huge determinants
+    a = np.full(300, 1e3)
+    b = np.concatenate([[0.0], np.full(300, 1e-3)])
+    z = 1j * eo.matsubara_points(20.0, 16)
+    ref = eo.frac_points(z, 1.0, 300, 1, 1.0, a, b)
+    got = ctx.cf_eval(z, 1.0, 300, 1, 1.0, a, b)
+    assert np.max(np.abs(ref - got) / np.abs(ref)) < 1e-12
+
+
+def test_eigsh_matches_dense(ctx):
+    p = dict(Ns=8, t=models.ring_hopping(8), U=np.full(8, 4.0), xmu=np.full(8, 2.0))
+    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
+    ctx.set_model(pm)
+    for sec in [(4, 4), (3, 4), (2, 5)]:
+        n = ctx.set_sector(*sec)
+        srs = eo.SRS(om, *sec)
+        H = srs.dense()
+        ref = np.linalg.eigvalsh(0.5 * (H + H.T))[:4]
+        ev, X, nav = ctx.eigsh(4, 11)
+        assert len(ev) == 4
+        assert np.max(np.abs(ev - ref) / np.abs(ref)) < 1e-10
+        for i in range(4):
+            r = srs.av(X[:, i]) - ev[i] * X[:, i]
+            assert np.linalg.norm(r) < 1e-8
+        assert abs(np.linalg.norm(X[:, 0]) - 1.0) < 1e-12
+
+
+def test_eigsh_small_and_trivial_sectors(ctx):
+    p = dict(Ns=4, t=models.ring_hopping(4), U=np.full(4, 5.0), xmu=np.full(4, 2.5))
+    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
+    ctx.set_model(pm)
+    ctx.set_sector(0, 0)
+    ev, X, _ = ctx.eigsh(2)
+    assert ev.shape == (1,) and ev[0] == 0.0 and X[0, 0] == 1.0
+    n = ctx.set_sector(2, 2)
+    srs = eo.SRS(om, 2, 2)
+    ref = np.linalg.eigvalsh(srs.dense())
+    ev, X, _ = ctx.eigsh(100)  # ncv = ntot: full spectrum except the top state (Storage.h:53-54)
+    assert len(ev) == n - 1
+    assert np.allclose(ev, ref[:-1], rtol=0, atol=1e-12)
+
+
+def test_gf_chi_golden_via_device(ctx):
+    """GreensFunction / ChiLoc flow on the device primitives vs gom1/xiats/xiatd."""
+    d = load_input("GF_Chi_input.json")
+    Ns, t, U, xmu, H, J = hubbard_arrays(d)
+    pm = ea.HubbardModel(Ns, t, U, xmu, H, J)
+    ctx.set_model(pm)
+    beta, nomega, nlanc = 20.0, 200, 100
+    pairs = []
+    k = 0
+    for sec in eo.sector_order(Ns):
+        ctx.set_sector(*sec)
+        ev, X, _ = ctx.eigsh(100)
+        for i in range(len(ev)):
+            pairs.append((ev[i], X[:, i].copy(), k, sec))
+            k += 1
+    order = eo.eigenpair_order([p[0] for p in pairs])
+    pairs = [pairs[i] for i in order]
+    e0 = pairs[0][0]
+    Z = sum(math.exp(-(p[0] - e0) * beta) for p in pairs)
+    z = 1j * eo.matsubara_points(beta, nomega)
+    G = np.zeros((nomega, 2), complex)
+    for (E, vec, _, sec) in pairs:
+        bf = math.exp(-(E - e0) * beta)
+        if bf < 1e-12:
+            continue
+        expb = 0.0 if beta * (E - e0) > 25 else math.exp(-beta * (E - e0))
+        for spin in range(2):
+            for ann, isign in ((False, 1), (True, -1)):
+                nxt, out = ctx.adag(sec, 0 + spin * Ns, ann, vec)
+                if out is None:
+                    continue
+                norm = ctx.vv(out, out)
+                if not abs(norm) > 1e-10:
+                    continue
+                ctx.set_sector(*nxt)
+                nl, a, b = ctx.lanczos(out / math.sqrt(norm), nlanc)
+                G[:, spin] += ctx.cf_eval(z + E * isign, norm, nl, isign, expb, a, b)
+    G /= Z
+    ref = load_dat("gom1.dat")
+    for s in range(2):
+        assert np.max(np.abs(G[:, s] - (ref[:, 1] + 1j * ref[:, 2]))) < 1e-10
