@@ -47,6 +47,7 @@ _SIGS = {
     "edh_destroy": (ctypes.c_int, [_vp]),
     "edh_set_model": (ctypes.c_int, [_vp, _vp]),
     "edh_set_sector": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _i64p]),
+    "edh_constant_shift": (ctypes.c_int, [_vp, ctypes.c_double]),
     "edh_sector_info": (ctypes.c_int, [_vp, _i64p, _i64p, _i64p]),
     "edh_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "edh_free": (ctypes.c_int, [_vp, _vp]),

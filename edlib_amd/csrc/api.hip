@@ -257,6 +257,19 @@ int edh_set_sector(edh_ctx* ctx, int nup, int ndn, int64_t* dim) {
   API_END
 }
 
+int edh_constant_shift(edh_ctx* ctx, double shift) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  for (double& x : c.tabs.up.ediag) x += shift;
+  EDH_HIP(hipMemcpyAsync(c.dev.up.ediag, c.tabs.up.ediag.data(), c.tabs.up.ediag.size() * sizeof(double),
+                         hipMemcpyHostToDevice, c.stream));
+  EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
 int edh_sector_info(edh_ctx* ctx, int64_t* dup, int64_t* ddn, int64_t* tbytes) {
   API_BEGIN
   require_ctx(ctx);

@@ -1,0 +1,123 @@
+// Drop-in checks written like the reference's own tests, on the engine:
+//   gfchi   -- test/Lanczos_Test.cpp:29-121 (G vs gom1.dat, chi_Sz vs xiats.dat,
+//              chi_N vs xiatd.dat) with HIPSpinResolvedStorage in place of the
+//              CSR storage;
+//   4ring   -- test/HubbardModel_Test.cpp:27-71 (E0, ground sector, observables);
+//   4ring_nomag -- test/StaticObservables_Test.cpp:27-72.
+// Usage: edtest <case> <golden dir>.  Prints one JSON line; exit code 0 on pass.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "edlib_amd/GreensFunction.h"
+
+using namespace EDLib;
+
+static std::vector<std::array<double, 3>> read_dat(const std::string& f, int n) {
+  std::ifstream in(f);
+  std::vector<std::array<double, 3>> out;
+  for (int i = 0; i < n; ++i) {
+    std::array<double, 3> r;
+    in >> r[0] >> r[1] >> r[2];
+    out.push_back(r);
+  }
+  return out;
+}
+
+static int gfchi(const std::string& dir) {
+  params p;
+  define_parameters(p);
+  p["NSITES"] = 4;
+  p["NSPINS"] = 2;
+  p["INPUT_FILE"] = dir + "/GF_Chi_input.json";
+  p["arpack.SECTOR"] = false;
+  p["storage.MAX_DIM"] = 36;
+  p["storage.EIGENVALUES_ONLY"] = 0;
+  p["arpack.NEV"] = 100;
+  p["lanc.BETA"] = 20;
+  p["lanc.EMIN"] = -4.0;
+  p["lanc.EMAX"] = 4.0;
+  p["lanc.NOMEGA"] = 1000;
+  HIPHubbardHamiltonian ham(p);
+  ham.diag(false);
+  gf::MatsubaraMesh fmesh(p, gf::FERMIONIC), bmesh(p, gf::BOSONIC);
+  gf::GreensFunction<HIPHubbardHamiltonian> greens(p, ham, fmesh);
+  greens.compute();
+  StaticObservables<HIPHubbardHamiltonian> so(p);
+  auto obs = so.calculate_static_observables(ham);
+  gf::ChiLoc<HIPHubbardHamiltonian> susc(p, ham, bmesh);
+  double avg = 0.0;
+  for (double x : obs["M"]) avg += x / (2.0 * obs["M"].size());
+  susc.compute(EDH_OP_SZ, &avg);
+  gf::GFData chisz = susc.G();
+  avg = 0.0;
+  for (double x : obs["N"]) avg += x / double(obs["N"].size());
+  susc.compute(EDH_OP_N, &avg);
+  gf::GFData chin = susc.G();
+  auto g = read_dat(dir + "/gom1.dat", 200);
+  auto xs = read_dat(dir + "/xiats.dat", 200);
+  auto xd = read_dat(dir + "/xiatd.dat", 200);
+  double dg = 0, ds = 0, dn = 0;
+  for (int i = 0; i < 200; ++i) {
+    for (int s = 0; s < 2; ++s) dg = std::max(dg, std::abs(greens.G()(i, 0, s) - std::complex<double>(g[i][1], g[i][2])));
+    ds = std::max(ds, std::abs(chisz(i, 0) - std::complex<double>(-0.25 * xs[i][1], 0.0)));
+    dn = std::max(dn, std::abs(chin(i, 0) - std::complex<double>(-xd[i][1], 0.0)));
+  }
+  bool ok = dg <= 1e-10 && ds <= 1e-9 && dn <= 1e-9;
+  std::printf("{\"case\": \"gfchi\", \"max_dG\": %.3e, \"max_dChiSz\": %.3e, \"max_dChiN\": %.3e, \"npairs\": %zu, \"ok\": %s}\n",
+              dg, ds, dn, ham.eigenpairs().size(), ok ? "true" : "false");
+  return ok ? 0 : 1;
+}
+
+static int ring4(const std::string& dir, bool mag) {
+  params p;
+  define_parameters(p);
+  p["NSITES"] = 4;
+  p["NSPINS"] = 2;
+  p["INPUT_FILE"] = dir + (mag ? "/4ring_input.json" : "/4ring_nomagfield_input.json");
+  p["arpack.SECTOR"] = false;
+  p["storage.MAX_DIM"] = 36;
+  p["storage.EIGENVALUES_ONLY"] = 0;
+  p["arpack.NEV"] = 1;
+  HIPHubbardHamiltonian ham(p);
+  ham.diag(false);
+  const auto& gs = *ham.eigenpairs().begin();
+  StaticObservables<HIPHubbardHamiltonian> so(p);
+  auto obs = so.calculate_static_observables(ham);
+  bool ok = true;
+  if (mag) {
+    ok = std::abs(gs.eigenvalue() - (-11.8443)) < 1e-4 && gs.sector().nup() == 2 && gs.sector().ndown() == 2;
+    for (int o = 0; o < 4; ++o) ok = ok && std::abs(obs["N"][o] - 1.0) < 1e-8 && obs["N_up"][o] > obs["N_dn"][o] && obs["M"][o] > 0.0;
+  } else {
+    ok = obs["N"].size() == 4 && obs["M_i M_j"].size() == 16 && obs["N_eff"].size() == 1;
+    for (int o = 0; o < 4; ++o)
+      ok = ok && std::abs(obs["N"][o] - 1.0) < 1e-8 && std::abs(obs["N_up"][o] - 0.5) < 1e-8 &&
+           std::abs(obs["N_dn"][o] - 0.5) < 1e-8 && std::abs(obs["M"][o]) < 1e-7 &&
+           std::abs(obs["D_occ"][o] - obs["D_occ"][0]) < 1e-8;
+    ok = ok && obs["N_eff"][0] > 0.0;
+  }
+  std::printf("{\"case\": \"%s\", \"E0\": %.12f, \"nup\": %d, \"ndn\": %d, \"N0\": %.12f, \"M0\": %.3e, \"ok\": %s}\n",
+              mag ? "4ring" : "4ring_nomag", gs.eigenvalue(), gs.sector().nup(), gs.sector().ndown(), obs["N"][0],
+              obs["M"][0], ok ? "true" : "false");
+  return ok ? 0 : 1;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: edtest <gfchi|4ring|4ring_nomag> <golden dir>\n");
+    return 2;
+  }
+  std::string c = argv[1], dir = argv[2];
+  try {
+    if (c == "gfchi") return gfchi(dir);
+    if (c == "4ring") return ring4(dir, true);
+    if (c == "4ring_nomag") return ring4(dir, false);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 3;
+  }
+  return 2;
+}

@@ -97,6 +97,10 @@ int edh_set_model(edh_ctx* ctx, const edh_model* m);
  * (SpinResolvedStorage.h:120-155,172-250): builds the per-spin tables and
  * uploads them.  *dim receives the sector dimension (ntot()). */
 int edh_set_sector(edh_ctx* ctx, int nup, int ndn, int64_t* dim);
+/* SpinResolvedStorage::constant_shift (SpinResolvedStorage.h:464-466): adds
+ * `shift` to every diagonal element of the current sector (until the next
+ * edh_set_sector, as the reference's refill discards it). */
+int edh_constant_shift(edh_ctx* ctx, double shift);
 /* Per-spin dimensions and algorithmic table bytes of the current sector. */
 int edh_sector_info(edh_ctx* ctx, int64_t* dim_up, int64_t* dim_dn, int64_t* table_bytes);
 

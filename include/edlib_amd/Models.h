@@ -1,0 +1,155 @@
+// Models with the reference's constructors and accessors, backed by the
+// engine's model objects (edh_model_*): HubbardModel.h:41-222,
+// SingleImpurityAndersonModel.h:185-417, FermionicModel.h:21-119.
+#ifndef EDLIB_AMD_MODELS_H
+#define EDLIB_AMD_MODELS_H
+
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../edhip.h"
+#include "Params.h"
+#include "Symmetry.h"
+
+namespace EDLib {
+
+inline void edh_throw(int code) {
+  if (code != EDH_OK) throw std::runtime_error(std::string("edhip: ") + edh_last_error());
+}
+
+namespace Model {
+
+// FermionicModel.h:21-119 (bit encoding and signs; Dyson/bare GF out of scope).
+class FermionicModel {
+ public:
+  FermionicModel(params& p) : Ns_(p["NSITES"].as<int>()), ms_(p["NSPINS"].as<int>()), Ip_(Ns_ * ms_) {}
+  int checkState(long long nst, int im, int Ip) const { return (int)((nst >> (Ip - 1 - im)) & 1ll); }
+  void a(int i, long long jold, long long& k, int& isign) const {
+    isign = (__builtin_popcountll((unsigned long long)jold >> (Ip_ - i)) & 1) ? -1 : 1;
+    k = jold - (1ll << (Ip_ - i - 1));
+  }
+  void adag(int i, long long jold, long long& k, int& isign) const {
+    isign = (__builtin_popcountll((unsigned long long)jold >> (Ip_ - i)) & 1) ? -1 : 1;
+    k = jold + (1ll << (Ip_ - i - 1));
+  }
+  int orbitals() const { return Ns_; }
+  int max_total_electrons() const { return Ip_; }
+  int spins() const { return ms_; }
+
+ protected:
+  int Ns_, ms_, Ip_;
+};
+
+// Holds the engine's model object.
+class EngineModel {
+ public:
+  const edh_model* engine_model() const { return m_.get(); }
+
+ protected:
+  struct Free {
+    void operator()(edh_model* m) const { edh_model_free(m); }
+  };
+  std::unique_ptr<edh_model, Free> m_;
+  void adopt(edh_model* m) { m_.reset(m); }
+};
+
+template <typename prcsn>
+class HubbardModel : public FermionicModel, public EngineModel {
+ public:
+  typedef prcsn precision;
+  typedef Symmetry::SzSymmetry SYMMETRY;
+  typedef Symmetry::SzSymmetry::Sector Sector;
+
+  // Reads hopping/values, interaction/values, chemical_potential/values and
+  // the optional magnetic_field/values, exchange/values (HubbardModel.h:55-66).
+  HubbardModel(params& p) : FermionicModel(p), symmetry_(p) {
+    alps_shim::input_file in(p["INPUT_FILE"].as<std::string>());
+    std::vector<double> t = vec(in, "hopping/values", Ns_ * Ns_), U = vec(in, "interaction/values", Ns_),
+                        xmu = vec(in, "chemical_potential/values", Ns_);
+    std::vector<double> H(Ns_, 0.0), J(Ns_ * Ns_, 0.0);
+    if (in.is_data("magnetic_field/values")) H = vec(in, "magnetic_field/values", Ns_);
+    if (in.is_data("exchange/values")) J = vec(in, "exchange/values", Ns_ * Ns_);
+    edh_model* m = nullptr;
+    edh_throw(edh_model_hubbard(&m, Ns_, t.data(), U.data(), J.data(), xmu.data(), H.data()));
+    adopt(m);
+  }
+  precision diagonal(long long state) const {
+    double x = 0;
+    edh_throw(edh_model_diagonal(m_.get(), state, &x));
+    return (precision)x;
+  }
+  int interacting_orbitals() const { return Ns_; }
+  Symmetry::SzSymmetry& symmetry() { return symmetry_; }
+  const Symmetry::SzSymmetry& symmetry() const { return symmetry_; }
+
+ private:
+  static std::vector<double> vec(const alps_shim::input_file& in, const std::string& k, size_t n) {
+    const alps_shim::dataset& d = in.get(k);
+    if (d.data.size() != n) throw std::invalid_argument("input dataset " + k + " has the wrong size");
+    return d.data;
+  }
+  Symmetry::SzSymmetry symmetry_;
+};
+
+template <typename prcsn>
+class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
+ public:
+  typedef prcsn precision;
+  typedef Symmetry::SzSymmetry SYMMETRY;
+  typedef Symmetry::SzSymmetry::Sector Sector;
+
+  // Reads Bath/Epsk, Bath/Vk_m, H0_m, mu and interaction/values, with the
+  // reference's argument checks (SingleImpurityAndersonModel.h:195-230).
+  SingleImpurityAndersonModel(params& p) : FermionicModel(p), symmetry_(p), ml_(p["siam.NORBITALS"].as<int>()) {
+    if (ml_ > Ns_)
+      throw std::invalid_argument(
+          "Incorrect values for the total number of sites and the number of orbitals. Please check input file.");
+    if (ms_ != 2) throw std::invalid_argument("Incorrect values for the number of spins. Please check input file.");
+    alps_shim::input_file in(p["INPUT_FILE"].as<std::string>());
+    const int nb = Ns_ - ml_;
+    const alps_shim::dataset& e = in.get("Bath/Epsk/values");
+    if ((int)(e.data.size() / 2) != nb)
+      throw std::invalid_argument(
+          "Total number of state does not equal to sum of the total number of bath levels and the number of impurity "
+          "orbitals");
+    std::vector<double> Vk(ml_ * nb * 2, 0.0), H0(ml_ * ml_ * 2, 0.0);
+    for (int im = 0; im < ml_; ++im) {
+      std::ostringstream s1, s2;
+      s1 << "Bath/Vk_" << im << "/values";
+      s2 << "H0_" << im << "/values";
+      const alps_shim::dataset& v = in.get(s1.str());
+      const alps_shim::dataset& h = in.get(s2.str());
+      if ((int)(h.data.size() / 2) > ml_)
+        throw std::invalid_argument("Inter orbital hoppings array dimension are bigger than number of impurity orbitals");
+      for (size_t q = 0; q < v.data.size() && q < (size_t)nb * 2; ++q) Vk[im * nb * 2 + q] = v.data[q];
+      for (size_t q = 0; q < h.data.size(); ++q) H0[im * ml_ * 2 + q] = h.data[q];
+    }
+    const alps_shim::dataset& U = in.get("interaction/values");
+    if (U.shape.size() != 6 || (int)U.shape[2] != ml_)
+      throw std::invalid_argument("Incorrect number of orbitals. Please check input file.");
+    const double mu = in.get("mu").data.at(0);
+    edh_model* m = nullptr;
+    edh_throw(edh_model_siam(&m, Ns_, ml_, e.data.data(), Vk.data(), H0.data(), U.data.data(), mu));
+    adopt(m);
+  }
+  precision diagonal(long long state) const {
+    double x = 0;
+    edh_throw(edh_model_diagonal(m_.get(), state, &x));
+    return (precision)x;
+  }
+  int interacting_orbitals() const { return ml_; }
+  Symmetry::SzSymmetry& symmetry() { return symmetry_; }
+  const Symmetry::SzSymmetry& symmetry() const { return symmetry_; }
+
+ private:
+  Symmetry::SzSymmetry symmetry_;
+  int ml_;
+};
+
+}  // namespace Model
+}  // namespace EDLib
+
+#endif
