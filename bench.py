@@ -129,7 +129,7 @@ def main():
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
-                     "kernel": "hv_kernel<CROSS_POPC,false,true>", "kernel_ms": round(kern_ms, 4)},
+                     "kernel": "edh::hv_rows_kernel (J=16 columns/thread, popcount cross term, row in LDS)", "kernel_ms": round(kern_ms, 4)},
     }
     if not args.no_cpu_baseline:
         wcpu = w.copy()

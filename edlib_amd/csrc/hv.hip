@@ -8,9 +8,12 @@
 //           + sum_V c * v[u'][d']                   impurity interaction, on the fly
 // diag(u,d) = Eup[u] + Edn[d] + X(bits_u, bits_d), X from the cross table mode.
 //
-// One work-group processes R up-rows per pass; each row is staged in LDS
-// (scaled by vscale) so the down-spin gathers hit LDS.  The down-spin table is
-// a sliced ELL with slice height 64, so a wavefront's entry loads coalesce.
+// One work-group processes one up-row at a time, in the processing order
+// row_order (reverse Cuthill-McKee of the up-spin hopping graph, so that the
+// neighbour rows read by the up-spin term are reused from the Infinity Cache).
+// The row is staged in LDS (scaled by vscale) so the down-spin gathers hit
+// LDS; the down-spin table is a sliced ELL with slice height 64, so a
+// wavefront's entry loads coalesce.
 #include "edh_internal.h"
 
 namespace edh {
@@ -18,9 +21,6 @@ namespace edh {
 struct HvParams {
   int Ns, nb;  // sites per spin, bath bits (Ns - ml) below the impurity bits
   int Dup, Ddn;
-  int R;    // rows per pass
-  int tpr;  // threads per row
-  int ngroups;
   const int32_t* row_order;
   // up spin
   const int32_t* up_bits;
@@ -75,83 +75,129 @@ __device__ __forceinline__ double cross_term(const HvParams& p, int u, int bu, i
   return 0.0;
 }
 
-template <int CROSS, bool KAN, bool LDS>
-__global__ void hv_kernel(HvParams p) {
+// One block processes one up-row at a time (grid-stride over the processing
+// order).  Each thread owns J columns d = c0 + tid + j*T of the row and keeps
+// their accumulators in registers, so every neighbour row of the up-spin
+// term issues J independent coalesced loads per thread.
+template <int J, int CROSS, bool KAN, bool LDS>
+__global__ void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
   __shared__ double s_red[16];
+  const int T = blockDim.x;
   const int tid = threadIdx.x;
-  const int g = tid / p.tpr;
-  const int lt = tid - g * p.tpr;
   const int Ddn = p.Ddn;
   const int na = 1 << (p.Ns - p.nb);
   const unsigned lowmask = (1u << p.nb) - 1u;
   if (p.done && *p.done) return;
   const double vscale = p.dscale ? p.dscale[0] : p.vscale;
   const double wscale = p.dscale ? p.dscale[1] : p.wscale;
+  const double* __restrict__ V = p.v;
+  const double* __restrict__ vals = p.values;
   double dsum = 0.0;
 
-  for (int grp = blockIdx.x; grp < p.ngroups; grp += gridDim.x) {
-    const int slot = grp * p.R + g;
-    const bool active = (g < p.R) && (slot < p.Dup);
-    // Row index is uniform per row group; readfirstlane lets the compiler use scalar loads.
-    int u = active ? p.row_order[slot] : 0;
-    u = __builtin_amdgcn_readfirstlane(u);
+  for (int slot = blockIdx.x; slot < p.Dup; slot += gridDim.x) {
+    const int u = __builtin_amdgcn_readfirstlane(p.row_order[slot]);
     const size_t rowoff = (size_t)u * Ddn;
-    double* sv = s_v + (size_t)g * Ddn;
     if (LDS) {
-      if (active)
-        for (int d = lt; d < Ddn; d += p.tpr) sv[d] = p.v[rowoff + d] * vscale;
+      for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
       __syncthreads();
     }
-    if (active) {
-      const int bu = p.up_bits[u];
-      const double eu = p.up_e[u];
-      const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
-      for (int d = lt; d < Ddn; d += p.tpr) {
-        const int bd = p.dn_bits[d];
-        const double vd = LDS ? sv[d] : p.v[rowoff + d] * vscale;
-        const double dg = eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd, na);
-        double acc = dg * vd + (p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0);
-        // down-spin hopping (sliced ELL, gathers from the staged row)
-        const int sl = d >> 6, lane = d & 63;
-        const int len = p.dn_sllen[sl];
-        const uint32_t* ep = p.dn_ell + p.dn_slptr[sl] + lane;
-        for (int k = 0; k < len; ++k) {
-          const uint32_t e = ep[k * kSlice];
-          const int c = (int)(e & kColMask);
-          const double x = LDS ? sv[c] : p.v[rowoff + c] * vscale;
-          acc = fma(p.values[e >> kColBits], x, acc);
+    const int bu = p.up_bits[u];
+    const double eu = p.up_e[u];
+    const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+    for (int c0 = 0; c0 < Ddn; c0 += T * J) {
+      double acc[J], vv[J];
+      int bd[J];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = c0 + tid + j * T;
+        acc[j] = 0.0;
+        vv[j] = 0.0;
+        bd[j] = 0;
+        if (d < Ddn) {
+          vv[j] = LDS ? s_v[d] : V[rowoff + d] * vscale;
+          if (p.accumulate) acc[j] = wscale * p.wprev[rowoff + d];
+          bd[j] = p.dn_bits[d];
         }
-        // up-spin hopping: linear combination of whole rows, coalesced over d
-        for (int q = hb; q < he; ++q) {
-          const uint32_t e = p.up_packed[q];
-          const size_t src = (size_t)(e & kColMask) * Ddn + d;
-          acc = fma(p.values[e >> kColBits], p.v[src] * vscale, acc);
+      }
+      // diagonal, generated from the bit patterns
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = c0 + tid + j * T;
+        if (d < Ddn) acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd[j], na), vv[j], acc[j]);
+      }
+      // down-spin hopping: sliced ELL, gathers from the staged row
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = c0 + tid + j * T;
+        if (d < Ddn) {
+          const int sl = d >> 6;
+          const int len = p.dn_sllen[sl];
+          const uint32_t* __restrict__ ep = p.dn_ell + p.dn_slptr[sl] + (d & 63);
+          double a0 = 0.0, a1 = 0.0;
+          int k = 0;
+          for (; k + 1 < len; k += 2) {
+            const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
+            const int c0_ = (int)(e0 & kColMask), c1_ = (int)(e1 & kColMask);
+            const double x0 = LDS ? s_v[c0_] : V[rowoff + c0_] * vscale;
+            const double x1 = LDS ? s_v[c1_] : V[rowoff + c1_] * vscale;
+            a0 = fma(vals[e0 >> kColBits], x0, a0);
+            a1 = fma(vals[e1 >> kColBits], x1, a1);
+          }
+          if (k < len) {
+            const uint32_t e0 = ep[k * kSlice];
+            const int c0_ = (int)(e0 & kColMask);
+            a0 = fma(vals[e0 >> kColBits], LDS ? s_v[c0_] : V[rowoff + c0_] * vscale, a0);
+          }
+          acc[j] += a0 + a1;
         }
-        if (KAN) {
-          const int a = bu >> p.nb, b = bd >> p.nb;
-          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-          for (int q = kb; q < ke; ++q) {
-            const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
-            const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
-            acc = fma(p.kan_c[q], p.v[(size_t)up2 * Ddn + dn2] * vscale, acc);
+      }
+      // up-spin hopping: linear combination of whole rows (coalesced)
+      for (int q = hb; q < he; ++q) {
+        const uint32_t e = p.up_packed[q];
+        const double h = vals[e >> kColBits] * vscale;
+        const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn + c0 + tid;
+        double x[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) x[j] = (c0 + tid + j * T < Ddn) ? src[j * T] : 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
+      }
+      if (KAN) {
+        const int a = bu >> p.nb;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int d = c0 + tid + j * T;
+          if (d < Ddn) {
+            const int b = bd[j] >> p.nb;
+            const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+            for (int q = kb; q < ke; ++q) {
+              const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
+              const int dn2 = p.dn_rank[(bd[j] & lowmask) | (p.kan_b[q] << p.nb)];
+              acc[j] = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc[j]);
+            }
           }
         }
-        p.w[rowoff + d] = acc;
-        dsum = fma(vd, acc, dsum);
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = c0 + tid + j * T;
+        if (d < Ddn) {
+          p.w[rowoff + d] = acc[j];
+          dsum = fma(vv[j], acc[j], dsum);
+        }
       }
     }
     if (LDS) __syncthreads();
   }
   if (p.dot_partial) {
-    // block reduction: wavefront shuffle, then across waves through LDS
     for (int off = 32; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, 64);
     const int wid = tid >> 6;
     if ((tid & 63) == 0) s_red[wid] = dsum;
     __syncthreads();
     if (tid == 0) {
       double t = 0.0;
-      for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += s_red[q];
+      for (int q = 0; q < (T >> 6); ++q) t += s_red[q];
       p.dot_partial[blockIdx.x] = t;
     }
   }
@@ -175,44 +221,58 @@ size_t lds_budget() {
 }
 
 struct Launch {
-  int R, tpr, block, grid, ngroups;
+  int J, block, grid;
   size_t smem;
   bool lds;
 };
 
+// Columns per thread J and threads per block T: T = 64 * ceil(Ddn / (64 J))
+// with J the smallest of {1,2,4,8,16} that keeps T <= 1024 and near 256
+// threads for short rows.
 Launch plan(const DevSector& ds) {
   Launch L;
-  const size_t rowb = (size_t)ds.dn.D * sizeof(double);
+  const int64_t Ddn = ds.dn.D;
+  const size_t rowb = (size_t)Ddn * sizeof(double);
   L.lds = rowb <= lds_budget();
-  int R = 1;
-  if (L.lds) {
-    while (R < 8 && (size_t)(R * 2) * rowb <= 64 * 1024 && ds.dn.D * R * 2 <= 8 * 512) R *= 2;
-  }
-  L.R = R;
-  L.block = (R == 1 && ds.dn.D >= 4096) ? 1024 : 512;
-  L.tpr = L.block / R;
-  L.ngroups = (int)((ds.up.D + R - 1) / R);
-  L.grid = L.ngroups;
-  L.smem = L.lds ? (size_t)R * rowb : 0;
+  int J = 1;
+  while (J < 16 && (Ddn + J - 1) / J > 256) J *= 2;
+  int64_t T = ((Ddn + J - 1) / J + 63) / 64 * 64;
+  if (T > 1024) T = 1024;
+  if (T < 64) T = 64;
+  L.J = J;
+  L.block = (int)T;
+  L.grid = (int)std::min<int64_t>(ds.up.D, 1 << 20);
+  L.smem = L.lds ? rowb : 0;
   return L;
 }
 
-template <int CROSS, bool KAN, bool LDS>
+template <int J, int CROSS, bool KAN, bool LDS>
 void launch_t(const HvParams& p, const Launch& L, hipStream_t s) {
-  auto k = hv_kernel<CROSS, KAN, LDS>;
-  if (L.smem > 64 * 1024) EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.smem));
+  auto k = hv_rows_kernel<J, CROSS, KAN, LDS>;
+  if (L.smem > 64 * 1024)
+    EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.smem));
   hipLaunchKernelGGL(k, dim3(L.grid), dim3(L.block), L.smem, s, p);
   EDH_HIP(hipGetLastError());
 }
 
-template <int CROSS>
+template <int J, int CROSS>
 void launch_c(const HvParams& p, const Launch& L, bool kan, hipStream_t s) {
   if (kan) {
-    if (L.lds) launch_t<CROSS, true, true>(p, L, s);
-    else launch_t<CROSS, true, false>(p, L, s);
+    if (L.lds) launch_t<J, CROSS, true, true>(p, L, s);
+    else launch_t<J, CROSS, true, false>(p, L, s);
   } else {
-    if (L.lds) launch_t<CROSS, false, true>(p, L, s);
-    else launch_t<CROSS, false, false>(p, L, s);
+    if (L.lds) launch_t<J, CROSS, false, true>(p, L, s);
+    else launch_t<J, CROSS, false, false>(p, L, s);
+  }
+}
+
+template <int J>
+void launch_j(const HvParams& p, const Launch& L, int cross, bool kan, hipStream_t s) {
+  switch (cross) {
+    case CROSS_NONE: launch_c<J, CROSS_NONE>(p, L, kan, s); break;
+    case CROSS_POPC: launch_c<J, CROSS_POPC>(p, L, kan, s); break;
+    case CROSS_IMP: launch_c<J, CROSS_IMP>(p, L, kan, s); break;
+    default: launch_c<J, CROSS_GENERAL>(p, L, kan, s); break;
   }
 }
 
@@ -229,9 +289,6 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
   p.nb = ds.Ns - ds.imp_ml;
   p.Dup = (int)ds.up.D;
   p.Ddn = (int)ds.dn.D;
-  p.R = L.R;
-  p.tpr = L.tpr;
-  p.ngroups = L.ngroups;
   p.row_order = ds.row_order;
   p.up_bits = ds.up.bits;
   p.up_e = ds.up.ediag;
@@ -263,11 +320,12 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
   p.dscale = a.dscale;
   p.done = a.done;
   const bool kan = ds.kan_ml > 0;
-  switch (ds.cross_mode) {
-    case CROSS_NONE: launch_c<CROSS_NONE>(p, L, kan, s); break;
-    case CROSS_POPC: launch_c<CROSS_POPC>(p, L, kan, s); break;
-    case CROSS_IMP: launch_c<CROSS_IMP>(p, L, kan, s); break;
-    default: launch_c<CROSS_GENERAL>(p, L, kan, s); break;
+  switch (L.J) {
+    case 1: launch_j<1>(p, L, ds.cross_mode, kan, s); break;
+    case 2: launch_j<2>(p, L, ds.cross_mode, kan, s); break;
+    case 4: launch_j<4>(p, L, ds.cross_mode, kan, s); break;
+    case 8: launch_j<8>(p, L, ds.cross_mode, kan, s); break;
+    default: launch_j<16>(p, L, ds.cross_mode, kan, s); break;
   }
 }
 

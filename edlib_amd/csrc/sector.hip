@@ -1,4 +1,6 @@
 // Upload of one sector's tables to the device.
+#include <algorithm>
+
 #include "edh_internal.h"
 
 namespace edh {
@@ -26,6 +28,40 @@ void upload_spin(DevSector& ds, const SpinTables& sp, DevSpin& d, hipStream_t s)
   d.sl_len = upload(ds, sp.sl_len, s);
   d.ell = upload(ds, sp.ell, s);
   d.rank = upload(ds, sp.rank, s);
+}
+
+// Reverse Cuthill-McKee order of the graph whose adjacency is the CSR
+// structure (rows and columns are basis states of one spin): breadth-first
+// from a minimum-degree vertex of each component, neighbours by increasing
+// degree, then reversed.
+std::vector<int32_t> rcm_order(const std::vector<int32_t>& rowp, const std::vector<int32_t>& col, int64_t n) {
+  std::vector<int32_t> deg(n), order;
+  order.reserve(n);
+  for (int64_t i = 0; i < n; ++i) deg[i] = rowp[i + 1] - rowp[i];
+  std::vector<char> seen(n, 0);
+  std::vector<int32_t> byDeg(n);
+  for (int64_t i = 0; i < n; ++i) byDeg[i] = (int32_t)i;
+  std::stable_sort(byDeg.begin(), byDeg.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+  std::vector<int32_t> nb;
+  for (int32_t start : byDeg) {
+    if (seen[start]) continue;
+    size_t head = order.size();
+    order.push_back(start);
+    seen[start] = 1;
+    while (head < order.size()) {
+      const int32_t x = order[head++];
+      nb.clear();
+      for (int q = rowp[x]; q < rowp[x + 1]; ++q)
+        if (!seen[col[q]]) {
+          seen[col[q]] = 1;
+          nb.push_back(col[q]);
+        }
+      std::stable_sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
 }
 
 }  // namespace
@@ -65,8 +101,7 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s)
     ds.kan_b = upload(ds, b, s);
     ds.kan_c = upload(ds, c, s);
   }
-  std::vector<int32_t> order(st.up.D);
-  for (int64_t u = 0; u < st.up.D; ++u) order[u] = (int32_t)u;
+  std::vector<int32_t> order = rcm_order(st.up.rowp, st.up.col, st.up.D);
   ds.row_order = upload(ds, order, s);
   EDH_HIP(hipStreamSynchronize(s));
 }

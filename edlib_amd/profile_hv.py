@@ -1,0 +1,67 @@
+"""Profiling workload for rocprofv3 (kernel trace and HBM counters).
+
+Runs `--reps` Lanczos-form H*v launches on the 16-site 4x4 Hubbard sector
+(8,8) and one device-to-device copy of `--copy-mib` MiB (the blit kernel
+whose FETCH_SIZE / WRITE_SIZE calibrate the counters against a known byte
+count).  Usage: python -m edlib_amd.profile_hv [--reps N] [--copy-mib M]
+"""
+import argparse
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+def torus(L1, L2, t=1.0):
+    Ns = L1 * L2
+    T = np.zeros((Ns, Ns))
+    for r in range(L1):
+        for c in range(L2):
+            i = r * L2 + c
+            for j in (((r + 1) % L1) * L2 + c, r * L2 + (c + 1) % L2):
+                T[i, j] = T[j, i] = t
+    return T
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--sites", type=int, default=16, choices=(12, 16))
+    a = ap.parse_args()
+    import edlib_amd as ea
+    from edlib_amd import _lib
+    if a.sites == 16:
+        Ns, t, U = 16, torus(4, 4), 6.0
+    else:
+        Ns, t, U = 12, torus(3, 4), 8.0
+        for r in range(3):
+            for c in range(4):
+                i = r * 4 + c
+                for (dr, dc) in ((1, 1), (1, -1)):
+                    j = ((r + dr) % 3) * 4 + (c + dc) % 4
+                    t[i, j] = t[j, i] = -0.3
+    ctx = ea.Context(0)
+    ctx.set_model(ea.HubbardModel(Ns, t, np.full(Ns, U), np.full(Ns, U / 2)))
+    n = ctx.set_sector(Ns // 2, Ns // 2)
+    rng = np.random.default_rng(20261202 + 8)
+    vb, wb = ctx.buffer(n), ctx.buffer(n)
+    vb.upload(rng.uniform(-1, 1, n))
+    wb.upload(rng.uniform(-1, 1, n))
+    for _ in range(a.reps):
+        ctx.av_device(vb, wb, clear=False)
+    ctx.synchronize()
+    nbytes = a.copy_mib << 20
+    src, dst = ctx.buffer(nbytes // 8), ctx.buffer(nbytes // 8)
+    _lib.check(_lib.lib().edh_memcpy(ctx.handle, dst.ptr, src.ptr, nbytes, 2))
+    ctx.synchronize()
+    _, _, tb = ctx.sector_info()
+    print("profile_hv: sites=%d n=%d reps=%d algorithmic_bytes=%d copy_bytes=%d" % (Ns, n, a.reps, 24 * n + tb, nbytes),
+          flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
