@@ -107,6 +107,12 @@ def main():
     ctx.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     kern_ms = ctx.av_time(vb, wb, clear=False, warmup=1, reps=max(args.steps, 5))
+    chosen = ctx.hv_variant
+    per_variant = {}
+    for var, name in ((0, "single_pass_ms"), (1, "two_pass_ms")):
+        ctx.hv_variant = var
+        per_variant[name] = round(ctx.av_time(vb, wb, clear=False, warmup=1, reps=5), 4)
+    ctx.hv_variant = chosen
     achieved = B / (kern_ms * 1e-3) / 1e9
     value = B / wall / 1e9
 
@@ -129,7 +135,9 @@ def main():
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
-                     "kernel": "edh::hv_rows_kernel (J=16 columns/thread, popcount cross term, row in LDS)", "kernel_ms": round(kern_ms, 4)},
+                     "kernel": ("edh::hv_rows_kernel (single pass)" if chosen == 0 else
+                                "edh::hv_rows_kernel (no up term) + edh::hv_upslab_kernel (two passes)"),
+                     "kernel_ms": round(kern_ms, 4), "variants": per_variant},
     }
     if not args.no_cpu_baseline:
         wcpu = w.copy()

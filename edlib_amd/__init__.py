@@ -189,6 +189,16 @@ class Context:
         check(_lib.lib().edh_sector_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    @property
+    def hv_variant(self):
+        v = ctypes.c_int()
+        check(_lib.lib().edh_get_hv_variant(self._h, ctypes.byref(v)))
+        return v.value
+
+    @hv_variant.setter
+    def hv_variant(self, variant):
+        check(_lib.lib().edh_set_hv_variant(self._h, int(variant)))
+
     def buffer(self, n):
         return DeviceBuffer(self, n)
 

@@ -349,6 +349,26 @@ int edh_av(edh_ctx* ctx, const double* v, double* w, int64_t n, int clear, int m
   API_END
 }
 
+int edh_get_hv_variant(edh_ctx* ctx, int* variant) {
+  API_BEGIN
+  require_ctx(ctx);
+  require_sector(ctx->c);
+  EDH_CHECK(variant, EDH_ERR_ARG, "null argument");
+  *variant = ctx->c.dev.variant;
+  API_END
+}
+
+int edh_set_hv_variant(edh_ctx* ctx, int variant) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(variant >= -1 && variant <= 1, EDH_ERR_ARG, "variant must be -1, 0 or 1");
+  c.dev.variant = variant < 0 ? hv_autotune(c.dev, c.stream) : variant;
+  API_END
+}
+
 int edh_av_time(edh_ctx* ctx, const double* v, double* w, int clear, int warmup, int reps, double* avg_ms) {
   API_BEGIN
   require_ctx(ctx);

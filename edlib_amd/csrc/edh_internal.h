@@ -110,6 +110,10 @@ struct SpinTables {
   std::vector<uint32_t> packed; // packed CSR entries
   std::vector<int32_t> sl_ptr, sl_len;  // sliced ELL
   std::vector<uint32_t> ell;
+  // Compact ELL (same slices): column in the low 14 bits, index into vt16 in
+  // the high 2, when the spin has <= 16384 states and <= 4 distinct values.
+  std::vector<uint16_t> ell16;
+  double vt16[4] = {0, 0, 0, 0};
   std::vector<int32_t> rank;    // rank by bit pattern (2^Ns), -1 if not in the basis
 };
 
@@ -137,6 +141,11 @@ struct SectorTables {
 
 void build_sector(const Model& m, int nup, int ndn, SectorTables& st, bool need_rank);
 
+// H*v variants: single pass (rows, up-spin term from the cache hierarchy) or
+// two passes (rows without the up-spin term, then XCD-aware column slabs for
+// it).  HV_AUTO picks the one timed fastest for the sector.
+enum HvVariant { HV_SINGLE = 0, HV_TWO_PASS = 1, HV_AUTO = -1 };
+
 // Device mirror of one sector.
 struct DevSpin {
   int64_t D = 0;
@@ -147,6 +156,8 @@ struct DevSpin {
   int32_t* sl_ptr = nullptr;
   int32_t* sl_len = nullptr;
   uint32_t* ell = nullptr;
+  uint16_t* ell16 = nullptr;
+  double vt16[4] = {0, 0, 0, 0};
   int32_t* rank = nullptr;
 };
 
@@ -166,6 +177,7 @@ struct DevSector {
   int32_t* kan_b = nullptr;
   double* kan_c = nullptr;
   int32_t* row_order = nullptr;  // processing order of up rows
+  int variant = HV_SINGLE;       // chosen H*v variant
   int64_t table_bytes = 0;
   std::vector<void*> allocs;
   void release();
@@ -186,7 +198,8 @@ struct HvArgs {
   const int* done = nullptr;       // optional device stop flag: the launch does nothing when set
 };
 
-void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant = 0);
+void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant = HV_AUTO);
 int hv_num_blocks(const DevSector& ds);
+int hv_autotune(DevSector& ds, hipStream_t s);
 
 }  // namespace edh

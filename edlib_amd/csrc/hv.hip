@@ -2,18 +2,27 @@
 //
 // Replaces SpinResolvedStorage::av (SpinResolvedStorage.h:58-115).  The state
 // vector is the D_up x D_dn row-major matrix V[u][d] (down index contiguous).
-//   w[u][d] = (diag(u,d) v[u][d] + w_in)            diagonal, on the fly
-//           + sum_j Hdn[d][j] v[u][j]               down hopping (row in LDS)
-//           + sum_k Hup[u][k] v[k][d]               up hopping (coalesced rows)
-//           + sum_V c * v[u'][d']                   impurity interaction, on the fly
-// diag(u,d) = Eup[u] + Edn[d] + X(bits_u, bits_d), X from the cross table mode.
+//   w[u][d] = diag(u,d) v[u][d] + w_in[u][d]          diagonal, on the fly
+//           + sum_j Hdn[d][j] v[u][j]                  down hopping
+//           + sum_k Hup[u][k] v[k][d]                  up hopping
+//           + sum_V c v[u'][d']                        impurity interaction, on the fly
+// with diag(u,d) = Eup[u] + Edn[d] + X(bits_u, bits_d).
 //
-// One work-group processes one up-row at a time, in the processing order
-// row_order (reverse Cuthill-McKee of the up-spin hopping graph, so that the
-// neighbour rows read by the up-spin term are reused from the Infinity Cache).
-// The row is staged in LDS (scaled by vscale) so the down-spin gathers hit
-// LDS; the down-spin table is a sliced ELL with slice height 64, so a
-// wavefront's entry loads coalesce.
+// Rows kernel: one work-group per up-row at a time (rows taken in a reverse
+// Cuthill-McKee order of the up-spin hopping graph).  The row is staged in
+// LDS, so the down-spin gathers hit LDS; each thread owns J columns and keeps
+// their accumulators in registers, so the up-spin term issues J independent
+// coalesced row loads per neighbour.  The down-spin table is a sliced ELL
+// (slice height 64 = one wavefront) with 16-bit entries when the spin basis
+// and its value set are small.
+//
+// Up-spin slab kernel (two-pass variant): the rows kernel runs without the
+// up-spin term, then each XCD sweeps all rows over its own column slab of 32
+// columns, so the slab (D_up x 256 B) stays in that XCD's L2 while every
+// neighbour row segment is re-read from it.
+#include <algorithm>
+#include <cstdlib>
+
 #include "edh_internal.h"
 
 namespace edh {
@@ -34,6 +43,8 @@ struct HvParams {
   const int32_t* dn_slptr;
   const int32_t* dn_sllen;
   const uint32_t* dn_ell;
+  const uint16_t* dn_ell16;
+  double dn_vt[4];
   const int32_t* dn_rank;
   const double* values;
   // cross term
@@ -55,6 +66,8 @@ struct HvParams {
   double* dot_partial;
   const double* dscale;
   const int* done;
+  // slab kernel geometry
+  int nslabs, rows_per_wg, nrb;
 };
 
 template <int CROSS>
@@ -75,11 +88,19 @@ __device__ __forceinline__ double cross_term(const HvParams& p, int u, int bu, i
   return 0.0;
 }
 
-// One block processes one up-row at a time (grid-stride over the processing
-// order).  Each thread owns J columns d = c0 + tid + j*T of the row and keeps
-// their accumulators in registers, so every neighbour row of the up-spin
-// term issues J independent coalesced loads per thread.
-template <int J, int CROSS, bool KAN, bool LDS>
+__device__ __forceinline__ void block_reduce_store(double x, double* s_red, double* out) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) s_red[wid] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += s_red[q];
+    *out = t;
+  }
+}
+
+template <int J, int CROSS, bool KAN, bool LDS, bool E16, bool UP>
 __global__ void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
   __shared__ double s_red[16];
@@ -100,6 +121,7 @@ __global__ void hv_rows_kernel(HvParams p) {
     const size_t rowoff = (size_t)u * Ddn;
     if (LDS) {
       for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
+      if (E16 && tid == 0) s_v[Ddn] = 0.0;  // target of the padding entries
       __syncthreads();
     }
     const int bu = p.up_bits[u];
@@ -116,7 +138,7 @@ __global__ void hv_rows_kernel(HvParams p) {
         bd[j] = 0;
         if (d < Ddn) {
           vv[j] = LDS ? s_v[d] : V[rowoff + d] * vscale;
-          if (p.accumulate) acc[j] = wscale * p.wprev[rowoff + d];
+          if (p.accumulate) acc[j] = wscale * __builtin_nontemporal_load(p.wprev + rowoff + d);
           bd[j] = p.dn_bits[d];
         }
       }
@@ -133,35 +155,52 @@ __global__ void hv_rows_kernel(HvParams p) {
         if (d < Ddn) {
           const int sl = d >> 6;
           const int len = p.dn_sllen[sl];
-          const uint32_t* __restrict__ ep = p.dn_ell + p.dn_slptr[sl] + (d & 63);
+          const int base = p.dn_slptr[sl] + (d & 63);
           double a0 = 0.0, a1 = 0.0;
-          int k = 0;
-          for (; k + 1 < len; k += 2) {
-            const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
-            const int c0_ = (int)(e0 & kColMask), c1_ = (int)(e1 & kColMask);
-            const double x0 = LDS ? s_v[c0_] : V[rowoff + c0_] * vscale;
-            const double x1 = LDS ? s_v[c1_] : V[rowoff + c1_] * vscale;
-            a0 = fma(vals[e0 >> kColBits], x0, a0);
-            a1 = fma(vals[e1 >> kColBits], x1, a1);
-          }
-          if (k < len) {
-            const uint32_t e0 = ep[k * kSlice];
-            const int c0_ = (int)(e0 & kColMask);
-            a0 = fma(vals[e0 >> kColBits], LDS ? s_v[c0_] : V[rowoff + c0_] * vscale, a0);
+          if (E16) {
+            const uint16_t* __restrict__ ep = p.dn_ell16 + base;
+            int k = 0;
+            for (; k + 1 < len; k += 2) {
+              const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
+              a0 = fma(p.dn_vt[e0 >> 14], s_v[e0 & 0x3FFF], a0);
+              a1 = fma(p.dn_vt[e1 >> 14], s_v[e1 & 0x3FFF], a1);
+            }
+            if (k < len) {
+              const uint32_t e0 = ep[k * kSlice];
+              a0 = fma(p.dn_vt[e0 >> 14], s_v[e0 & 0x3FFF], a0);
+            }
+          } else {
+            const uint32_t* __restrict__ ep = p.dn_ell + base;
+            int k = 0;
+            for (; k + 1 < len; k += 2) {
+              const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
+              const int c0_ = (int)(e0 & kColMask), c1_ = (int)(e1 & kColMask);
+              const double x0 = LDS ? s_v[c0_] : V[rowoff + c0_] * vscale;
+              const double x1 = LDS ? s_v[c1_] : V[rowoff + c1_] * vscale;
+              a0 = fma(vals[e0 >> kColBits], x0, a0);
+              a1 = fma(vals[e1 >> kColBits], x1, a1);
+            }
+            if (k < len) {
+              const uint32_t e0 = ep[k * kSlice];
+              const int c0_ = (int)(e0 & kColMask);
+              a0 = fma(vals[e0 >> kColBits], LDS ? s_v[c0_] : V[rowoff + c0_] * vscale, a0);
+            }
           }
           acc[j] += a0 + a1;
         }
       }
       // up-spin hopping: linear combination of whole rows (coalesced)
-      for (int q = hb; q < he; ++q) {
-        const uint32_t e = p.up_packed[q];
-        const double h = vals[e >> kColBits] * vscale;
-        const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn + c0 + tid;
-        double x[J];
+      if (UP) {
+        for (int q = hb; q < he; ++q) {
+          const uint32_t e = p.up_packed[q];
+          const double h = vals[e >> kColBits] * vscale;
+          const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn + c0 + tid;
+          double x[J];
 #pragma unroll
-        for (int j = 0; j < J; ++j) x[j] = (c0 + tid + j * T < Ddn) ? src[j * T] : 0.0;
+          for (int j = 0; j < J; ++j) x[j] = (c0 + tid + j * T < Ddn) ? src[j * T] : 0.0;
 #pragma unroll
-        for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
+          for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
+        }
       }
       if (KAN) {
         const int a = bu >> p.nb;
@@ -183,24 +222,60 @@ __global__ void hv_rows_kernel(HvParams p) {
       for (int j = 0; j < J; ++j) {
         const int d = c0 + tid + j * T;
         if (d < Ddn) {
-          p.w[rowoff + d] = acc[j];
+          __builtin_nontemporal_store(acc[j], p.w + rowoff + d);
           dsum = fma(vv[j], acc[j], dsum);
         }
       }
     }
     if (LDS) __syncthreads();
   }
-  if (p.dot_partial) {
-    for (int off = 32; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, 64);
-    const int wid = tid >> 6;
-    if ((tid & 63) == 0) s_red[wid] = dsum;
-    __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-      for (int q = 0; q < (T >> 6); ++q) t += s_red[q];
-      p.dot_partial[blockIdx.x] = t;
+  if (UP && p.dot_partial) block_reduce_store(dsum, s_red, p.dot_partial + blockIdx.x);
+}
+
+// Up-spin term over column slabs of 32: w[u][c] += sum_k Hup[u][k] v[k][c].
+// Work-group b runs on XCD b % 8 (round-robin dispatch); the eight XCDs take
+// eight consecutive slabs, all row blocks of a slab group are consecutive in
+// b, so each XCD sweeps all rows over one slab at a time.  A wavefront covers
+// two rows (lanes 0-31 and 32-63) of the slab.
+constexpr int kSlabW = 32;
+constexpr int kSlabWG = 256;
+
+__global__ void hv_upslab_kernel(HvParams p) {
+  __shared__ double s_red[16];
+  if (p.done && *p.done) return;
+  const double vscale = p.dscale ? p.dscale[0] : p.vscale;
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int idx = b >> 3;
+  const int grp = idx / p.nrb;
+  const int rb = idx - grp * p.nrb;
+  const int slab = grp * 8 + xcd;
+  double dsum = 0.0;
+  if (slab < p.nslabs) {
+    const int Ddn = p.Ddn;
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int c = slab * kSlabW + (lane & 31);
+    const int wave = threadIdx.x >> 6;
+    const int r0 = rb * p.rows_per_wg;
+    const int r1 = min(r0 + p.rows_per_wg, p.Dup);
+    const double* __restrict__ V = p.v;
+    for (int r = r0 + wave * 2 + half; r < r1; r += (kSlabWG / 64) * 2) {
+      const int hb = p.up_rowp[r], he = p.up_rowp[r + 1];
+      double acc = 0.0;
+      if (c < Ddn) {
+        for (int q = hb; q < he; ++q) {
+          const uint32_t e = p.up_packed[q];
+          acc = fma(p.values[e >> kColBits], V[(size_t)(e & kColMask) * Ddn + c], acc);
+        }
+        const size_t o = (size_t)r * Ddn + c;
+        const double wn = __builtin_nontemporal_load(p.w + o) + acc * vscale;
+        __builtin_nontemporal_store(wn, p.w + o);
+        if (p.dot_partial) dsum = fma(V[o] * vscale, wn, dsum);
+      }
     }
   }
+  if (p.dot_partial) block_reduce_store(dsum, s_red, p.dot_partial + blockIdx.x);
 }
 
 namespace {
@@ -216,29 +291,27 @@ size_t lds_budget() {
     EDH_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     cached[dev] = v > 0 ? v : 64 * 1024;
   }
-  // leave room for the block's static reduction scratch
-  return (size_t)cached[dev] - 1024;
+  return (size_t)cached[dev] - 1024;  // room for the block's static reduction scratch
 }
 
 struct Launch {
   int J, block, grid;
   size_t smem;
-  bool lds;
+  bool lds, e16;
 };
 
-// Columns per thread J and threads per block T: T = 64 * ceil(Ddn / (64 J))
-// with J the smallest of {1,2,4,8,16} that keeps T <= 1024 and near 256
-// threads for short rows.
+// Columns per thread J (smallest power of two <= 16 keeping rows near 256
+// threads) and threads per block T = 64 * ceil(ceil(Ddn / J) / 64) <= 1024.
 Launch plan(const DevSector& ds) {
   Launch L;
   const int64_t Ddn = ds.dn.D;
-  const size_t rowb = (size_t)Ddn * sizeof(double);
+  const size_t rowb = (size_t)(Ddn + 1) * sizeof(double);
   L.lds = rowb <= lds_budget();
+  L.e16 = L.lds && ds.dn.ell16 != nullptr;
   int J = 1;
   while (J < 16 && (Ddn + J - 1) / J > 256) J *= 2;
   int64_t T = ((Ddn + J - 1) / J + 63) / 64 * 64;
-  if (T > 1024) T = 1024;
-  if (T < 64) T = 64;
+  T = std::min<int64_t>(std::max<int64_t>(T, 64), 1024);
   L.J = J;
   L.block = (int)T;
   L.grid = (int)std::min<int64_t>(ds.up.D, 1 << 20);
@@ -246,9 +319,9 @@ Launch plan(const DevSector& ds) {
   return L;
 }
 
-template <int J, int CROSS, bool KAN, bool LDS>
-void launch_t(const HvParams& p, const Launch& L, hipStream_t s) {
-  auto k = hv_rows_kernel<J, CROSS, KAN, LDS>;
+template <int J, int CROSS, bool KAN, bool LDS, bool E16>
+void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
+  auto k = up ? hv_rows_kernel<J, CROSS, KAN, LDS, E16, true> : hv_rows_kernel<J, CROSS, KAN, LDS, E16, false>;
   if (L.smem > 64 * 1024)
     EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.smem));
   hipLaunchKernelGGL(k, dim3(L.grid), dim3(L.block), L.smem, s, p);
@@ -256,34 +329,38 @@ void launch_t(const HvParams& p, const Launch& L, hipStream_t s) {
 }
 
 template <int J, int CROSS>
-void launch_c(const HvParams& p, const Launch& L, bool kan, hipStream_t s) {
+void launch_c(const HvParams& p, const Launch& L, bool kan, bool up, hipStream_t s) {
   if (kan) {
-    if (L.lds) launch_t<J, CROSS, true, true>(p, L, s);
-    else launch_t<J, CROSS, true, false>(p, L, s);
+    if (L.e16) launch_t<J, CROSS, true, true, true>(p, L, up, s);
+    else if (L.lds) launch_t<J, CROSS, true, true, false>(p, L, up, s);
+    else launch_t<J, CROSS, true, false, false>(p, L, up, s);
   } else {
-    if (L.lds) launch_t<J, CROSS, false, true>(p, L, s);
-    else launch_t<J, CROSS, false, false>(p, L, s);
+    if (L.e16) launch_t<J, CROSS, false, true, true>(p, L, up, s);
+    else if (L.lds) launch_t<J, CROSS, false, true, false>(p, L, up, s);
+    else launch_t<J, CROSS, false, false, false>(p, L, up, s);
   }
 }
 
 template <int J>
-void launch_j(const HvParams& p, const Launch& L, int cross, bool kan, hipStream_t s) {
+void launch_j(const HvParams& p, const Launch& L, int cross, bool kan, bool up, hipStream_t s) {
   switch (cross) {
-    case CROSS_NONE: launch_c<J, CROSS_NONE>(p, L, kan, s); break;
-    case CROSS_POPC: launch_c<J, CROSS_POPC>(p, L, kan, s); break;
-    case CROSS_IMP: launch_c<J, CROSS_IMP>(p, L, kan, s); break;
-    default: launch_c<J, CROSS_GENERAL>(p, L, kan, s); break;
+    case CROSS_NONE: launch_c<J, CROSS_NONE>(p, L, kan, up, s); break;
+    case CROSS_POPC: launch_c<J, CROSS_POPC>(p, L, kan, up, s); break;
+    case CROSS_IMP: launch_c<J, CROSS_IMP>(p, L, kan, up, s); break;
+    default: launch_c<J, CROSS_GENERAL>(p, L, kan, up, s); break;
   }
 }
 
-}  // namespace
+// Grid of the slab kernel: ceil(nslabs / 8) slab groups x 8 XCDs x row blocks.
+int slab_geometry(const DevSector& ds, int& nslabs, int& rows_per_wg, int& nrb) {
+  nslabs = (int)((ds.dn.D + kSlabW - 1) / kSlabW);
+  rows_per_wg = 64;
+  nrb = (int)((ds.up.D + rows_per_wg - 1) / rows_per_wg);
+  const int ngroups = (nslabs + 7) / 8;
+  return ngroups * 8 * nrb;
+}
 
-int hv_num_blocks(const DevSector& ds) { return plan(ds).grid; }
-
-void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant) {
-  (void)variant;
-  if (ds.N == 0) return;
-  Launch L = plan(ds);
+HvParams make_params(const DevSector& ds, const HvArgs& a) {
   HvParams p;
   p.Ns = ds.Ns;
   p.nb = ds.Ns - ds.imp_ml;
@@ -300,6 +377,8 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
   p.dn_slptr = ds.dn.sl_ptr;
   p.dn_sllen = ds.dn.sl_len;
   p.dn_ell = ds.dn.ell;
+  p.dn_ell16 = ds.dn.ell16;
+  for (int q = 0; q < 4; ++q) p.dn_vt[q] = ds.dn.vt16[q];
   p.dn_rank = ds.dn.rank;
   p.values = ds.values;
   p.cross_u = ds.cross_u;
@@ -319,14 +398,87 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
   p.dot_partial = a.dot_partial;
   p.dscale = a.dscale;
   p.done = a.done;
+  slab_geometry(ds, p.nslabs, p.rows_per_wg, p.nrb);
+  return p;
+}
+
+}  // namespace
+
+int hv_num_blocks(const DevSector& ds) {
+  int nslabs, rpw, nrb;
+  const int grid = slab_geometry(ds, nslabs, rpw, nrb);
+  return ds.variant == HV_TWO_PASS ? grid : plan(ds).grid;
+}
+
+void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant) {
+  if (ds.N == 0) return;
+  if (variant == HV_AUTO) variant = ds.variant;
+  Launch L = plan(ds);
+  HvParams p = make_params(ds, a);
   const bool kan = ds.kan_ml > 0;
+  const bool up = variant != HV_TWO_PASS;
   switch (L.J) {
-    case 1: launch_j<1>(p, L, ds.cross_mode, kan, s); break;
-    case 2: launch_j<2>(p, L, ds.cross_mode, kan, s); break;
-    case 4: launch_j<4>(p, L, ds.cross_mode, kan, s); break;
-    case 8: launch_j<8>(p, L, ds.cross_mode, kan, s); break;
-    default: launch_j<16>(p, L, ds.cross_mode, kan, s); break;
+    case 1: launch_j<1>(p, L, ds.cross_mode, kan, up, s); break;
+    case 2: launch_j<2>(p, L, ds.cross_mode, kan, up, s); break;
+    case 4: launch_j<4>(p, L, ds.cross_mode, kan, up, s); break;
+    case 8: launch_j<8>(p, L, ds.cross_mode, kan, up, s); break;
+    default: launch_j<16>(p, L, ds.cross_mode, kan, up, s); break;
   }
+  if (!up) {
+    // the rows pass wrote the partial result; the slab pass adds the up-spin
+    // term and produces the dot partials (the rows pass skips them)
+    int nslabs, rpw, nrb;
+    const int grid = slab_geometry(ds, nslabs, rpw, nrb);
+    hipLaunchKernelGGL(hv_upslab_kernel, dim3(grid), dim3(kSlabWG), 0, s, p);
+    EDH_HIP(hipGetLastError());
+  }
+}
+
+// Times both variants on scratch vectors and keeps the faster one (an
+// environment override EDH_HV_VARIANT=0/1 fixes the choice).
+int hv_autotune(DevSector& ds, hipStream_t s) {
+  const char* env = getenv("EDH_HV_VARIANT");
+  if (env && (env[0] == '0' || env[0] == '1')) return env[0] - '0';
+  if (ds.N < (1 << 16)) return HV_SINGLE;  // launch-bound sizes: one kernel
+  double *v = nullptr, *w = nullptr;
+  if (hipMalloc(&v, ds.N * sizeof(double)) != hipSuccess) return HV_SINGLE;
+  if (hipMalloc(&w, ds.N * sizeof(double)) != hipSuccess) {
+    (void)hipFree(v);
+    return HV_SINGLE;
+  }
+  EDH_HIP(hipMemsetAsync(v, 0, ds.N * sizeof(double), s));
+  EDH_HIP(hipMemsetAsync(w, 0, ds.N * sizeof(double), s));
+  HvArgs a{};
+  a.v = v;
+  a.w = w;
+  a.wprev = w;
+  a.vscale = 1.0;
+  a.wscale = 1.0;
+  a.mode = 1;
+  hipEvent_t e0, e1;
+  EDH_HIP(hipEventCreate(&e0));
+  EDH_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int choice = HV_SINGLE;
+  for (int var = HV_SINGLE; var <= HV_TWO_PASS; ++var) {
+    launch_hv(ds, a, s, var);  // warm-up
+    EDH_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < 2; ++r) launch_hv(ds, a, s, var);
+    EDH_HIP(hipEventRecord(e1, s));
+    EDH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    EDH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      choice = var;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  EDH_HIP(hipStreamSynchronize(s));
+  (void)hipFree(v);
+  (void)hipFree(w);
+  return choice;
 }
 
 }  // namespace edh

@@ -351,6 +351,34 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
         sp.ell[sp.sl_ptr[s] + k * kSlice + L] = e;
       }
     }
+  // Compact ELL when the spin has < 16384 states and <= 4 distinct values:
+  // padding entries point at column D, a zero slot the kernel appends to the
+  // row it stages in LDS.
+  sp.ell16.clear();
+  {
+    std::vector<double> vt;
+    bool ok = sp.D < 16384;
+    for (size_t q = 0; q < sp.val.size() && ok; ++q)
+      if (std::find(vt.begin(), vt.end(), sp.val[q]) == vt.end()) {
+        if (vt.size() == 4) ok = false;
+        else vt.push_back(sp.val[q]);
+      }
+    if (ok) {
+      for (int q = 0; q < 4; ++q) sp.vt16[q] = q < (int)vt.size() ? vt[q] : 0.0;
+      sp.ell16.assign(sp.ell.size(), (uint16_t)sp.D);
+      for (int64_t sl = 0; sl + 1 < (int64_t)sp.sl_ptr.size(); ++sl)
+        for (int L = 0; L < kSlice; ++L) {
+          const int64_t r = sl * kSlice + L;
+          if (r >= sp.D) continue;
+          const int cnt = sp.rowp[r + 1] - sp.rowp[r];
+          for (int k = 0; k < cnt; ++k) {
+            const int q = sp.rowp[r] + k;
+            const int vi = (int)(std::find(vt.begin(), vt.end(), sp.val[q]) - vt.begin());
+            sp.ell16[sp.sl_ptr[sl] + k * kSlice + L] = (uint16_t)(sp.col[q] | (vi << 14));
+          }
+        }
+    }
+  }
   if (need_rank) sp.rank = std::move(rank);
 }
 

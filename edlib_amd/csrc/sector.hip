@@ -27,6 +27,8 @@ void upload_spin(DevSector& ds, const SpinTables& sp, DevSpin& d, hipStream_t s)
   d.sl_ptr = upload(ds, sp.sl_ptr, s);
   d.sl_len = upload(ds, sp.sl_len, s);
   d.ell = upload(ds, sp.ell, s);
+  d.ell16 = upload(ds, sp.ell16, s);
+  for (int q = 0; q < 4; ++q) d.vt16[q] = sp.vt16[q];
   d.rank = upload(ds, sp.rank, s);
 }
 
@@ -104,6 +106,7 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s)
   std::vector<int32_t> order = rcm_order(st.up.rowp, st.up.col, st.up.D);
   ds.row_order = upload(ds, order, s);
   EDH_HIP(hipStreamSynchronize(s));
+  ds.variant = hv_autotune(ds, s);
 }
 
 }  // namespace edh

@@ -30,6 +30,8 @@ def main():
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--sites", type=int, default=16, choices=(12, 16))
+    ap.add_argument("--variant", type=int, default=-1, choices=(-1, 0, 1),
+                    help="H*v variant: -1 = autotuned, 0 = single pass, 1 = two passes")
     a = ap.parse_args()
     import edlib_amd as ea
     from edlib_amd import _lib
@@ -46,6 +48,8 @@ def main():
     ctx = ea.Context(0)
     ctx.set_model(ea.HubbardModel(Ns, t, np.full(Ns, U), np.full(Ns, U / 2)))
     n = ctx.set_sector(Ns // 2, Ns // 2)
+    if a.variant >= 0:
+        ctx.hv_variant = a.variant
     rng = np.random.default_rng(20261202 + 8)
     vb, wb = ctx.buffer(n), ctx.buffer(n)
     vb.upload(rng.uniform(-1, 1, n))
@@ -58,8 +62,8 @@ def main():
     _lib.check(_lib.lib().edh_memcpy(ctx.handle, dst.ptr, src.ptr, nbytes, 2))
     ctx.synchronize()
     _, _, tb = ctx.sector_info()
-    print("profile_hv: sites=%d n=%d reps=%d algorithmic_bytes=%d copy_bytes=%d" % (Ns, n, a.reps, 24 * n + tb, nbytes),
-          flush=True)
+    print("profile_hv: sites=%d n=%d variant=%d reps=%d algorithmic_bytes=%d copy_bytes=%d" %
+          (Ns, n, ctx.hv_variant, a.reps, 24 * n + tb, nbytes), flush=True)
     ctx.close()
 
 

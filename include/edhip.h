@@ -116,6 +116,13 @@ int edh_synchronize(edh_ctx* ctx);
  * w = H v + (clear ? 0 : w).  With EDH_MEM_HOST the call is synchronous. */
 int edh_av(edh_ctx* ctx, const double* v, double* w, int64_t n, int clear, int mem);
 
+/* H*v kernel variant of the current sector: 0 = single pass (rows), 1 = two
+ * passes (rows, then XCD-aware column slabs for the up-spin term).  The
+ * sector set-up times both and keeps the faster; edh_set_hv_variant
+ * overrides the choice until the next edh_set_sector (-1 re-runs the timing). */
+int edh_get_hv_variant(edh_ctx* ctx, int* variant);
+int edh_set_hv_variant(edh_ctx* ctx, int variant);
+
 /* Times `reps` back-to-back av launches on device vectors with HIP events on
  * the context stream after `warmup` untimed launches; *avg_ms = mean time. */
 int edh_av_time(edh_ctx* ctx, const double* v, double* w, int clear, int warmup, int reps, double* avg_ms);

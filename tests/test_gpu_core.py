@@ -56,18 +56,41 @@ def test_av_parity(ctx, name, kind, p):
         assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False)) < 1e-12
 
 
-def test_av_parity_12site_torus(ctx):
+@pytest.fixture(scope="module")
+def torus12():
     # 3x4 torus with t, t' (the 853,776-dimensional half-filled sector)
     p = dict(Ns=12, t=models.torus_hopping(3, 4, 1.0, -0.3), U=np.full(12, 8.0), xmu=np.full(12, 4.0))
-    om, pm = models.oracle_model("hubbard", p), models.product_model("hubbard", p)
-    ctx.set_model(pm)
+    om = models.oracle_model("hubbard", p)
+    return p, om, eo.SRS(om, 6, 6)
+
+
+@pytest.mark.parametrize("variant", [0, 1], ids=["single_pass", "two_pass"])
+def test_av_parity_12site_torus(ctx, torus12, variant):
+    p, om, srs = torus12
+    ctx.set_model(models.product_model("hubbard", p))
     n = ctx.set_sector(6, 6)
-    assert n == 853776
-    srs = eo.SRS(om, 6, 6)
+    assert n == 853776 and ctx.hv_variant in (0, 1)
+    ctx.hv_variant = variant
     rng = np.random.default_rng(20261202 + 2)
     v = rng.uniform(-1, 1, n)
     w0 = rng.uniform(-1, 1, n)
     assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False, omp=True)) < 1e-12
+    assert _rel(ctx.av(v), srs.av(v, omp=True)) < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1], ids=["single_pass", "two_pass"])
+def test_lanczos_12site_torus(ctx, torus12, variant):
+    p, om, srs = torus12
+    ctx.set_model(models.product_model("hubbard", p))
+    n = ctx.set_sector(6, 6)
+    ctx.hv_variant = variant
+    v = np.random.default_rng(8).uniform(-1, 1, n)
+    v /= np.linalg.norm(v)
+    nl_o, a_o, b_o = srs.lanczos(v, 25)
+    nl_g, a_g, b_g = ctx.lanczos(v, 25)
+    assert nl_o == nl_g == 25
+    assert np.max(np.abs(a_o - a_g)) < 1e-10
+    assert np.max(np.abs(b_o[1:25] - b_g[1:25])) < 1e-10
 
 
 @pytest.mark.parametrize("name,kind,p", CASES[1:2] + CASES[3:4], ids=["hubbard_ring8", "siam_kanamori"])
