@@ -17,8 +17,8 @@
 // and its value set are small.
 //
 // Up-spin slab kernel (two-pass variant): the rows kernel runs without the
-// up-spin term, then each XCD sweeps all rows over its own column slab of 32
-// columns, so the slab (D_up x 256 B) stays in that XCD's L2 while every
+// up-spin term, then each XCD sweeps all rows over its own column slab of 16
+// columns, so the slab (D_up x 128 B) stays in that XCD's L2 while every
 // neighbour row segment is re-read from it.
 #include <algorithm>
 #include <cstdlib>
@@ -101,7 +101,7 @@ __device__ __forceinline__ void block_reduce_store(double x, double* s_red, doub
 }
 
 template <int J, int CROSS, bool KAN, bool LDS, bool E16, bool UP>
-__global__ void hv_rows_kernel(HvParams p) {
+__global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
   __shared__ double s_red[16];
   const int T = blockDim.x;
@@ -148,45 +148,40 @@ __global__ void hv_rows_kernel(HvParams p) {
         const int d = c0 + tid + j * T;
         if (d < Ddn) acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd[j], na), vv[j], acc[j]);
       }
-      // down-spin hopping: sliced ELL, gathers from the staged row
+      // down-spin hopping: sliced ELL, gathers from the staged row.  Slot k
+      // of all J columns is loaded together (J independent loads in flight).
+      {
+        int len[J], base[J];
+        int maxlen = 0;
 #pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const int d = c0 + tid + j * T;
-        if (d < Ddn) {
-          const int sl = d >> 6;
-          const int len = p.dn_sllen[sl];
-          const int base = p.dn_slptr[sl] + (d & 63);
-          double a0 = 0.0, a1 = 0.0;
+        for (int j = 0; j < J; ++j) {
+          const int d = c0 + tid + j * T;
+          len[j] = 0;
+          base[j] = 0;
+          if (d < Ddn) {
+            const int sl = d >> 6;
+            len[j] = p.dn_sllen[sl];
+            base[j] = p.dn_slptr[sl] + (d & 63);
+          }
+          maxlen = max(maxlen, len[j]);
+        }
+        for (int k = 0; k < maxlen; ++k) {
           if (E16) {
-            const uint16_t* __restrict__ ep = p.dn_ell16 + base;
-            int k = 0;
-            for (; k + 1 < len; k += 2) {
-              const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
-              a0 = fma(p.dn_vt[e0 >> 14], s_v[e0 & 0x3FFF], a0);
-              a1 = fma(p.dn_vt[e1 >> 14], s_v[e1 & 0x3FFF], a1);
-            }
-            if (k < len) {
-              const uint32_t e0 = ep[k * kSlice];
-              a0 = fma(p.dn_vt[e0 >> 14], s_v[e0 & 0x3FFF], a0);
-            }
+            uint32_t e[J];
+#pragma unroll
+            for (int j = 0; j < J; ++j) e[j] = k < len[j] ? (uint32_t)p.dn_ell16[base[j] + k * kSlice] : (uint32_t)Ddn;
+#pragma unroll
+            for (int j = 0; j < J; ++j) acc[j] = fma(p.dn_vt[e[j] >> 14], s_v[e[j] & 0x3FFF], acc[j]);
           } else {
-            const uint32_t* __restrict__ ep = p.dn_ell + base;
-            int k = 0;
-            for (; k + 1 < len; k += 2) {
-              const uint32_t e0 = ep[k * kSlice], e1 = ep[(k + 1) * kSlice];
-              const int c0_ = (int)(e0 & kColMask), c1_ = (int)(e1 & kColMask);
-              const double x0 = LDS ? s_v[c0_] : V[rowoff + c0_] * vscale;
-              const double x1 = LDS ? s_v[c1_] : V[rowoff + c1_] * vscale;
-              a0 = fma(vals[e0 >> kColBits], x0, a0);
-              a1 = fma(vals[e1 >> kColBits], x1, a1);
-            }
-            if (k < len) {
-              const uint32_t e0 = ep[k * kSlice];
-              const int c0_ = (int)(e0 & kColMask);
-              a0 = fma(vals[e0 >> kColBits], LDS ? s_v[c0_] : V[rowoff + c0_] * vscale, a0);
+            uint32_t e[J];
+#pragma unroll
+            for (int j = 0; j < J; ++j) e[j] = k < len[j] ? p.dn_ell[base[j] + k * kSlice] : 0u;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+              const int c = (int)(e[j] & kColMask);
+              acc[j] = fma(vals[e[j] >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
             }
           }
-          acc[j] += a0 + a1;
         }
       }
       // up-spin hopping: linear combination of whole rows (coalesced)
@@ -232,12 +227,12 @@ __global__ void hv_rows_kernel(HvParams p) {
   if (UP && p.dot_partial) block_reduce_store(dsum, s_red, p.dot_partial + blockIdx.x);
 }
 
-// Up-spin term over column slabs of 32: w[u][c] += sum_k Hup[u][k] v[k][c].
-// Work-group b runs on XCD b % 8 (round-robin dispatch); the eight XCDs take
-// eight consecutive slabs, all row blocks of a slab group are consecutive in
-// b, so each XCD sweeps all rows over one slab at a time.  A wavefront covers
-// two rows (lanes 0-31 and 32-63) of the slab.
-constexpr int kSlabW = 32;
+// Up-spin term over column slabs of 16 (one 128-byte line per row segment):
+// w[u][c] += sum_k Hup[u][k] v[k][c].  Work-group b runs on XCD b % 8
+// (round-robin dispatch); the eight XCDs take eight consecutive slabs, all
+// row blocks of a slab group are consecutive in b, so each XCD sweeps all
+// rows over one slab at a time.  A wavefront covers four rows of the slab.
+constexpr int kSlabW = 16;
 constexpr int kSlabWG = 256;
 
 __global__ void hv_upslab_kernel(HvParams p) {
@@ -254,13 +249,13 @@ __global__ void hv_upslab_kernel(HvParams p) {
   if (slab < p.nslabs) {
     const int Ddn = p.Ddn;
     const int lane = threadIdx.x & 63;
-    const int half = lane >> 5;
-    const int c = slab * kSlabW + (lane & 31);
+    const int quarter = lane >> 4;
+    const int c = slab * kSlabW + (lane & 15);
     const int wave = threadIdx.x >> 6;
     const int r0 = rb * p.rows_per_wg;
     const int r1 = min(r0 + p.rows_per_wg, p.Dup);
     const double* __restrict__ V = p.v;
-    for (int r = r0 + wave * 2 + half; r < r1; r += (kSlabWG / 64) * 2) {
+    for (int r = r0 + wave * 4 + quarter; r < r1; r += (kSlabWG / 64) * 4) {
       const int hb = p.up_rowp[r], he = p.up_rowp[r + 1];
       double acc = 0.0;
       if (c < Ddn) {
@@ -300,8 +295,9 @@ struct Launch {
   bool lds, e16;
 };
 
-// Columns per thread J (smallest power of two <= 16 keeping rows near 256
-// threads) and threads per block T = 64 * ceil(ceil(Ddn / J) / 64) <= 1024.
+// Columns per thread J (smallest power of two <= 8 keeping rows near 256
+// threads) and threads per block T = 64 * ceil(ceil(Ddn / J) / 64) <= 1024;
+// longer rows take several column chunks.
 Launch plan(const DevSector& ds) {
   Launch L;
   const int64_t Ddn = ds.dn.D;
@@ -309,7 +305,7 @@ Launch plan(const DevSector& ds) {
   L.lds = rowb <= lds_budget();
   L.e16 = L.lds && ds.dn.ell16 != nullptr;
   int J = 1;
-  while (J < 16 && (Ddn + J - 1) / J > 256) J *= 2;
+  while (J < 8 && (Ddn + J - 1) / J > 256) J *= 2;
   int64_t T = ((Ddn + J - 1) / J + 63) / 64 * 64;
   T = std::min<int64_t>(std::max<int64_t>(T, 64), 1024);
   L.J = J;
@@ -421,8 +417,7 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
     case 1: launch_j<1>(p, L, ds.cross_mode, kan, up, s); break;
     case 2: launch_j<2>(p, L, ds.cross_mode, kan, up, s); break;
     case 4: launch_j<4>(p, L, ds.cross_mode, kan, up, s); break;
-    case 8: launch_j<8>(p, L, ds.cross_mode, kan, up, s); break;
-    default: launch_j<16>(p, L, ds.cross_mode, kan, up, s); break;
+    default: launch_j<8>(p, L, ds.cross_mode, kan, up, s); break;
   }
   if (!up) {
     // the rows pass wrote the partial result; the slab pass adds the up-spin
