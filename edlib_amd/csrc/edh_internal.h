@@ -110,6 +110,7 @@ struct SpinTables {
   std::vector<uint32_t> packed; // packed CSR entries
   std::vector<int32_t> sl_ptr, sl_len;  // sliced ELL
   std::vector<uint32_t> ell;
+  std::vector<uint8_t> ell_pad;  // 1 where the ELL slot is padding
   // Compact ELL (same slices): column in the low 14 bits, index into vt16 in
   // the high 2, when the spin has <= 16384 states and <= 4 distinct values.
   std::vector<uint16_t> ell16;
@@ -178,6 +179,7 @@ struct DevSector {
   double* kan_c = nullptr;
   int32_t* row_order = nullptr;  // processing order of up rows
   int variant = HV_SINGLE;       // chosen H*v variant
+  int* counters = nullptr;       // work queues of the slab pass (8 ints)
   int64_t table_bytes = 0;
   std::vector<void*> allocs;
   void release();

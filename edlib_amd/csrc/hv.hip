@@ -228,41 +228,71 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
 }
 
 // Up-spin term over column slabs of 16 (one 128-byte line per row segment):
-// w[u][c] += sum_k Hup[u][k] v[k][c].  Work-group b runs on XCD b % 8
-// (round-robin dispatch); the eight XCDs take eight consecutive slabs, all
-// row blocks of a slab group are consecutive in b, so each XCD sweeps all
-// rows over one slab at a time.  A wavefront covers four rows of the slab.
+// w[u][c] += sum_k Hup[u][k] v[k][c].  Persistent work-groups read the XCD
+// they run on (HW_REG_XCC_ID) and claim (slab, row block) items from that
+// XCD's queue: XCD x owns slabs x, x+8, x+16, ... and its work-groups sweep
+// all row blocks of one slab before the next, so the slab (D_up x 128 B)
+// stays in that XCD's L2 while the ~17 neighbour segments of each row are
+// re-read from it.  A work-group that finds its queue empty takes items from
+// the other XCDs' queues, so every item is processed whatever XCD ids are
+// reported.  A wavefront covers four rows of the slab.
 constexpr int kSlabW = 16;
 constexpr int kSlabWG = 256;
+constexpr int kXcd = 8;
 
-__global__ void hv_upslab_kernel(HvParams p) {
+__global__ void zero_counters_kernel(int* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* counters) {
   __shared__ double s_red[16];
+  __shared__ int s_item;
   if (p.done && *p.done) return;
   const double vscale = p.dscale ? p.dscale[0] : p.vscale;
-  const int b = blockIdx.x;
-  const int xcd = b & 7;
-  const int idx = b >> 3;
-  const int grp = idx / p.nrb;
-  const int rb = idx - grp * p.nrb;
-  const int slab = grp * 8 + xcd;
+  unsigned xcc;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+  const int home = (int)(xcc & (kXcd - 1));
+  const int Ddn = p.Ddn;
+  const int lane = threadIdx.x & 63;
+  const int quarter = lane >> 4;
+  const int wave = threadIdx.x >> 6;
+  const double* __restrict__ V = p.v;
   double dsum = 0.0;
-  if (slab < p.nslabs) {
-    const int Ddn = p.Ddn;
-    const int lane = threadIdx.x & 63;
-    const int quarter = lane >> 4;
-    const int c = slab * kSlabW + (lane & 15);
-    const int wave = threadIdx.x >> 6;
-    const int r0 = rb * p.rows_per_wg;
-    const int r1 = min(r0 + p.rows_per_wg, p.Dup);
-    const double* __restrict__ V = p.v;
-    for (int r = r0 + wave * 4 + quarter; r < r1; r += (kSlabWG / 64) * 4) {
-      const int hb = p.up_rowp[r], he = p.up_rowp[r + 1];
-      double acc = 0.0;
-      if (c < Ddn) {
-        for (int q = hb; q < he; ++q) {
-          const uint32_t e = p.up_packed[q];
-          acc = fma(p.values[e >> kColBits], V[(size_t)(e & kColMask) * Ddn + c], acc);
+  for (int t = 0; t < kXcd; ++t) {
+    const int x = (home + t) & (kXcd - 1);
+    const int nsl_x = p.nslabs > x ? (p.nslabs - x + kXcd - 1) / kXcd : 0;
+    const int items = nsl_x * p.nrb;
+    for (;;) {
+      if (threadIdx.x == 0) s_item = atomicAdd(counters + x, 1);
+      __syncthreads();
+      const int item = s_item;
+      __syncthreads();
+      if (item >= items) break;
+      const int slab = x + kXcd * (item / p.nrb);
+      const int rb = item - (item / p.nrb) * p.nrb;
+      const int c = slab * kSlabW + (lane & 15);
+      const int r0 = rb * p.rows_per_wg;
+      const int r1 = min(r0 + p.rows_per_wg, p.Dup);
+      for (int r = r0 + wave * 4 + quarter; r < r1; r += (kSlabWG / 64) * 4) {
+        if (c >= Ddn) continue;
+        const int hb = p.up_rowp[r], he = p.up_rowp[r + 1];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int q = hb;
+        for (; q + 3 < he; q += 4) {
+          const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1], e2 = p.up_packed[q + 2],
+                         e3 = p.up_packed[q + 3];
+          const double x0 = V[(size_t)(e0 & kColMask) * Ddn + c], x1 = V[(size_t)(e1 & kColMask) * Ddn + c];
+          const double x2 = V[(size_t)(e2 & kColMask) * Ddn + c], x3 = V[(size_t)(e3 & kColMask) * Ddn + c];
+          a0 = fma(p.values[e0 >> kColBits], x0, a0);
+          a1 = fma(p.values[e1 >> kColBits], x1, a1);
+          a2 = fma(p.values[e2 >> kColBits], x2, a2);
+          a3 = fma(p.values[e3 >> kColBits], x3, a3);
         }
+        for (; q < he; ++q) {
+          const uint32_t e = p.up_packed[q];
+          a0 = fma(p.values[e >> kColBits], V[(size_t)(e & kColMask) * Ddn + c], a0);
+        }
+        const double acc = (a0 + a1) + (a2 + a3);
         const size_t o = (size_t)r * Ddn + c;
         const double wn = __builtin_nontemporal_load(p.w + o) + acc * vscale;
         __builtin_nontemporal_store(wn, p.w + o);
@@ -347,13 +377,13 @@ void launch_j(const HvParams& p, const Launch& L, int cross, bool kan, bool up, 
   }
 }
 
-// Grid of the slab kernel: ceil(nslabs / 8) slab groups x 8 XCDs x row blocks.
+// Slab pass geometry; returns its (persistent) grid size.
 int slab_geometry(const DevSector& ds, int& nslabs, int& rows_per_wg, int& nrb) {
   nslabs = (int)((ds.dn.D + kSlabW - 1) / kSlabW);
-  rows_per_wg = 64;
+  rows_per_wg = 32;
   nrb = (int)((ds.up.D + rows_per_wg - 1) / rows_per_wg);
-  const int ngroups = (nslabs + 7) / 8;
-  return ngroups * 8 * nrb;
+  const int64_t items = (int64_t)nslabs * nrb;
+  return (int)std::min<int64_t>(items, 2048);
 }
 
 HvParams make_params(const DevSector& ds, const HvArgs& a) {
@@ -424,7 +454,8 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
     // term and produces the dot partials (the rows pass skips them)
     int nslabs, rpw, nrb;
     const int grid = slab_geometry(ds, nslabs, rpw, nrb);
-    hipLaunchKernelGGL(hv_upslab_kernel, dim3(grid), dim3(kSlabWG), 0, s, p);
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, ds.counters, kXcd);
+    hipLaunchKernelGGL(hv_upslab_kernel, dim3(grid), dim3(kSlabWG), 0, s, p, ds.counters);
     EDH_HIP(hipGetLastError());
   }
 }

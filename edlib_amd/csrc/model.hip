@@ -329,8 +329,12 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
   sp.packed.resize(sp.col.size());
   for (size_t q = 0; q < sp.col.size(); ++q)
     sp.packed[q] = (uint32_t)sp.col[q] | ((uint32_t)value_index(values, vidx, sp.val[q]) << kColBits);
-  // sliced ELL: slice s holds rows [64s, 64s+64), entry k of lane L at ptr + k*64 + L;
-  // padding entries point at the row itself with value index 0 (= 0.0).
+  // Sliced ELL: slice s holds rows [64s, 64s+64), entry k of lane L at
+  // ptr + k*64 + L.  Within a row the entries are reordered so that, slot by
+  // slot, each 16-lane group gathers from distinct 8-byte LDS banks (column
+  // mod 16): greedy over lanes, picking the pending entry whose bank is least
+  // used in the slot so far.  Padding entries read one shared zero slot
+  // (broadcast), so they never conflict.
   int64_t nsl = (sp.D + kSlice - 1) / kSlice;
   sp.sl_ptr.assign(nsl + 1, 0);
   sp.sl_len.assign(nsl, 0);
@@ -342,15 +346,32 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
     sp.sl_ptr[s + 1] = sp.sl_ptr[s] + len * kSlice;
   }
   sp.ell.assign(sp.sl_ptr[nsl], 0u);
-  for (int64_t s = 0; s < nsl; ++s)
+  sp.ell_pad.assign(sp.sl_ptr[nsl], 1);
+  std::vector<std::vector<int>> pending(kSlice);
+  for (int64_t s = 0; s < nsl; ++s) {
     for (int L = 0; L < kSlice; ++L) {
-      int64_t r = s * kSlice + L;
-      for (int k = 0; k < sp.sl_len[s]; ++k) {
-        uint32_t e = (uint32_t)std::min<int64_t>(r, sp.D - 1);
-        if (r < sp.D && k < sp.rowp[r + 1] - sp.rowp[r]) e = sp.packed[sp.rowp[r] + k];
-        sp.ell[sp.sl_ptr[s] + k * kSlice + L] = e;
+      pending[L].clear();
+      const int64_t r = s * kSlice + L;
+      if (r < sp.D)
+        for (int q = sp.rowp[r]; q < sp.rowp[r + 1]; ++q) pending[L].push_back(q);
+    }
+    for (int k = 0; k < sp.sl_len[s]; ++k) {
+      int used[4][16] = {{0}};
+      for (int L = 0; L < kSlice; ++L) {
+        if (pending[L].empty()) continue;
+        int* u = used[L >> 4];
+        size_t best = 0;
+        for (size_t c = 1; c < pending[L].size(); ++c)
+          if (u[sp.col[pending[L][c]] & 15] < u[sp.col[pending[L][best]] & 15]) best = c;
+        const int q = pending[L][best];
+        pending[L].erase(pending[L].begin() + best);
+        ++u[sp.col[q] & 15];
+        sp.ell[sp.sl_ptr[s] + k * kSlice + L] = sp.packed[q];
+        sp.ell_pad[sp.sl_ptr[s] + k * kSlice + L] = 0;
       }
     }
+  }
+  // padding entries of the 32-bit ELL: column 0 with value index 0 (= 0.0)
   // Compact ELL when the spin has < 16384 states and <= 4 distinct values:
   // padding entries point at column D, a zero slot the kernel appends to the
   // row it stages in LDS.
@@ -366,17 +387,13 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
     if (ok) {
       for (int q = 0; q < 4; ++q) sp.vt16[q] = q < (int)vt.size() ? vt[q] : 0.0;
       sp.ell16.assign(sp.ell.size(), (uint16_t)sp.D);
-      for (int64_t sl = 0; sl + 1 < (int64_t)sp.sl_ptr.size(); ++sl)
-        for (int L = 0; L < kSlice; ++L) {
-          const int64_t r = sl * kSlice + L;
-          if (r >= sp.D) continue;
-          const int cnt = sp.rowp[r + 1] - sp.rowp[r];
-          for (int k = 0; k < cnt; ++k) {
-            const int q = sp.rowp[r] + k;
-            const int vi = (int)(std::find(vt.begin(), vt.end(), sp.val[q]) - vt.begin());
-            sp.ell16[sp.sl_ptr[sl] + k * kSlice + L] = (uint16_t)(sp.col[q] | (vi << 14));
-          }
-        }
+      for (size_t q = 0; q < sp.ell.size(); ++q) {
+        if (sp.ell_pad[q]) continue;
+        const uint32_t e = sp.ell[q];
+        const double v = values[e >> kColBits];
+        const int vi = (int)(std::find(vt.begin(), vt.end(), v) - vt.begin());
+        sp.ell16[q] = (uint16_t)((e & kColMask) | ((uint32_t)vi << 14));
+      }
     }
   }
   if (need_rank) sp.rank = std::move(rank);

@@ -103,6 +103,7 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s)
     ds.kan_b = upload(ds, b, s);
     ds.kan_c = upload(ds, c, s);
   }
+  ds.counters = upload(ds, std::vector<int32_t>(64, 0), s);
   std::vector<int32_t> order = rcm_order(st.up.rowp, st.up.col, st.up.D);
   ds.row_order = upload(ds, order, s);
   EDH_HIP(hipStreamSynchronize(s));
