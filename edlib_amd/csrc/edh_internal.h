@@ -115,6 +115,11 @@ struct SpinTables {
   // the high 2, when the spin has <= 16384 states and <= 4 distinct values.
   std::vector<uint16_t> ell16;
   double vt16[4] = {0, 0, 0, 0};
+  int nvt16 = 0;  // distinct values used by ell16
+  // The compact ELL regrouped by slot quads: quad q of lane L of slice s at
+  // q4_ptr[s] + q*64 + L packs slots 4q..4q+3 (lowest 16 bits first).
+  std::vector<int32_t> q4_ptr;
+  std::vector<uint64_t> ell_q4;
   std::vector<int32_t> rank;    // rank by bit pattern (2^Ns), -1 if not in the basis
 };
 
@@ -159,6 +164,9 @@ struct DevSpin {
   uint32_t* ell = nullptr;
   uint16_t* ell16 = nullptr;
   double vt16[4] = {0, 0, 0, 0};
+  int nvt16 = 0;
+  int32_t* q4_ptr = nullptr;
+  uint64_t* ell_q4 = nullptr;
   int32_t* rank = nullptr;
 };
 

@@ -13,8 +13,8 @@
 // LDS, so the down-spin gathers hit LDS; each thread owns J columns and keeps
 // their accumulators in registers, so the up-spin term issues J independent
 // coalesced row loads per neighbour.  The down-spin table is a sliced ELL
-// (slice height 64 = one wavefront) with 16-bit entries when the spin basis
-// and its value set are small.
+// (slice height 64 = one wavefront); when the spin basis and its value set
+// are small, 16-bit entries packed by slot quads are software pipelined.
 //
 // Up-spin slab kernel (two-pass variant): the rows kernel runs without the
 // up-spin term, then each XCD sweeps all rows over its own column slab of 16
@@ -43,7 +43,8 @@ struct HvParams {
   const int32_t* dn_slptr;
   const int32_t* dn_sllen;
   const uint32_t* dn_ell;
-  const uint16_t* dn_ell16;
+  const int32_t* dn_q4ptr;
+  const uint64_t* dn_q4;
   double dn_vt[4];
   const int32_t* dn_rank;
   const double* values;
@@ -100,7 +101,19 @@ __device__ __forceinline__ void block_reduce_store(double x, double* s_red, doub
   }
 }
 
-template <int J, int CROSS, bool KAN, bool LDS, bool E16, bool UP>
+// Decodes a compact entry's value: EF == 2 uses one class bit (two values),
+// EF == 1 two class bits (four values).
+template <int EF>
+__device__ __forceinline__ double q4_value(const HvParams& p, uint32_t e) {
+  if (EF == 2) return (e & 0x4000u) ? p.dn_vt[1] : p.dn_vt[0];
+  const double lo = (e & 0x4000u) ? p.dn_vt[1] : p.dn_vt[0];
+  const double hi = (e & 0x4000u) ? p.dn_vt[3] : p.dn_vt[2];
+  return (e & 0x8000u) ? hi : lo;
+}
+
+// EF: down-spin entry format, 0 = 32-bit sliced ELL, 1/2 = 16-bit entries in
+// slot quads (four / two distinct values).
+template <int J, int CROSS, bool KAN, bool LDS, int EF, bool UP>
 __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
   __shared__ double s_red[16];
@@ -121,66 +134,93 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
     const size_t rowoff = (size_t)u * Ddn;
     if (LDS) {
       for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
-      if (E16 && tid == 0) s_v[Ddn] = 0.0;  // target of the padding entries
+      if (EF && tid == 0) s_v[Ddn] = 0.0;  // target of the padding entries
       __syncthreads();
     }
     const int bu = p.up_bits[u];
     const double eu = p.up_e[u];
     const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
     for (int c0 = 0; c0 < Ddn; c0 += T * J) {
-      double acc[J], vv[J];
+      // Columns past the row end are clamped to the last column: every load
+      // below is unconditional (no per-column branch, so all loads of a step
+      // are in flight together) and such columns are never stored.
+      int dc[J];
+#pragma unroll
+      for (int j = 0; j < J; ++j) dc[j] = min(c0 + tid + j * T, Ddn - 1);
+      double acc[J];
       int bd[J];
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const int d = c0 + tid + j * T;
-        acc[j] = 0.0;
-        vv[j] = 0.0;
-        bd[j] = 0;
-        if (d < Ddn) {
-          vv[j] = LDS ? s_v[d] : V[rowoff + d] * vscale;
-          if (p.accumulate) acc[j] = wscale * __builtin_nontemporal_load(p.wprev + rowoff + d);
-          bd[j] = p.dn_bits[d];
-        }
+        bd[j] = p.dn_bits[dc[j]];
+        acc[j] = p.accumulate ? wscale * __builtin_nontemporal_load(p.wprev + rowoff + dc[j]) : 0.0;
       }
       // diagonal, generated from the bit patterns
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const int d = c0 + tid + j * T;
-        if (d < Ddn) acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd[j], na), vv[j], acc[j]);
+        const double vd = LDS ? s_v[dc[j]] : V[rowoff + dc[j]] * vscale;
+        acc[j] = fma(eu + p.dn_e[dc[j]] + cross_term<CROSS>(p, u, bu, bd[j], na), vd, acc[j]);
       }
-      // down-spin hopping: sliced ELL, gathers from the staged row.  Slot k
-      // of all J columns is loaded together (J independent loads in flight).
-      {
+      // down-spin hopping, gathers from the staged row
+      if (EF) {
+        // Slot quads (four 16-bit entries in one 64-bit word), software
+        // pipelined: quad q+1 of every column is in flight while quad q is
+        // consumed.  Loads are clamped to valid addresses and replaced by the
+        // padding quad past a column's end, so the loop has no branches.
+        int nq[J], qb[J];
+        int maxq = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int sl = dc[j] >> 6;
+          nq[j] = (p.dn_sllen[sl] + 3) >> 2;
+          qb[j] = p.dn_q4ptr[sl] + (dc[j] & 63);
+          maxq = max(maxq, nq[j]);
+        }
+        const uint64_t pad1 = (uint64_t)(uint32_t)Ddn;
+        const uint64_t padq = pad1 | (pad1 << 16) | (pad1 << 32) | (pad1 << 48);
+        uint64_t cur[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const uint64_t t = p.dn_q4[qb[j]];
+          cur[j] = nq[j] > 0 ? t : padq;
+        }
+        for (int q = 0; q < maxq; ++q) {
+          uint64_t nx[J];
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+            const int qq = min(q + 1, max(nq[j] - 1, 0));
+            const uint64_t t = p.dn_q4[qb[j] + qq * kSlice];
+            nx[j] = (q + 1 < nq[j]) ? t : padq;
+          }
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+#pragma unroll
+            for (int sub = 0; sub < 4; ++sub) {
+              const uint32_t e = (uint32_t)(cur[j] >> (16 * sub)) & 0xFFFFu;
+              acc[j] = fma(q4_value<EF>(p, e), s_v[e & 0x3FFFu], acc[j]);
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < J; ++j) cur[j] = nx[j];
+        }
+      } else {
+        // 32-bit sliced ELL: slot k of all J columns loaded together
         int len[J], base[J];
         int maxlen = 0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int d = c0 + tid + j * T;
-          len[j] = 0;
-          base[j] = 0;
-          if (d < Ddn) {
-            const int sl = d >> 6;
-            len[j] = p.dn_sllen[sl];
-            base[j] = p.dn_slptr[sl] + (d & 63);
-          }
+          const int sl = dc[j] >> 6;
+          len[j] = p.dn_sllen[sl];
+          base[j] = p.dn_slptr[sl] + (dc[j] & 63);
           maxlen = max(maxlen, len[j]);
         }
         for (int k = 0; k < maxlen; ++k) {
-          if (E16) {
-            uint32_t e[J];
+          uint32_t e[J];
 #pragma unroll
-            for (int j = 0; j < J; ++j) e[j] = k < len[j] ? (uint32_t)p.dn_ell16[base[j] + k * kSlice] : (uint32_t)Ddn;
+          for (int j = 0; j < J; ++j) e[j] = k < len[j] ? p.dn_ell[base[j] + k * kSlice] : 0u;
 #pragma unroll
-            for (int j = 0; j < J; ++j) acc[j] = fma(p.dn_vt[e[j] >> 14], s_v[e[j] & 0x3FFF], acc[j]);
-          } else {
-            uint32_t e[J];
-#pragma unroll
-            for (int j = 0; j < J; ++j) e[j] = k < len[j] ? p.dn_ell[base[j] + k * kSlice] : 0u;
-#pragma unroll
-            for (int j = 0; j < J; ++j) {
-              const int c = (int)(e[j] & kColMask);
-              acc[j] = fma(vals[e[j] >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
-            }
+          for (int j = 0; j < J; ++j) {
+            const int c = (int)(e[j] & kColMask);
+            acc[j] = fma(vals[e[j] >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
           }
         }
       }
@@ -189,10 +229,10 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         for (int q = hb; q < he; ++q) {
           const uint32_t e = p.up_packed[q];
           const double h = vals[e >> kColBits] * vscale;
-          const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn + c0 + tid;
+          const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
           double x[J];
 #pragma unroll
-          for (int j = 0; j < J; ++j) x[j] = (c0 + tid + j * T < Ddn) ? src[j * T] : 0.0;
+          for (int j = 0; j < J; ++j) x[j] = src[dc[j]];
 #pragma unroll
           for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
         }
@@ -201,8 +241,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         const int a = bu >> p.nb;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int d = c0 + tid + j * T;
-          if (d < Ddn) {
+          {
             const int b = bd[j] >> p.nb;
             const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
             for (int q = kb; q < ke; ++q) {
@@ -218,7 +257,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         const int d = c0 + tid + j * T;
         if (d < Ddn) {
           __builtin_nontemporal_store(acc[j], p.w + rowoff + d);
-          dsum = fma(vv[j], acc[j], dsum);
+          if (UP && p.dot_partial) dsum = fma(LDS ? s_v[d] : V[rowoff + d] * vscale, acc[j], dsum);
         }
       }
     }
@@ -322,7 +361,8 @@ size_t lds_budget() {
 struct Launch {
   int J, block, grid;
   size_t smem;
-  bool lds, e16;
+  bool lds;
+  int ef;  // down-spin entry format (see hv_rows_kernel)
 };
 
 // Columns per thread J (smallest power of two <= 8 keeping rows near 256
@@ -333,7 +373,7 @@ Launch plan(const DevSector& ds) {
   const int64_t Ddn = ds.dn.D;
   const size_t rowb = (size_t)(Ddn + 1) * sizeof(double);
   L.lds = rowb <= lds_budget();
-  L.e16 = L.lds && ds.dn.ell16 != nullptr;
+  L.ef = (L.lds && ds.dn.ell_q4 != nullptr) ? (ds.dn.nvt16 <= 2 ? 2 : 1) : 0;
   int J = 1;
   while (J < 8 && (Ddn + J - 1) / J > 256) J *= 2;
   int64_t T = ((Ddn + J - 1) / J + 63) / 64 * 64;
@@ -345,9 +385,9 @@ Launch plan(const DevSector& ds) {
   return L;
 }
 
-template <int J, int CROSS, bool KAN, bool LDS, bool E16>
+template <int J, int CROSS, bool KAN, bool LDS, int EF>
 void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
-  auto k = up ? hv_rows_kernel<J, CROSS, KAN, LDS, E16, true> : hv_rows_kernel<J, CROSS, KAN, LDS, E16, false>;
+  auto k = up ? hv_rows_kernel<J, CROSS, KAN, LDS, EF, true> : hv_rows_kernel<J, CROSS, KAN, LDS, EF, false>;
   if (L.smem > 64 * 1024)
     EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.smem));
   hipLaunchKernelGGL(k, dim3(L.grid), dim3(L.block), L.smem, s, p);
@@ -357,13 +397,15 @@ void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
 template <int J, int CROSS>
 void launch_c(const HvParams& p, const Launch& L, bool kan, bool up, hipStream_t s) {
   if (kan) {
-    if (L.e16) launch_t<J, CROSS, true, true, true>(p, L, up, s);
-    else if (L.lds) launch_t<J, CROSS, true, true, false>(p, L, up, s);
-    else launch_t<J, CROSS, true, false, false>(p, L, up, s);
+    if (L.ef == 2) launch_t<J, CROSS, true, true, 2>(p, L, up, s);
+    else if (L.ef == 1) launch_t<J, CROSS, true, true, 1>(p, L, up, s);
+    else if (L.lds) launch_t<J, CROSS, true, true, 0>(p, L, up, s);
+    else launch_t<J, CROSS, true, false, 0>(p, L, up, s);
   } else {
-    if (L.e16) launch_t<J, CROSS, false, true, true>(p, L, up, s);
-    else if (L.lds) launch_t<J, CROSS, false, true, false>(p, L, up, s);
-    else launch_t<J, CROSS, false, false, false>(p, L, up, s);
+    if (L.ef == 2) launch_t<J, CROSS, false, true, 2>(p, L, up, s);
+    else if (L.ef == 1) launch_t<J, CROSS, false, true, 1>(p, L, up, s);
+    else if (L.lds) launch_t<J, CROSS, false, true, 0>(p, L, up, s);
+    else launch_t<J, CROSS, false, false, 0>(p, L, up, s);
   }
 }
 
@@ -403,7 +445,8 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   p.dn_slptr = ds.dn.sl_ptr;
   p.dn_sllen = ds.dn.sl_len;
   p.dn_ell = ds.dn.ell;
-  p.dn_ell16 = ds.dn.ell16;
+  p.dn_q4ptr = ds.dn.q4_ptr;
+  p.dn_q4 = ds.dn.ell_q4;
   for (int q = 0; q < 4; ++q) p.dn_vt[q] = ds.dn.vt16[q];
   p.dn_rank = ds.dn.rank;
   p.values = ds.values;

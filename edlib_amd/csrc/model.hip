@@ -386,6 +386,7 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
       }
     if (ok) {
       for (int q = 0; q < 4; ++q) sp.vt16[q] = q < (int)vt.size() ? vt[q] : 0.0;
+      sp.nvt16 = (int)vt.size();
       sp.ell16.assign(sp.ell.size(), (uint16_t)sp.D);
       for (size_t q = 0; q < sp.ell.size(); ++q) {
         if (sp.ell_pad[q]) continue;
@@ -394,6 +395,19 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
         const int vi = (int)(std::find(vt.begin(), vt.end(), v) - vt.begin());
         sp.ell16[q] = (uint16_t)((e & kColMask) | ((uint32_t)vi << 14));
       }
+      const int64_t nsl = (int64_t)sp.sl_len.size();
+      sp.q4_ptr.assign(nsl + 1, 0);
+      for (int64_t sl = 0; sl < nsl; ++sl) sp.q4_ptr[sl + 1] = sp.q4_ptr[sl] + (sp.sl_len[sl] + 3) / 4 * kSlice;
+      const uint64_t pad = (uint64_t)(uint16_t)sp.D;
+      sp.ell_q4.assign(sp.q4_ptr[nsl], pad | (pad << 16) | (pad << 32) | (pad << 48));
+      for (int64_t sl = 0; sl < nsl; ++sl)
+        for (int k = 0; k < sp.sl_len[sl]; ++k)
+          for (int L = 0; L < kSlice; ++L) {
+            const uint64_t e = sp.ell16[sp.sl_ptr[sl] + k * kSlice + L];
+            uint64_t& w = sp.ell_q4[sp.q4_ptr[sl] + (k / 4) * kSlice + L];
+            const int sh = 16 * (k % 4);
+            w = (w & ~(0xFFFFull << sh)) | (e << sh);
+          }
     }
   }
   if (need_rank) sp.rank = std::move(rank);

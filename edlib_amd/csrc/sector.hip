@@ -29,6 +29,9 @@ void upload_spin(DevSector& ds, const SpinTables& sp, DevSpin& d, hipStream_t s)
   d.ell = upload(ds, sp.ell, s);
   d.ell16 = upload(ds, sp.ell16, s);
   for (int q = 0; q < 4; ++q) d.vt16[q] = sp.vt16[q];
+  d.nvt16 = sp.nvt16;
+  d.q4_ptr = upload(ds, sp.q4_ptr, s);
+  d.ell_q4 = upload(ds, sp.ell_q4, s);
   d.rank = upload(ds, sp.rank, s);
 }
 
