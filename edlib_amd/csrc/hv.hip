@@ -172,7 +172,9 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         for (int j = 0; j < J; ++j) {
           const int sl = dc[j] >> 6;
           nq[j] = (p.dn_sllen[sl] + 3) >> 2;
-          qb[j] = p.dn_q4ptr[sl] + (dc[j] & 63);
+          // an empty slice has no quads: point at quad 0 (the array is not
+          // empty on this path) so the clamped loads below stay in bounds
+          qb[j] = nq[j] > 0 ? p.dn_q4ptr[sl] + (dc[j] & 63) : 0;
           maxq = max(maxq, nq[j]);
         }
         const uint64_t pad1 = (uint64_t)(uint32_t)Ddn;
@@ -210,13 +212,18 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         for (int j = 0; j < J; ++j) {
           const int sl = dc[j] >> 6;
           len[j] = p.dn_sllen[sl];
-          base[j] = p.dn_slptr[sl] + (dc[j] & 63);
+          base[j] = len[j] > 0 ? p.dn_slptr[sl] + (dc[j] & 63) : 0;
           maxlen = max(maxlen, len[j]);
         }
         for (int k = 0; k < maxlen; ++k) {
+          // k < maxlen means some column has entries, so the array is not
+          // empty; indices are clamped into each column's own slots
           uint32_t e[J];
 #pragma unroll
-          for (int j = 0; j < J; ++j) e[j] = k < len[j] ? p.dn_ell[base[j] + k * kSlice] : 0u;
+          for (int j = 0; j < J; ++j) {
+            const uint32_t t = p.dn_ell[base[j] + min(k, max(len[j] - 1, 0)) * kSlice];
+            e[j] = k < len[j] ? t : 0u;
+          }
 #pragma unroll
           for (int j = 0; j < J; ++j) {
             const int c = (int)(e[j] & kColMask);

@@ -287,6 +287,7 @@ int value_index(std::vector<double>& values, std::map<uint64_t, int>& idx, doubl
 void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<double>& values,
                 std::map<uint64_t, int>& vidx, bool need_rank) {
   const int Ns = m.Ns;
+  sp = SpinTables();  // no table of an earlier sector or model may survive
   sp.nel = nel;
   sp.bits = spin_basis(Ns, nel);
   sp.D = (int64_t)sp.bits.size();
@@ -418,6 +419,7 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
 void build_sector(const Model& m, int nup, int ndn, SectorTables& st, bool need_rank) {
   const int Ns = m.Ns;
   EDH_CHECK(nup >= 0 && nup <= Ns && ndn >= 0 && ndn <= Ns, EDH_ERR_ARG, "sector out of range");
+  st = SectorTables();
   st.nup = nup;
   st.ndn = ndn;
   st.imp_ml = m.ml;
