@@ -40,12 +40,13 @@ def torus(L1, L2, t=1.0):
     return T
 
 
-def load_traffic():
-    """HBM traffic per launch measured by rocprofv3 counters (profiles/), or None."""
+def load_traffic(variant):
+    """Memory traffic per launch of the given H*v variant, measured by the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE counters and kept under profiles/, or None."""
     p = os.path.join(ROOT, "profiles", "hv_16site_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            return json.load(f).get("traffic_bytes_per_launch_by_variant", {}).get(str(variant))
     return None
 
 
@@ -134,7 +135,7 @@ def main():
                    "algorithmic_bytes_per_step": B, "parallelism": "single GPU"},
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(chosen),
                      "kernel": ("edh::hv_rows_kernel (single pass)" if chosen == 0 else
                                 "edh::hv_rows_kernel (no up term) + edh::hv_upslab_kernel (two passes)"),
                      "kernel_ms": round(kern_ms, 4), "variants": per_variant},
