@@ -120,6 +120,10 @@ struct SpinTables {
   // q4_ptr[s] + q*64 + L packs slots 4q..4q+3 (lowest 16 bits first).
   std::vector<int32_t> q4_ptr;
   std::vector<uint64_t> ell_q4;
+  // Signed byte-offset ELL (same quads, 32-bit slots) when the values are
+  // {a} or {a, -a}: slot = 8 * column | (value == -a), padding = 8 * D.
+  std::vector<uint32_t> ell_s4;
+  double s4_scale = 0.0;
   std::vector<int32_t> rank;    // rank by bit pattern (2^Ns), -1 if not in the basis
 };
 
@@ -167,6 +171,8 @@ struct DevSpin {
   int nvt16 = 0;
   int32_t* q4_ptr = nullptr;
   uint64_t* ell_q4 = nullptr;
+  uint32_t* ell_s4 = nullptr;
+  double s4_scale = 0.0;
   int32_t* rank = nullptr;
 };
 

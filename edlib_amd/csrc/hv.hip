@@ -13,8 +13,12 @@
 // LDS, so the down-spin gathers hit LDS; each thread owns J columns and keeps
 // their accumulators in registers, so the up-spin term issues J independent
 // coalesced row loads per neighbour.  The down-spin table is a sliced ELL
-// (slice height 64 = one wavefront); when the spin basis and its value set
-// are small, 16-bit entries packed by slot quads are software pipelined.
+// (slice height 64 = one wavefront).  When the hopping values are {a} or
+// {a, -a} (uniform hopping) each slot is the LDS byte offset of its column
+// with the sign in bit 0, read four slots per 16-byte load, so a slot costs
+// one mask, one sign insert and one FMA; otherwise, when the spin basis and
+// its value set are small, 16-bit entries are packed by slot quads.  Either
+// way all quads of a column group are loaded at once, one group ahead.
 //
 // Up-spin slab kernel (two-pass variant): the rows kernel runs without the
 // up-spin term, then each XCD sweeps all rows over its own column slab of 16
@@ -46,6 +50,8 @@ struct HvParams {
   const int32_t* dn_q4ptr;
   const uint64_t* dn_q4;
   double dn_vt[4];
+  const uint32_t* dn_s4;
+  double dn_s4_scale;
   const int32_t* dn_rank;
   const double* values;
   // cross term
@@ -67,8 +73,8 @@ struct HvParams {
   double* dot_partial;
   const double* dscale;
   const int* done;
-  // slab kernel geometry
-  int nslabs, rows_per_wg, nrb;
+  // slab kernel geometry; off32: every element offset fits 32 bits
+  int nslabs, rows_per_wg, nrb, off32;
 };
 
 template <int CROSS>
@@ -111,8 +117,24 @@ __device__ __forceinline__ double q4_value(const HvParams& p, uint32_t e) {
   return (e & 0x8000u) ? hi : lo;
 }
 
-// EF: down-spin entry format, 0 = 32-bit sliced ELL, 1/2 = 16-bit entries in
-// slot quads (four / two distinct values).
+// One signed byte-offset slot: a += (+-1) * row[e / 8], the sign in bit 0.
+__device__ __forceinline__ double s4_fma(uint32_t e, const char* sv, double a) {
+  const double sg = __hiloint2double((int)(0x3FF00000u | (e << 31)), 0);
+  return fma(sg, *reinterpret_cast<const double*>(sv + (e & ~1u)), a);
+}
+
+// Quads held in registers per column group (slices with more quads finish
+// in a plain loop).
+constexpr int kQuadRegs = 6;
+
+// EF: down-spin entry format: 0 = 32-bit sliced ELL (column | value index),
+// 1/2 = 16-bit entries in slot quads (four / two distinct values),
+// 3 = signed byte offsets in 16-byte slot quads (values {a} or {a, -a}).
+//
+// Thread tid of a block with T threads owns columns c0 + tid + j*T (j < J)
+// of each chunk c0; a wavefront therefore covers exactly one ELL slice per j,
+// so slot counts are wavefront-uniform and the slot loops need no padding
+// beyond their own slice.
 template <int J, int CROSS, bool KAN, bool LDS, int EF, bool UP>
 __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
@@ -141,68 +163,129 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
     const double eu = p.up_e[u];
     const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
     for (int c0 = 0; c0 < Ddn; c0 += T * J) {
-      // Columns past the row end are clamped to the last column: every load
-      // below is unconditional (no per-column branch, so all loads of a step
-      // are in flight together) and such columns are never stored.
-      int dc[J];
-#pragma unroll
-      for (int j = 0; j < J; ++j) dc[j] = min(c0 + tid + j * T, Ddn - 1);
+      // Columns past the row end are clamped to the last column: they are
+      // loaded like the others but never stored.
+#define DCOL(j) min(c0 + tid + (j) * T, Ddn - 1)
       double acc[J];
-      int bd[J];
+      // accumulated input and diagonal (generated from the bit patterns)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        bd[j] = p.dn_bits[dc[j]];
-        acc[j] = p.accumulate ? wscale * __builtin_nontemporal_load(p.wprev + rowoff + dc[j]) : 0.0;
-      }
-      // diagonal, generated from the bit patterns
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const double vd = LDS ? s_v[dc[j]] : V[rowoff + dc[j]] * vscale;
-        acc[j] = fma(eu + p.dn_e[dc[j]] + cross_term<CROSS>(p, u, bu, bd[j], na), vd, acc[j]);
+        const int d = DCOL(j);
+        const int bd = p.dn_bits[d];
+        const double wp = p.accumulate ? wscale * __builtin_nontemporal_load(p.wprev + rowoff + d) : 0.0;
+        const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
+        acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd, na), vd, wp);
       }
       // down-spin hopping, gathers from the staged row
-      if (EF) {
-        // Slot quads (four 16-bit entries in one 64-bit word), software
-        // pipelined: quad q+1 of every column is in flight while quad q is
-        // consumed.  Loads are clamped to valid addresses and replaced by the
-        // padding quad past a column's end, so the loop has no branches.
-        int nq[J], qb[J];
-        int maxq = 0;
+      if (EF == 3) {
+        // all quads of column group j are loaded at once and those of group
+        // j+1 are in flight while group j is consumed
+        const uint4* __restrict__ S4 = reinterpret_cast<const uint4*>(p.dn_s4);
+        const char* sv = reinterpret_cast<const char*>(s_v);
+        uint4 qa[kQuadRegs], qn[kQuadRegs];
+        int nqa, nqn = 0;
+        const uint4* qpa;
+        {
+          const int d0 = DCOL(0);
+          const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
+          nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+          qpa = S4 + p.dn_q4ptr[sl] + (d0 & 63);
+#pragma unroll
+          for (int q = 0; q < kQuadRegs; ++q)
+            if (q < nqa) qa[q] = qpa[q * kSlice];
+        }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int sl = dc[j] >> 6;
-          nq[j] = (p.dn_sllen[sl] + 3) >> 2;
-          // an empty slice has no quads: point at quad 0 (the array is not
-          // empty on this path) so the clamped loads below stay in bounds
-          qb[j] = nq[j] > 0 ? p.dn_q4ptr[sl] + (dc[j] & 63) : 0;
-          maxq = max(maxq, nq[j]);
-        }
-        const uint64_t pad1 = (uint64_t)(uint32_t)Ddn;
-        const uint64_t padq = pad1 | (pad1 << 16) | (pad1 << 32) | (pad1 << 48);
-        uint64_t cur[J];
+          const uint4* qpn = qpa;
+          if (j + 1 < J) {
+            const int d1 = DCOL(j + 1);
+            const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
+            nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+            qpn = S4 + p.dn_q4ptr[sl] + (d1 & 63);
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const uint64_t t = p.dn_q4[qb[j]];
-          cur[j] = nq[j] > 0 ? t : padq;
-        }
-        for (int q = 0; q < maxq; ++q) {
-          uint64_t nx[J];
-#pragma unroll
-          for (int j = 0; j < J; ++j) {
-            const int qq = min(q + 1, max(nq[j] - 1, 0));
-            const uint64_t t = p.dn_q4[qb[j] + qq * kSlice];
-            nx[j] = (q + 1 < nq[j]) ? t : padq;
+            for (int q = 0; q < kQuadRegs; ++q)
+              if (q < nqn) qn[q] = qpn[q * kSlice];
           }
+          double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-          for (int j = 0; j < J; ++j) {
-#pragma unroll
-            for (int sub = 0; sub < 4; ++sub) {
-              const uint32_t e = (uint32_t)(cur[j] >> (16 * sub)) & 0xFFFFu;
-              acc[j] = fma(q4_value<EF>(p, e), s_v[e & 0x3FFFu], acc[j]);
+          for (int q = 0; q < kQuadRegs; ++q) {
+            if (q < nqa) {
+              a0 = s4_fma(qa[q].x, sv, a0);
+              a1 = s4_fma(qa[q].y, sv, a1);
+              a0 = s4_fma(qa[q].z, sv, a0);
+              a1 = s4_fma(qa[q].w, sv, a1);
             }
           }
+          for (int q = kQuadRegs; q < nqa; ++q) {
+            const uint4 c = qpa[q * kSlice];
+            a0 = s4_fma(c.x, sv, a0);
+            a1 = s4_fma(c.y, sv, a1);
+            a0 = s4_fma(c.z, sv, a0);
+            a1 = s4_fma(c.w, sv, a1);
+          }
+          acc[j] = fma(p.dn_s4_scale, a0 + a1, acc[j]);
+          if (j + 1 < J) {
 #pragma unroll
-          for (int j = 0; j < J; ++j) cur[j] = nx[j];
+            for (int q = 0; q < kQuadRegs; ++q) qa[q] = qn[q];
+            nqa = nqn;
+            qpa = qpn;
+          }
+        }
+      } else if (EF) {
+        // 16-bit entries, four per 64-bit quad, same pipelining
+        constexpr int kQ = 8;
+        uint64_t qa[kQ], qn[kQ];
+        int nqa, nqn = 0;
+        const uint64_t* qpa;
+        {
+          const int d0 = DCOL(0);
+          const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
+          nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+          qpa = p.dn_q4 + p.dn_q4ptr[sl] + (d0 & 63);
+#pragma unroll
+          for (int q = 0; q < kQ; ++q)
+            if (q < nqa) qa[q] = qpa[q * kSlice];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const uint64_t* qpn = qpa;
+          if (j + 1 < J) {
+            const int d1 = DCOL(j + 1);
+            const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
+            nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+            qpn = p.dn_q4 + p.dn_q4ptr[sl] + (d1 & 63);
+#pragma unroll
+            for (int q = 0; q < kQ; ++q)
+              if (q < nqn) qn[q] = qpn[q * kSlice];
+          }
+          double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+          for (int q = 0; q < kQ; ++q) {
+            if (q < nqa) {
+              const uint64_t c = qa[q];
+              const uint32_t e0 = (uint32_t)c & 0xFFFFu, e1 = (uint32_t)(c >> 16) & 0xFFFFu;
+              const uint32_t e2 = (uint32_t)(c >> 32) & 0xFFFFu, e3 = (uint32_t)(c >> 48);
+              a0 = fma(q4_value<EF>(p, e0), s_v[e0 & 0x3FFFu], a0);
+              a1 = fma(q4_value<EF>(p, e1), s_v[e1 & 0x3FFFu], a1);
+              a0 = fma(q4_value<EF>(p, e2), s_v[e2 & 0x3FFFu], a0);
+              a1 = fma(q4_value<EF>(p, e3), s_v[e3 & 0x3FFFu], a1);
+            }
+          }
+          for (int q = kQ; q < nqa; ++q) {
+            const uint64_t c = qpa[q * kSlice];
+#pragma unroll
+            for (int sub = 0; sub < 4; ++sub) {
+              const uint32_t e = (uint32_t)(c >> (16 * sub)) & 0xFFFFu;
+              a0 = fma(q4_value<EF>(p, e), s_v[e & 0x3FFFu], a0);
+            }
+          }
+          acc[j] += a0 + a1;
+          if (j + 1 < J) {
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) qa[q] = qn[q];
+            nqa = nqn;
+            qpa = qpn;
+          }
         }
       } else {
         // 32-bit sliced ELL: slot k of all J columns loaded together
@@ -210,9 +293,9 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         int maxlen = 0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int sl = dc[j] >> 6;
+          const int sl = DCOL(j) >> 6;
           len[j] = p.dn_sllen[sl];
-          base[j] = len[j] > 0 ? p.dn_slptr[sl] + (dc[j] & 63) : 0;
+          base[j] = len[j] > 0 ? p.dn_slptr[sl] + (DCOL(j) & 63) : 0;
           maxlen = max(maxlen, len[j]);
         }
         for (int k = 0; k < maxlen; ++k) {
@@ -239,7 +322,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
           double x[J];
 #pragma unroll
-          for (int j = 0; j < J; ++j) x[j] = src[dc[j]];
+          for (int j = 0; j < J; ++j) x[j] = src[DCOL(j)];
 #pragma unroll
           for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
         }
@@ -248,14 +331,13 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         const int a = bu >> p.nb;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          {
-            const int b = bd[j] >> p.nb;
-            const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-            for (int q = kb; q < ke; ++q) {
-              const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
-              const int dn2 = p.dn_rank[(bd[j] & lowmask) | (p.kan_b[q] << p.nb)];
-              acc[j] = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc[j]);
-            }
+          const int bd = p.dn_bits[DCOL(j)];
+          const int b = bd >> p.nb;
+          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+          for (int q = kb; q < ke; ++q) {
+            const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
+            const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
+            acc[j] = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc[j]);
           }
         }
       }
@@ -267,6 +349,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           if (UP && p.dot_partial) dsum = fma(LDS ? s_v[d] : V[rowoff + d] * vscale, acc[j], dsum);
         }
       }
+#undef DCOL
     }
     if (LDS) __syncthreads();
   }
@@ -281,9 +364,12 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
 // stays in that XCD's L2 while the ~17 neighbour segments of each row are
 // re-read from it.  A work-group that finds its queue empty takes items from
 // the other XCDs' queues, so every item is processed whatever XCD ids are
-// reported.  A wavefront covers four rows of the slab.
+// reported.  The CSR entries of an item's rows are staged in LDS as (row
+// offset, value) pairs; a wavefront covers four rows of the slab.
 constexpr int kSlabW = 16;
 constexpr int kSlabWG = 256;
+constexpr int kSlabRows = 64;
+constexpr int kSlabEnt = 2048;
 constexpr int kXcd = 8;
 
 __global__ void zero_counters_kernel(int* c, int n) {
@@ -293,15 +379,18 @@ __global__ void zero_counters_kernel(int* c, int n) {
 __global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* counters) {
   __shared__ double s_red[16];
   __shared__ int s_item;
+  __shared__ uint32_t s_off[kSlabEnt];
+  __shared__ double s_val[kSlabEnt];
   if (p.done && *p.done) return;
   const double vscale = p.dscale ? p.dscale[0] : p.vscale;
   unsigned xcc;
   asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
   const int home = (int)(xcc & (kXcd - 1));
   const int Ddn = p.Ddn;
-  const int lane = threadIdx.x & 63;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int quarter = lane >> 4;
-  const int wave = threadIdx.x >> 6;
+  const int wave = tid >> 6;
   const double* __restrict__ V = p.v;
   double dsum = 0.0;
   for (int t = 0; t < kXcd; ++t) {
@@ -309,34 +398,48 @@ __global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* cou
     const int nsl_x = p.nslabs > x ? (p.nslabs - x + kXcd - 1) / kXcd : 0;
     const int items = nsl_x * p.nrb;
     for (;;) {
-      if (threadIdx.x == 0) s_item = atomicAdd(counters + x, 1);
-      __syncthreads();
+      if (tid == 0) s_item = atomicAdd(counters + x, 1);
+      __syncthreads();  // also: the previous item's readers of s_off/s_val are done
       const int item = s_item;
-      __syncthreads();
-      if (item >= items) break;
+      if (item >= items) {
+        __syncthreads();  // everyone has read s_item before it is rewritten
+        break;
+      }
       const int slab = x + kXcd * (item / p.nrb);
       const int rb = item - (item / p.nrb) * p.nrb;
       const int c = slab * kSlabW + (lane & 15);
       const int r0 = rb * p.rows_per_wg;
       const int r1 = min(r0 + p.rows_per_wg, p.Dup);
+      const int e0 = p.up_rowp[r0], ne = p.up_rowp[r1] - e0;
+      const bool staged = p.off32 && ne <= kSlabEnt;
+      if (staged)
+        for (int i = tid; i < ne; i += kSlabWG) {
+          const uint32_t e = p.up_packed[e0 + i];
+          s_off[i] = (e & kColMask) * (uint32_t)Ddn;
+          s_val[i] = p.values[e >> kColBits];
+        }
+      __syncthreads();
       for (int r = r0 + wave * 4 + quarter; r < r1; r += (kSlabWG / 64) * 4) {
         if (c >= Ddn) continue;
         const int hb = p.up_rowp[r], he = p.up_rowp[r + 1];
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int q = hb;
-        for (; q + 3 < he; q += 4) {
-          const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1], e2 = p.up_packed[q + 2],
-                         e3 = p.up_packed[q + 3];
-          const double x0 = V[(size_t)(e0 & kColMask) * Ddn + c], x1 = V[(size_t)(e1 & kColMask) * Ddn + c];
-          const double x2 = V[(size_t)(e2 & kColMask) * Ddn + c], x3 = V[(size_t)(e3 & kColMask) * Ddn + c];
-          a0 = fma(p.values[e0 >> kColBits], x0, a0);
-          a1 = fma(p.values[e1 >> kColBits], x1, a1);
-          a2 = fma(p.values[e2 >> kColBits], x2, a2);
-          a3 = fma(p.values[e3 >> kColBits], x3, a3);
-        }
-        for (; q < he; ++q) {
-          const uint32_t e = p.up_packed[q];
-          a0 = fma(p.values[e >> kColBits], V[(size_t)(e & kColMask) * Ddn + c], a0);
+        if (staged) {
+          int q = hb - e0;
+          const int qe = he - e0;
+          for (; q + 3 < qe; q += 4) {
+            const double x0 = V[s_off[q] + c], x1 = V[s_off[q + 1] + c];
+            const double x2 = V[s_off[q + 2] + c], x3 = V[s_off[q + 3] + c];
+            a0 = fma(s_val[q], x0, a0);
+            a1 = fma(s_val[q + 1], x1, a1);
+            a2 = fma(s_val[q + 2], x2, a2);
+            a3 = fma(s_val[q + 3], x3, a3);
+          }
+          for (; q < qe; ++q) a0 = fma(s_val[q], V[s_off[q] + c], a0);
+        } else {
+          for (int q = hb; q < he; ++q) {
+            const uint32_t e = p.up_packed[q];
+            a0 = fma(p.values[e >> kColBits], V[(size_t)(e & kColMask) * Ddn + c], a0);
+          }
         }
         const double acc = (a0 + a1) + (a2 + a3);
         const size_t o = (size_t)r * Ddn + c;
@@ -372,18 +475,22 @@ struct Launch {
   int ef;  // down-spin entry format (see hv_rows_kernel)
 };
 
-// Columns per thread J (smallest power of two <= 8 keeping rows near 256
-// threads) and threads per block T = 64 * ceil(ceil(Ddn / J) / 64) <= 1024;
-// longer rows take several column chunks.
+// Columns per thread J (smallest power of two <= 4 keeping rows near 256
+// threads) and threads per block T <= 1024, a multiple of 64; longer rows
+// take several column chunks of T * J columns.
 Launch plan(const DevSector& ds) {
   Launch L;
   const int64_t Ddn = ds.dn.D;
   const size_t rowb = (size_t)(Ddn + 1) * sizeof(double);
   L.lds = rowb <= lds_budget();
-  L.ef = (L.lds && ds.dn.ell_q4 != nullptr) ? (ds.dn.nvt16 <= 2 ? 2 : 1) : 0;
+  L.ef = 0;
+  if (L.lds && ds.dn.ell_s4 != nullptr) L.ef = 3;
+  else if (L.lds && ds.dn.ell_q4 != nullptr) L.ef = ds.dn.nvt16 <= 2 ? 2 : 1;
   int J = 1;
-  while (J < 8 && (Ddn + J - 1) / J > 256) J *= 2;
-  int64_t T = ((Ddn + J - 1) / J + 63) / 64 * 64;
+  while (J < 4 && (Ddn + J - 1) / J > 256) J *= 2;
+  // as few chunks as 1024 threads allow, split evenly so that few lanes idle
+  const int64_t nch = (Ddn + (int64_t)J * 1024 - 1) / ((int64_t)J * 1024);
+  int64_t T = ((Ddn + J * nch - 1) / (J * nch) + 63) / 64 * 64;
   T = std::min<int64_t>(std::max<int64_t>(T, 64), 1024);
   L.J = J;
   L.block = (int)T;
@@ -404,12 +511,14 @@ void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
 template <int J, int CROSS>
 void launch_c(const HvParams& p, const Launch& L, bool kan, bool up, hipStream_t s) {
   if (kan) {
-    if (L.ef == 2) launch_t<J, CROSS, true, true, 2>(p, L, up, s);
+    if (L.ef == 3) launch_t<J, CROSS, true, true, 3>(p, L, up, s);
+    else if (L.ef == 2) launch_t<J, CROSS, true, true, 2>(p, L, up, s);
     else if (L.ef == 1) launch_t<J, CROSS, true, true, 1>(p, L, up, s);
     else if (L.lds) launch_t<J, CROSS, true, true, 0>(p, L, up, s);
     else launch_t<J, CROSS, true, false, 0>(p, L, up, s);
   } else {
-    if (L.ef == 2) launch_t<J, CROSS, false, true, 2>(p, L, up, s);
+    if (L.ef == 3) launch_t<J, CROSS, false, true, 3>(p, L, up, s);
+    else if (L.ef == 2) launch_t<J, CROSS, false, true, 2>(p, L, up, s);
     else if (L.ef == 1) launch_t<J, CROSS, false, true, 1>(p, L, up, s);
     else if (L.lds) launch_t<J, CROSS, false, true, 0>(p, L, up, s);
     else launch_t<J, CROSS, false, false, 0>(p, L, up, s);
@@ -429,7 +538,7 @@ void launch_j(const HvParams& p, const Launch& L, int cross, bool kan, bool up, 
 // Slab pass geometry; returns its (persistent) grid size.
 int slab_geometry(const DevSector& ds, int& nslabs, int& rows_per_wg, int& nrb) {
   nslabs = (int)((ds.dn.D + kSlabW - 1) / kSlabW);
-  rows_per_wg = 32;
+  rows_per_wg = kSlabRows;
   nrb = (int)((ds.up.D + rows_per_wg - 1) / rows_per_wg);
   const int64_t items = (int64_t)nslabs * nrb;
   return (int)std::min<int64_t>(items, 2048);
@@ -454,6 +563,8 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   p.dn_ell = ds.dn.ell;
   p.dn_q4ptr = ds.dn.q4_ptr;
   p.dn_q4 = ds.dn.ell_q4;
+  p.dn_s4 = ds.dn.ell_s4;
+  p.dn_s4_scale = ds.dn.s4_scale;
   for (int q = 0; q < 4; ++q) p.dn_vt[q] = ds.dn.vt16[q];
   p.dn_rank = ds.dn.rank;
   p.values = ds.values;
@@ -475,6 +586,7 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   p.dscale = a.dscale;
   p.done = a.done;
   slab_geometry(ds, p.nslabs, p.rows_per_wg, p.nrb);
+  p.off32 = (uint64_t)ds.N < (uint64_t(1) << 32);
   return p;
 }
 
@@ -496,8 +608,7 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
   switch (L.J) {
     case 1: launch_j<1>(p, L, ds.cross_mode, kan, up, s); break;
     case 2: launch_j<2>(p, L, ds.cross_mode, kan, up, s); break;
-    case 4: launch_j<4>(p, L, ds.cross_mode, kan, up, s); break;
-    default: launch_j<8>(p, L, ds.cross_mode, kan, up, s); break;
+    default: launch_j<4>(p, L, ds.cross_mode, kan, up, s); break;
   }
   if (!up) {
     // the rows pass wrote the partial result; the slab pass adds the up-spin

@@ -411,6 +411,31 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
           }
     }
   }
+  {
+    std::vector<double> vt;
+    for (size_t q = 0; q < sp.val.size() && vt.size() <= 2; ++q)
+      if (std::find(vt.begin(), vt.end(), sp.val[q]) == vt.end()) vt.push_back(sp.val[q]);
+    const bool ok = !vt.empty() && (vt.size() == 1 || (vt.size() == 2 && vt[1] == -vt[0])) &&
+                    sp.D < (int64_t(1) << 28);
+    if (ok) {
+      const int64_t nsl = (int64_t)sp.sl_len.size();
+      std::vector<int32_t> qp(nsl + 1, 0);
+      for (int64_t sl = 0; sl < nsl; ++sl) qp[sl + 1] = qp[sl] + (sp.sl_len[sl] + 3) / 4 * kSlice;
+      if (sp.q4_ptr.empty()) sp.q4_ptr = qp;
+      sp.s4_scale = vt[0];
+      sp.ell_s4.assign((size_t)qp[nsl] * 4, (uint32_t)(sp.D * 8));
+      for (int64_t sl = 0; sl < nsl; ++sl)
+        for (int k = 0; k < sp.sl_len[sl]; ++k)
+          for (int L = 0; L < kSlice; ++L) {
+            const size_t q = sp.sl_ptr[sl] + (size_t)k * kSlice + L;
+            if (sp.ell_pad[q]) continue;
+            const uint32_t e = sp.ell[q];
+            const double v = values[e >> kColBits];
+            sp.ell_s4[((size_t)qp[sl] + (k / 4) * kSlice + L) * 4 + (k % 4)] =
+                ((e & kColMask) << 3) | (v == vt[0] ? 0u : 1u);
+          }
+    }
+  }
   if (need_rank) sp.rank = std::move(rank);
 }
 

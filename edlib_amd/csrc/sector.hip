@@ -32,6 +32,8 @@ void upload_spin(DevSector& ds, const SpinTables& sp, DevSpin& d, hipStream_t s)
   d.nvt16 = sp.nvt16;
   d.q4_ptr = upload(ds, sp.q4_ptr, s);
   d.ell_q4 = upload(ds, sp.ell_q4, s);
+  d.ell_s4 = upload(ds, sp.ell_s4, s);
+  d.s4_scale = sp.s4_scale;
   d.rank = upload(ds, sp.rank, s);
 }
 
