@@ -68,6 +68,24 @@ _SIGS = {
                                    ctypes.c_double, _dp, _dp, _dp, _dp]),
     "edh_eigsh": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_uint64,
                                  _dp, _dp, _ip, _ip]),
+    "edh_batch_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
+    "edh_batch_destroy": (ctypes.c_int, [_vp]),
+    "edh_batch_add_sector": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, _ip, _i64p]),
+    "edh_batch_remove_sector": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "edh_batch_submit_eig": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_uint64, _ip]),
+    "edh_batch_submit_lanczos": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _ip]),
+    "edh_batch_submit_gf": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _dp, _ip]),
+    "edh_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "edh_batch_job_info": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _i64p, _ip, _ip]),
+    "edh_batch_eig_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
+    "edh_batch_lanczos_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
+    "edh_batch_free_job": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "edh_batch_memory": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "edh_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "edh_batch_cf_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, _dp, _dp, _dp, _dp]),
 }
 
 

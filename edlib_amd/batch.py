@@ -1,0 +1,126 @@
+"""Batched engine (edh_batch_* in include/edhip.h) for numpy callers.
+
+A Batch holds sectors of any number of models and the Krylov processes on
+them (eigensolver runs and Green's-function Lanczos runs).  ``step()``
+advances every active process by one Krylov step per sweep with grouped
+kernel launches; processes can be submitted between sweeps.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import EDH_MEM_HOST, check
+
+JOB_ACTIVE, JOB_DONE, JOB_FAILED = 0, 1, 2
+JOB_EIG, JOB_LANCZOS = 0, 1
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Batch:
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        check(_lib.lib().edh_batch_create(ctypes.byref(self._h), device))
+        self.sector_key = {}  # sector id -> (model key, nup, ndn)
+
+    def close(self):
+        if self._h.value:
+            check(_lib.lib().edh_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and self._h.value and _lib._lib is not None:
+                _lib._lib.edh_batch_destroy(self._h)
+        except Exception:
+            pass
+
+    def add_sector(self, model, nup, ndn):
+        sid, dim = ctypes.c_int(), ctypes.c_int64()
+        check(_lib.lib().edh_batch_add_sector(self._h, model.handle, nup, ndn, ctypes.byref(sid), ctypes.byref(dim)))
+        return sid.value, dim.value
+
+    def remove_sector(self, sid):
+        check(_lib.lib().edh_batch_remove_sector(self._h, sid))
+
+    def submit_eig(self, sid, nev, ncv, tol=1e-14, maxitr=1000, seed=20261202):
+        jid = ctypes.c_int()
+        check(_lib.lib().edh_batch_submit_eig(self._h, sid, nev, ncv, tol, maxitr, seed, ctypes.byref(jid)))
+        return jid.value
+
+    def submit_lanczos(self, sid, v0, nl):
+        v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        jid = ctypes.c_int()
+        check(_lib.lib().edh_batch_submit_lanczos(self._h, sid, v0.ctypes.data_as(ctypes.c_void_p), nl, EDH_MEM_HOST,
+                                                  ctypes.byref(jid)))
+        return jid.value
+
+    def submit_gf(self, eig_job, index, orb, annihilate, dst_sid, nl):
+        """Start vector c(+)_orb |psi_index> in dst_sid and its Lanczos run.
+        Returns (job id or -1 when the vector vanishes, squared norm)."""
+        norm, jid = ctypes.c_double(), ctypes.c_int()
+        check(_lib.lib().edh_batch_submit_gf(self._h, eig_job, index, orb, 1 if annihilate else 0, dst_sid, nl,
+                                             ctypes.byref(norm), ctypes.byref(jid)))
+        return jid.value, norm.value
+
+    def step(self, nsweeps=1):
+        act = ctypes.c_int()
+        check(_lib.lib().edh_batch_step(self._h, nsweeps, ctypes.byref(act)))
+        return act.value
+
+    def run(self, chunk=64):
+        """Sweeps until no process is active; returns the number of sweep calls."""
+        calls = 0
+        while self.step(chunk) > 0:
+            calls += 1
+        return calls
+
+    def job_info(self, jid):
+        st, kind, cnt, nav = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        n = ctypes.c_int64()
+        check(_lib.lib().edh_batch_job_info(self._h, jid, ctypes.byref(st), ctypes.byref(kind), ctypes.byref(n),
+                                            ctypes.byref(cnt), ctypes.byref(nav)))
+        return {"state": st.value, "kind": kind.value, "n": n.value, "count": cnt.value, "n_av": nav.value}
+
+    def eig_result(self, jid, vectors=False):
+        info = self.job_info(jid)
+        nev, n = info["count"], info["n"]
+        ev = np.zeros(max(nev, 1))
+        vecs = np.zeros((nev, n)) if (vectors and nev > 0) else None
+        nconv = ctypes.c_int()
+        check(_lib.lib().edh_batch_eig_result(self._h, jid, _d(ev), _d(vecs) if vecs is not None else None,
+                                              ctypes.byref(nconv)))
+        return ev[:nev], (vecs.T.copy() if vecs is not None else None)
+
+    def lanczos_result(self, jid, nl):
+        a, b = np.zeros(nl), np.zeros(nl + 1)
+        nlanc = ctypes.c_int()
+        check(_lib.lib().edh_batch_lanczos_result(self._h, jid, _d(a), _d(b), ctypes.byref(nlanc)))
+        return nlanc.value, a, b
+
+    def free_job(self, jid):
+        check(_lib.lib().edh_batch_free_job(self._h, jid))
+
+    def memory(self):
+        b = ctypes.c_size_t()
+        check(_lib.lib().edh_batch_memory(self._h, ctypes.byref(b)))
+        return b.value
+
+    def autotune(self, sids):
+        ids = np.ascontiguousarray(sids, dtype=np.int32)
+        blk = ctypes.c_int()
+        check(_lib.lib().edh_batch_autotune(self._h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            ctypes.byref(blk)))
+        return blk.value
+
+    def cf_eval(self, z, ev, nlanc, isign, expb, alpha, beta):
+        z = np.asarray(z, dtype=np.complex128).reshape(-1)
+        zr, zi = np.ascontiguousarray(z.real), np.ascontiguousarray(z.imag)
+        a, b = np.ascontiguousarray(alpha, dtype=np.float64), np.ascontiguousarray(beta, dtype=np.float64)
+        outr, outi = np.zeros(len(z)), np.zeros(len(z))
+        check(_lib.lib().edh_batch_cf_eval(self._h, len(z), _d(zr), _d(zi), float(ev), int(nlanc), int(isign),
+                                           float(expb), _d(a), _d(b), _d(outr), _d(outi)))
+        return outr + 1j * outi

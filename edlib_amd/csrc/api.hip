@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "batch.h"
 #include "edh_ctx.h"
 
 struct edh_model {
@@ -10,6 +11,9 @@ struct edh_model {
 };
 struct edh_ctx {
   edh::Ctx c;
+};
+struct edh_batch {
+  edh::Batch b;
 };
 
 namespace {
@@ -114,6 +118,13 @@ void adag_maps(int Ns, int nup, int ndn, int orb, bool annihilate, int& nup2, in
 }
 
 }  // namespace
+
+void adag_gather_maps(int Ns, int nup, int ndn, int orb, bool annihilate, int& nup2, int& ndn2,
+                      std::vector<int32_t>& mu, std::vector<int32_t>& md, std::vector<int8_t>& su,
+                      std::vector<int8_t>& sd) {
+  adag_maps(Ns, nup, ndn, orb, annihilate, nup2, ndn2, mu, md, su, sd);
+}
+
 }  // namespace edh
 
 using namespace edh;
@@ -634,6 +645,184 @@ int edh_eigsh(edh_ctx* ctx, int nev, int ncv, double tol, int maxitr, uint64_t s
   use_device(c);
   EDH_CHECK(evals && nconv && n_av && nev >= 1 && ncv >= 2, EDH_ERR_ARG, "bad argument");
   eigsh_run(c, nev, ncv, tol, maxitr, seed, evals, evecs, nconv, n_av);
+  API_END
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched engine
+// ---------------------------------------------------------------------------
+int edh_batch_create(edh_batch** out, int device) {
+  API_BEGIN
+  EDH_CHECK(out, EDH_ERR_ARG, "null argument");
+  int ndev = 0;
+  EDH_HIP(hipGetDeviceCount(&ndev));
+  EDH_CHECK(device >= 0 && device < ndev, EDH_ERR_ARG, "device index out of range");
+  EDH_HIP(hipSetDevice(device));
+  std::unique_ptr<edh_batch> b(new edh_batch());
+  b->b.device = device;
+  EDH_HIP(hipStreamCreate(&b->b.stream));
+  *out = b.release();
+  API_END
+}
+
+int edh_batch_destroy(edh_batch* b) {
+  delete b;
+  return EDH_OK;
+}
+
+namespace {
+edh::Batch& use_batch(edh_batch* b) {
+  EDH_CHECK(b != nullptr, EDH_ERR_ARG, "null batch");
+  EDH_HIP(hipSetDevice(b->b.device));
+  return b->b;
+}
+}  // namespace
+
+int edh_batch_add_sector(edh_batch* b, const edh_model* m, int nup, int ndn, int* sec_id, int64_t* dim) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(m && sec_id, EDH_ERR_ARG, "null argument");
+  *sec_id = B.add_sector(*m->m, nup, ndn);
+  if (dim) *dim = B.sector(*sec_id).tabs.N;
+  API_END
+}
+
+int edh_batch_remove_sector(edh_batch* b, int sec_id) {
+  API_BEGIN
+  use_batch(b).remove_sector(sec_id);
+  API_END
+}
+
+int edh_batch_submit_eig(edh_batch* b, int sec_id, int nev, int ncv, double tol, int maxitr, uint64_t seed,
+                         int* job_id) {
+  API_BEGIN
+  EDH_CHECK(job_id, EDH_ERR_ARG, "null argument");
+  *job_id = use_batch(b).submit_eig(sec_id, nev, ncv, tol, maxitr, seed);
+  API_END
+}
+
+int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl, int mem, int* job_id) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(v0 && job_id, EDH_ERR_ARG, "null argument");
+  const int64_t n = B.sector(sec_id).tabs.N;
+  const double* src = v0;
+  if (mem == EDH_MEM_HOST) {
+    B.gtab.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(double));
+    EDH_HIP(hipMemcpyAsync(B.gtab.p, v0, (size_t)n * sizeof(double), hipMemcpyHostToDevice, B.stream));
+    src = B.gtab.as<double>();
+  }
+  *job_id = B.submit_lanczos(sec_id, src, nl);
+  EDH_HIP(hipStreamSynchronize(B.stream));
+  API_END
+}
+
+int edh_batch_submit_gf(edh_batch* b, int eig_job, int index, int orb, int annihilate, int dst_sec, int nl,
+                        double* norm, int* job_id) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(norm && job_id, EDH_ERR_ARG, "null argument");
+  edh::BJob& e = B.job(eig_job);
+  EDH_CHECK(e.kind == edh::JOB_EIG && e.state == edh::JOB_DONE && e.evecs, EDH_ERR_STATE,
+            "batch: eigensolver job has no eigenvectors");
+  EDH_CHECK(index >= 0 && index < e.nev, EDH_ERR_ARG, "batch: eigenpair index out of range");
+  *job_id = B.submit_gf(e.sec, e.evecs + (size_t)index * e.n, orb, annihilate != 0, dst_sec, nl, norm);
+  API_END
+}
+
+int edh_batch_step(edh_batch* b, int nsweeps, int* n_active) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  int act = 0;
+  for (int i = 0; i < std::max(nsweeps, 1); ++i) {
+    act = B.step();
+    if (act == 0) break;
+  }
+  if (n_active) *n_active = act;
+  API_END
+}
+
+int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t* n, int* count, int* n_av) {
+  API_BEGIN
+  edh::BJob& j = use_batch(b).job(job_id);
+  if (state) *state = j.state;
+  if (kind) *kind = j.kind;
+  if (n) *n = j.n;
+  if (count) *count = j.kind == edh::JOB_EIG ? j.nev : j.nlanc;
+  if (n_av) *n_av = j.kind == edh::JOB_EIG ? j.n_av : j.nlanc;
+  API_END
+}
+
+int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs, int* nconv) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  edh::BJob& j = B.job(job_id);
+  EDH_CHECK(j.kind == edh::JOB_EIG && j.state != edh::JOB_ACTIVE, EDH_ERR_STATE, "batch: no eigensolver result");
+  if (nconv) *nconv = j.nconv;
+  if (evals)
+    for (int q = 0; q < (int)j.evals.size(); ++q) evals[q] = j.evals[q];
+  if (evecs && j.evecs && j.nev > 0) {
+    EDH_HIP(hipMemcpyAsync(evecs, j.evecs, (size_t)j.nev * j.n * sizeof(double), hipMemcpyDeviceToHost, B.stream));
+    EDH_HIP(hipStreamSynchronize(B.stream));
+  }
+  EDH_CHECK(j.state == edh::JOB_DONE, EDH_ERR_NOCONV, "batch: eigensolver did not converge");
+  API_END
+}
+
+int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc) {
+  API_BEGIN
+  edh::BJob& j = use_batch(b).job(job_id);
+  EDH_CHECK(j.kind == edh::JOB_LZ && j.state == edh::JOB_DONE, EDH_ERR_STATE, "batch: no Lanczos result");
+  if (nlanc) *nlanc = j.nlanc;
+  if (alpha) std::copy(j.alpha.begin(), j.alpha.end(), alpha);
+  if (beta) std::copy(j.beta.begin(), j.beta.end(), beta);
+  API_END
+}
+
+int edh_batch_free_job(edh_batch* b, int job_id) {
+  API_BEGIN
+  use_batch(b).free_job(job_id);
+  API_END
+}
+
+int edh_batch_memory(edh_batch* b, size_t* bytes) {
+  API_BEGIN
+  EDH_CHECK(bytes, EDH_ERR_ARG, "null argument");
+  *bytes = use_batch(b).bytes_in_use;
+  API_END
+}
+
+int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(nsec >= 0 && (nsec == 0 || sec_ids), EDH_ERR_ARG, "bad argument");
+  std::vector<int> ids(sec_ids, sec_ids + nsec);
+  const int r = B.autotune(ids);
+  if (block) *block = r;
+  API_END
+}
+
+int edh_batch_cf_eval(edh_batch* b, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign,
+                      double expb, const double* alpha, const double* beta, double* out_re, double* out_im) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(nz >= 0 && nlanc >= 1 && zre && zim && alpha && beta && out_re && out_im, EDH_ERR_ARG, "bad argument");
+  B.d_scal.ensure((size_t)(4 * nz + 2 * nlanc + 2) * sizeof(double));
+  double* dz = B.d_scal.as<double>();
+  double* dzi = dz + nz;
+  double* da = dzi + nz;
+  double* db = da + nlanc;
+  double* dr = db + nlanc + 1;
+  double* di = dr + nz;
+  EDH_HIP(hipMemcpyAsync(dz, zre, nz * 8, hipMemcpyHostToDevice, B.stream));
+  EDH_HIP(hipMemcpyAsync(dzi, zim, nz * 8, hipMemcpyHostToDevice, B.stream));
+  EDH_HIP(hipMemcpyAsync(da, alpha, nlanc * 8, hipMemcpyHostToDevice, B.stream));
+  EDH_HIP(hipMemcpyAsync(db, beta, (nlanc + 1) * 8, hipMemcpyHostToDevice, B.stream));
+  dev_cf_eval(nz, dz, dzi, ev, nlanc, isign, expb, da, db, dr, di, B.stream);
+  EDH_HIP(hipMemcpyAsync(out_re, dr, nz * 8, hipMemcpyDeviceToHost, B.stream));
+  EDH_HIP(hipMemcpyAsync(out_im, di, nz * 8, hipMemcpyDeviceToHost, B.stream));
+  EDH_HIP(hipStreamSynchronize(B.stream));
   API_END
 }
 

@@ -50,10 +50,24 @@ struct Ctx {
   ~Ctx();
 };
 
+// Ritz vectors kept by a thick restart of an m-vector basis with nev
+// wanted pairs of which nconv have converged.
+int restart_size(int nev, int m, int nconv);
+
 void eigsh_run(Ctx& c, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals, double* evecs,
                int* nconv, int* n_av);
 
 // Diagonal element of state (u, d) of the current sector from the host tables.
 double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d);
+
+// Dense sector matrix (row = source state) from the host tables.
+void dense_sector_matrix(const Model& m, const SectorTables& st, std::vector<double>& H);
+
+// Gather maps of c (annihilate) or c^+ on orbital orb from sector (nup, ndn)
+// into (nup2, ndn2): target (u', d') reads source (mu[u'], md[d']) times
+// su[u'] sd[d'] (index -1 = zero).
+void adag_gather_maps(int Ns, int nup, int ndn, int orb, bool annihilate, int& nup2, int& ndn2,
+                      std::vector<int32_t>& mu, std::vector<int32_t>& md, std::vector<int8_t>& su,
+                      std::vector<int8_t>& sd);
 
 }  // namespace edh

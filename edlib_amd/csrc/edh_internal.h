@@ -218,4 +218,13 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant 
 int hv_num_blocks(const DevSector& ds);
 int hv_autotune(DevSector& ds, hipStream_t s);
 
+// Grouped H*v over many operands (sectors of any models): one launch covers
+// the concatenated up-rows of all of them.  A job record is filled on the
+// host (hv_batch_fill, row0 = first global row of the operand) into a
+// buffer of hv_batch_job_bytes() per job, then copied to the device.
+size_t hv_batch_job_bytes();
+void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out);
+void launch_hv_batch(const void* djobs, int njobs, int64_t total_rows, int64_t max_ddn, int block, int grid,
+                     hipStream_t s);
+
 }  // namespace edh

@@ -5,6 +5,7 @@
 // reference: ncv = min(NCV, ntot), nev = min(NEV, ncv-1), which = "SA",
 // convergence when the Ritz residual bound <= tol * max(|theta|, eps^(2/3)).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 
@@ -131,11 +132,9 @@ double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d) {
   return x;
 }
 
-namespace {
-
 // Dense sector matrix from the host tables, row = source state (the
 // storages' convention: w[i] += H[i][j] v[j]).
-void dense_sector(const Model& m, const SectorTables& st, std::vector<double>& H) {
+void dense_sector_matrix(const Model& m, const SectorTables& st, std::vector<double>& H) {
   const int64_t Du = st.up.D, Dd = st.dn.D, n = st.N;
   H.assign(n * n, 0.0);
   for (int64_t u = 0; u < Du; ++u)
@@ -158,6 +157,8 @@ void dense_sector(const Model& m, const SectorTables& st, std::vector<double>& H
     }
 }
 
+namespace {
+
 uint64_t splitmix64(uint64_t& x) {
   uint64_t z = (x += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -166,6 +167,14 @@ uint64_t splitmix64(uint64_t& x) {
 }
 
 }  // namespace
+
+// ARPACK's dsaup2 rule: keep the nev wanted Ritz vectors plus one more per
+// converged pair, at most half of the new space (this is what its implicit
+// restart with exact shifts retains).
+int restart_size(int nev, int m, int nconv) {
+  const int kk = nev + std::min(nconv, (m - nev) / 2);
+  return std::max(1, std::min(kk, m - 2));
+}
 
 void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_t seed, double* evals,
                double* evecs_host, int* nconv, int* n_av) {
@@ -193,7 +202,7 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
   if (ncv >= n || n <= 64) {
     // The Krylov space is the whole sector: exact dense solution.
     std::vector<double> H, w;
-    dense_sector(*c.model, c.tabs, H);
+    dense_sector_matrix(*c.model, c.tabs, H);
     std::vector<double> Hs(H.size());
     for (int64_t i = 0; i < n; ++i)
       for (int64_t j = 0; j < n; ++j) Hs[i * n + j] = 0.5 * (H[i * n + j] + H[j * n + i]);
@@ -269,11 +278,12 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
     std::vector<double> A = T;
     sym_eig(m, A, theta);
     Y = A;  // columns = eigenvectors of T
-    bool conv = true;
+    int nconv_now = 0;
     for (int i = 0; i < nev; ++i) {
       const double bound = std::fabs(bres * Y[(size_t)(m - 1) * m + i]);
-      if (bound > tol * std::max(std::fabs(theta[i]), eps23)) conv = false;
+      if (bound <= tol * std::max(std::fabs(theta[i]), eps23)) ++nconv_now;
     }
+    const bool conv = nconv_now == nev;
     if (conv || iter + 1 >= maxitr) {
       std::vector<double> Cm((size_t)m * nev);
       for (int j = 0; j < nev; ++j)
@@ -291,7 +301,7 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
       return;
     }
     // thick restart: keep kk lowest Ritz vectors, continue from the residual
-    const int kk = std::min(std::max(nev + (m - nev) / 2, nev + 1), m - 2);
+    const int kk = restart_size(nev, m, nconv_now);
     std::vector<double> Cm((size_t)m * kk);
     for (int j = 0; j < kk; ++j)
       for (int i = 0; i < m; ++i) Cm[(size_t)j * m + i] = Y[(size_t)i * m + j];

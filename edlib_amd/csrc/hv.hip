@@ -26,6 +26,7 @@
 // neighbour row segment is re-read from it.
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 
 #include "edh_internal.h"
 
@@ -452,6 +453,87 @@ __global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* cou
   if (p.dot_partial) block_reduce_store(dsum, s_red, p.dot_partial + blockIdx.x);
 }
 
+
+// ---------------------------------------------------------------------------
+// Grouped H*v: every operand of a batch in one launch.  Work-group slot g is
+// global up-row g of the concatenation of all operands; the operand is found
+// by binary search over the jobs' first rows.  A row is staged in LDS when
+// the largest down dimension of the batch fits, and each thread takes
+// columns tid, tid + T, ...; the down-spin hopping reads the 32-bit sliced
+// ELL (present for every sector), so any model and value set is handled.
+struct HvBatchJob {
+  HvParams p;
+  int64_t row0;
+  int cross, kan;
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(512) void hv_batch_kernel(const HvBatchJob* __restrict__ jobs, int njobs,
+                                                        int64_t total_rows) {
+  extern __shared__ double s_v[];
+  const int tid = threadIdx.x;
+  const int T = blockDim.x;
+  for (int64_t g = blockIdx.x; g < total_rows; g += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].row0 <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const HvBatchJob& J = jobs[lo];
+    const HvParams& p = J.p;
+    const int u = (int)(g - J.row0);
+    const int Ddn = p.Ddn;
+    const double vscale = p.vscale, wscale = p.wscale;
+    const double* __restrict__ V = p.v;
+    const size_t rowoff = (size_t)u * Ddn;
+    if (LDS) {
+      for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
+      __syncthreads();
+    }
+    const int bu = p.up_bits[u];
+    const double eu = p.up_e[u];
+    const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+    const int na = 1 << (p.Ns - p.nb);
+    const unsigned lowmask = (1u << p.nb) - 1u;
+    const int cross = J.cross;
+    for (int d = tid; d < Ddn; d += T) {
+      const int bd = p.dn_bits[d];
+      const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
+      double x;
+      if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
+      else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
+      else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
+      else x = 0.0;
+      double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
+      acc = fma(eu + p.dn_e[d] + x, vd, acc);
+      const int sl = d >> 6;
+      const int len = p.dn_sllen[sl];
+      const int base = p.dn_slptr[sl] + (d & 63);
+      for (int k = 0; k < len; ++k) {
+        const uint32_t e = p.dn_ell[base + k * kSlice];
+        const int c = (int)(e & kColMask);
+        acc = fma(p.values[e >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc);
+      }
+      for (int q = hb; q < he; ++q) {
+        const uint32_t e = p.up_packed[q];
+        acc = fma(p.values[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
+      }
+      if (J.kan) {
+        const int a = bu >> p.nb, b = bd >> p.nb;
+        const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+        for (int q = kb; q < ke; ++q) {
+          const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
+          const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
+          acc = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
+        }
+      }
+      p.w[rowoff + d] = acc;
+    }
+    if (LDS) __syncthreads();
+  }
+}
+
 namespace {
 
 // Largest dynamic LDS a block may use on the current device (queried once per device).
@@ -626,7 +708,7 @@ void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant)
 int hv_autotune(DevSector& ds, hipStream_t s) {
   const char* env = getenv("EDH_HV_VARIANT");
   if (env && (env[0] == '0' || env[0] == '1')) return env[0] - '0';
-  if (ds.N < (1 << 16)) return HV_SINGLE;  // launch-bound sizes: one kernel
+  if (ds.N < (1 << 22)) return HV_SINGLE;  // short rows: the slab pass never pays
   double *v = nullptr, *w = nullptr;
   if (hipMalloc(&v, ds.N * sizeof(double)) != hipSuccess) return HV_SINGLE;
   if (hipMalloc(&w, ds.N * sizeof(double)) != hipSuccess) {
@@ -666,6 +748,38 @@ int hv_autotune(DevSector& ds, hipStream_t s) {
   (void)hipFree(v);
   (void)hipFree(w);
   return choice;
+}
+
+}  // namespace edh
+
+namespace edh {
+
+size_t hv_batch_job_bytes() { return sizeof(HvBatchJob); }
+
+void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out) {
+  HvBatchJob j;
+  j.p = make_params(ds, a);
+  j.row0 = row0;
+  j.cross = ds.cross_mode;
+  j.kan = ds.kan_ml > 0 ? 1 : 0;
+  std::memcpy(out, &j, sizeof(j));
+}
+
+void launch_hv_batch(const void* djobs, int njobs, int64_t total_rows, int64_t max_ddn, int block, int grid,
+                     hipStream_t s) {
+  if (njobs <= 0 || total_rows <= 0) return;
+  EDH_CHECK(block >= 64 && block <= 512 && block % 64 == 0, EDH_ERR_ARG, "grouped H*v: bad block size");
+  const size_t smem = (size_t)max_ddn * sizeof(double);
+  const HvBatchJob* jobs = (const HvBatchJob*)djobs;
+  if (smem <= lds_budget()) {
+    if (smem > 64 * 1024)
+      EDH_HIP(hipFuncSetAttribute((const void*)hv_batch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)smem));
+    hipLaunchKernelGGL(hv_batch_kernel<true>, dim3(grid), dim3(block), smem, s, jobs, njobs, total_rows);
+  } else {
+    hipLaunchKernelGGL(hv_batch_kernel<false>, dim3(grid), dim3(block), 0, s, jobs, njobs, total_rows);
+  }
+  EDH_HIP(hipGetLastError());
 }
 
 }  // namespace edh

@@ -1,0 +1,234 @@
+"""Impurity solves: eigenpairs of every sector and the local Green's function.
+
+Both drivers follow Hamiltonian::diag (Hamiltonian.h:54-101) and
+GreensFunction::compute (GreensFunction.h:109-159, 241-282, 389-462) for the
+local Green's function of the listed orbitals:
+
+* every (N_up, N_dn) sector: lowest NEV eigenpairs (ncv = NCV or 2*NEV+3,
+  tol, maxitr, which = "SA"; Storage.h:43-146);
+* the eigenpairs ordered by energy, ties in sector order (EigenPair.h:35-41);
+  E0 is the first, Z = sum_k exp(-beta (E_k - E0)) over every stored pair;
+* for every pair whose Boltzmann factor passes the cutoff
+  (GreensFunction.h:135), every listed orbital o and spin s:
+  c+_{o s}|psi> contributes to G_g with isign +1 and c_{o s}|psi> to G_l
+  with isign -1; each start vector is normalised, its squared norm is the
+  weight, and a Lanczos run of NLANC steps feeds the continued fraction at
+  z = i w_n + isign * E (Lanczos.h:82-100, 146-148, 201-224);
+* G = (G_g + G_l) / Z on the fermionic Matsubara mesh w_n = (2n+1) pi / beta.
+
+``solve_plain`` runs one sector, then one start vector, at a time on a
+Context -- what the drop-in storage does inside Hamiltonian and
+GreensFunction.  ``solve_batched`` advances every sector's eigensolver run
+together on a Batch, then every start vector's Lanczos run together.
+
+Sector order: sectors are listed by (dimension, N_up, N_dn).  The reference's
+SzSymmetry order (a std::sort with a non-strict comparator) only decides
+which of several exactly degenerate pairs comes first, which changes no
+result beyond rounding.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import Context, binom
+from .batch import JOB_DONE, Batch
+
+
+@dataclass
+class SolveParams:
+    nev: int = 4
+    ncv: int = 0            # 0: 2 * nev + 3
+    tol: float = 1e-14
+    maxitr: int = 1000
+    beta: float = 10.0
+    nomega: int = 32
+    nlanc: int = 100
+    cutoff: float = 1e-12
+    orbitals: tuple = (0,)
+    seed: int = 20261202
+    sectors: list = field(default_factory=list)  # empty: every (N_up, N_dn)
+
+    @property
+    def ncv_eff(self):
+        return self.ncv if self.ncv > 0 else 2 * self.nev + 3
+
+
+def sector_list(Ns, p):
+    secs = list(p.sectors) if p.sectors else [(i, j) for i in range(Ns + 1) for j in range(Ns + 1)]
+    return sorted(secs, key=lambda s: (binom(Ns, s[0]) * binom(Ns, s[1]), s[0], s[1]))
+
+
+def matsubara(beta, nomega):
+    return (2 * np.arange(nomega) + 1) * math.pi / beta
+
+
+def _expb(beta, e, e0):
+    return 0.0 if beta * (e - e0) > 25 else math.exp(-beta * (e - e0))
+
+
+def _passes(bf, cutoff):
+    return not (abs(cutoff - bf) > np.finfo(float).eps and bf < cutoff)
+
+
+def _order_pairs(pairs):
+    # stable sort by energy: ties keep the insertion (sector, index) order
+    return sorted(pairs, key=lambda q: q[0])
+
+
+def _gf_targets(Ns, sector, orbitals):
+    """(orbital, spin, annihilate, next sector) of every start vector of one pair."""
+    out = []
+    for o in orbitals:
+        for s in (0, 1):
+            for ann in (False, True):
+                n = sector[s]
+                if (ann and n == 0) or ((not ann) and n >= Ns):
+                    continue
+                d = -1 if ann else 1
+                nxt = (sector[0] + d, sector[1]) if s == 0 else (sector[0], sector[1] + d)
+                out.append((o, s, ann, nxt))
+    return out
+
+
+def solve_plain(model, p, device=0, ctx=None):
+    """One sector and one start vector at a time (the drop-in path)."""
+    Ns = model.Ns
+    own = ctx is None
+    ctx = Context(device) if own else ctx
+    t0 = time.perf_counter()
+    ctx.set_model(model)
+    secs = sector_list(Ns, p)
+    pairs = []
+    n_av = 0
+    for i, sec in enumerate(secs):
+        if binom(Ns, sec[0]) * binom(Ns, sec[1]) == 0:
+            continue
+        ctx.set_sector(*sec)
+        ev, vecs, nav = ctx.eigsh(p.nev, p.ncv_eff, p.tol, p.maxitr, p.seed + i, vectors=True)
+        n_av += nav
+        for k in range(len(ev)):
+            pairs.append((float(ev[k]), sec, vecs[:, k].copy()))
+    t_eig = time.perf_counter() - t0
+    pairs = _order_pairs(pairs)
+    e0 = pairs[0][0]
+    Z = sum(math.exp(-(e - e0) * p.beta) for e, _, _ in pairs)
+    wn = matsubara(p.beta, p.nomega)
+    norb = len(p.orbitals)
+    Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
+    Gl = np.zeros_like(Gg)
+    n_gf = 0
+    for e, sec, vec in pairs:
+        bf = math.exp(-(e - e0) * p.beta)
+        if not _passes(bf, p.cutoff):
+            continue
+        for q, (o, s, ann, nxt) in enumerate(_gf_targets(Ns, sec, p.orbitals)):
+            _, out = ctx.adag(sec, o + s * Ns, ann, vec)
+            norm = ctx.vv(out, out)
+            if not abs(norm) > 1e-10:
+                continue
+            out /= math.sqrt(norm)
+            ctx.set_sector(*nxt)
+            nlanc, a, b = ctx.lanczos(out, p.nlanc)
+            isign = -1 if ann else 1
+            g = ctx.cf_eval(1j * wn + e * isign, norm, nlanc, isign, _expb(p.beta, e, e0), a, b)
+            (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g
+            n_gf += 1
+    G = (Gg + Gl) / Z
+    t_all = time.perf_counter() - t0
+    if own:
+        ctx.close()
+    return {"E0": e0, "Z": Z, "G": G, "energies": np.array([q[0] for q in pairs]), "n_gf": n_gf, "n_av": n_av,
+            "t_eig": t_eig, "t_total": t_all}
+
+
+class BatchedSolve:
+    """One impurity solve on a shared Batch, driven phase by phase so that
+    several solves can share the batch's sweeps (see serve.py)."""
+
+    def __init__(self, batch, model, p):
+        self.b = batch
+        self.model = model
+        self.p = p
+        self.Ns = model.Ns
+        self.secs = [s for s in sector_list(self.Ns, p) if binom(self.Ns, s[0]) * binom(self.Ns, s[1]) > 0]
+        self.sid = {}
+        self.eig_jobs = []
+        self.gf_jobs = []
+        self.phase = "new"
+        self.result = None
+
+    def start(self):
+        for sec in self.secs:
+            self.sid[sec], _ = self.b.add_sector(self.model, *sec)
+        for i, sec in enumerate(sector_list(self.Ns, self.p)):
+            if sec in self.sid:
+                self.eig_jobs.append((sec, self.b.submit_eig(self.sid[sec], self.p.nev, self.p.ncv_eff, self.p.tol,
+                                                             self.p.maxitr, self.p.seed + i)))
+        self.phase = "eig"
+
+    def _all_done(self, jobs):
+        return all(self.b.job_info(j)["state"] != 0 for j in jobs)
+
+    def advance(self):
+        """Moves to the next phase when the current one has finished;
+        returns True when the solve is complete."""
+        p = self.p
+        if self.phase == "eig" and self._all_done([j for _, j in self.eig_jobs]):
+            pairs = []
+            for sec, j in self.eig_jobs:
+                ev, _ = self.b.eig_result(j)
+                for k in range(len(ev)):
+                    pairs.append((float(ev[k]), sec, j, k))
+            self.pairs = _order_pairs(pairs)
+            self.e0 = self.pairs[0][0]
+            self.Z = sum(math.exp(-(e - self.e0) * p.beta) for e, _, _, _ in self.pairs)
+            for e, sec, j, k in self.pairs:
+                bf = math.exp(-(e - self.e0) * p.beta)
+                if not _passes(bf, p.cutoff):
+                    continue
+                for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
+                    jid, norm = self.b.submit_gf(j, k, o + s * self.Ns, ann, self.sid[nxt], p.nlanc)
+                    if jid >= 0:
+                        self.gf_jobs.append((jid, norm, e, o, s, ann))
+            self.phase = "gf"
+        if self.phase == "gf" and self._all_done([j[0] for j in self.gf_jobs]):
+            wn = matsubara(p.beta, p.nomega)
+            norb = len(p.orbitals)
+            Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
+            Gl = np.zeros_like(Gg)
+            for jid, norm, e, o, s, ann in self.gf_jobs:
+                nlanc, a, bb = self.b.lanczos_result(jid, p.nlanc)
+                isign = -1 if ann else 1
+                g = self.b.cf_eval(1j * wn + e * isign, norm, nlanc, isign, _expb(p.beta, e, self.e0), a, bb)
+                (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g
+                self.b.free_job(jid)
+            for _, j in self.eig_jobs:
+                self.b.free_job(j)
+            for sid in self.sid.values():
+                self.b.remove_sector(sid)
+            self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
+                           "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(self.gf_jobs)}
+            self.phase = "done"
+        return self.phase == "done"
+
+
+def solve_batched(model, p, device=0, batch=None, autotune=True):
+    """Every sector's eigensolver run advanced together, then every start vector."""
+    own = batch is None
+    b = Batch(device) if own else batch
+    t0 = time.perf_counter()
+    s = BatchedSolve(b, model, p)
+    s.start()
+    if autotune:
+        b.autotune(list(s.sid.values()))
+    sweeps = 0
+    while not s.advance():
+        b.step(16)
+        sweeps += 1
+    out = dict(s.result)
+    out["t_total"] = time.perf_counter() - t0
+    if own:
+        b.close()
+    return out

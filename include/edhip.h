@@ -166,6 +166,64 @@ int edh_cf_eval(edh_ctx* ctx, int nz, const double* zre, const double* zim, doub
 int edh_eigsh(edh_ctx* ctx, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals,
               double* evecs, int* nconv, int* n_av);
 
+/* ------------------------------------------------------------------------
+ * Batched engine.  A batch holds sectors of any number of models and many
+ * Krylov processes on them: eigensolver runs (edh_eigsh's algorithm) and
+ * Green's-function Lanczos runs (edh_lanczos's recurrence).  edh_batch_step
+ * advances every active process by one Krylov step per sweep, each phase of
+ * the sweep one grouped kernel launch over all processes; processes may be
+ * submitted between sweeps (continuous batching).  A process's arithmetic
+ * depends only on its own operands, so its result is the same alone and in
+ * any mix.  These entries stand in for the per-sector loops of
+ * Hamiltonian::diag (Hamiltonian.h:54-101) and GreensFunction::compute
+ * (GreensFunction.h:109-159), which run one sector / one start vector at a
+ * time in the reference.
+ * ---------------------------------------------------------------------- */
+typedef struct edh_batch edh_batch;
+
+#define EDH_JOB_ACTIVE 0
+#define EDH_JOB_DONE 1
+#define EDH_JOB_FAILED 2
+#define EDH_JOB_EIG 0
+#define EDH_JOB_LANCZOS 1
+
+int edh_batch_create(edh_batch** out, int device);
+int edh_batch_destroy(edh_batch* b);
+/* Builds the tables of sector (nup, ndn) of model m on the device
+ * (SpinResolvedStorage::fill for that sector); returns its id and dimension. */
+int edh_batch_add_sector(edh_batch* b, const edh_model* m, int nup, int ndn, int* sec_id, int64_t* dim);
+int edh_batch_remove_sector(edh_batch* b, int sec_id);
+/* Eigensolver run on a sector (Storage::diag, Storage.h:43-146, as edh_eigsh);
+ * trivial sectors (n <= 64 or ncv >= n) are solved at submission. */
+int edh_batch_submit_eig(edh_batch* b, int sec_id, int nev, int ncv, double tol, int maxitr, uint64_t seed,
+                         int* job_id);
+/* Lanczos run (Lanczos::lanczos, Lanczos.h:36-77) from the normalised v0. */
+int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl, int mem, int* job_id);
+/* Green's-function start vector c(+)_orb |psi> of eigenpair `index` of a
+ * finished eigensolver job into sector dst_sec, *norm = <out|out>, and a
+ * Lanczos run from out/sqrt(norm) when norm > 1e-10, else *job_id = -1
+ * (GreensFunction.h:395-416,437-458 for one orbital). */
+int edh_batch_submit_gf(edh_batch* b, int eig_job, int index, int orb, int annihilate, int dst_sec, int nl,
+                        double* norm, int* job_id);
+/* Runs up to nsweeps sweeps (stops early when nothing is active). */
+int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
+/* state EDH_JOB_*, kind EDH_JOB_EIG/LANCZOS, dimension, count = nev or
+ * Lanczos steps, n_av = H*v applications. */
+int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t* n, int* count, int* n_av);
+/* evals (nev) ascending; evecs (host, n*nev column-major) may be NULL. */
+int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs, int* nconv);
+/* alpha[nlanc], beta[nl+1] (beta[0] unused), as edh_lanczos. */
+int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc);
+int edh_batch_free_job(edh_batch* b, int job_id);
+/* Device bytes held by the batch's sectors and processes. */
+int edh_batch_memory(edh_batch* b, size_t* bytes);
+/* Autotuner: times the grouped H*v over the given sectors for each block
+ * size and keeps the fastest (returned in *block). */
+int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block);
+/* Lanczos::get_frac_point on the batch's stream (as edh_cf_eval). */
+int edh_batch_cf_eval(edh_batch* b, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign,
+                      double expb, const double* alpha, const double* beta, double* out_re, double* out_im);
+
 #ifdef __cplusplus
 }
 #endif
