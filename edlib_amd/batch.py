@@ -58,13 +58,18 @@ class Batch:
                                                   ctypes.byref(jid)))
         return jid.value
 
-    def submit_gf(self, eig_job, index, orb, annihilate, dst_sid, nl):
-        """Start vector c(+)_orb |psi_index> in dst_sid and its Lanczos run.
-        Returns (job id or -1 when the vector vanishes, squared norm)."""
-        norm, jid = ctypes.c_double(), ctypes.c_int()
-        check(_lib.lib().edh_batch_submit_gf(self._h, eig_job, index, orb, 1 if annihilate else 0, dst_sid, nl,
-                                             ctypes.byref(norm), ctypes.byref(jid)))
-        return jid.value, norm.value
+    def submit_gf(self, requests, nl):
+        """Green's-function start vectors c(+)_orb |psi_index>, one per request
+        (eig_job, index, orb, annihilate, dst_sid), and their Lanczos runs.
+        Returns (job ids, -1 where the vector vanishes; squared norms)."""
+        k = len(requests)
+        cols = [np.ascontiguousarray([int(r[c]) for r in requests], dtype=np.int32) for c in range(5)]
+        norms = np.zeros(max(k, 1))
+        ids = np.zeros(max(k, 1), dtype=np.int32)
+        ip = [c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) for c in cols]
+        check(_lib.lib().edh_batch_submit_gf(self._h, k, ip[0], ip[1], ip[2], ip[3], ip[4], nl, _d(norms),
+                                             ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return ids[:k].tolist(), norms[:k].tolist()
 
     def step(self, nsweeps=1):
         act = ctypes.c_int()

@@ -709,25 +709,31 @@ int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl,
   const int64_t n = B.sector(sec_id).tabs.N;
   const double* src = v0;
   if (mem == EDH_MEM_HOST) {
-    B.gtab.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(double));
-    EDH_HIP(hipMemcpyAsync(B.gtab.p, v0, (size_t)n * sizeof(double), hipMemcpyHostToDevice, B.stream));
-    src = B.gtab.as<double>();
+    B.d_scal.ensure((size_t)std::max<int64_t>(n, 1) * sizeof(double));
+    EDH_HIP(hipMemcpyAsync(B.d_scal.p, v0, (size_t)n * sizeof(double), hipMemcpyHostToDevice, B.stream));
+    src = B.d_scal.as<double>();
   }
   *job_id = B.submit_lanczos(sec_id, src, nl);
   EDH_HIP(hipStreamSynchronize(B.stream));
   API_END
 }
 
-int edh_batch_submit_gf(edh_batch* b, int eig_job, int index, int orb, int annihilate, int dst_sec, int nl,
-                        double* norm, int* job_id) {
+int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
+                        const int* annihilate, const int* dst_secs, int nl, double* norms, int* job_ids) {
   API_BEGIN
   edh::Batch& B = use_batch(b);
-  EDH_CHECK(norm && job_id, EDH_ERR_ARG, "null argument");
-  edh::BJob& e = B.job(eig_job);
-  EDH_CHECK(e.kind == edh::JOB_EIG && e.state == edh::JOB_DONE && e.evecs, EDH_ERR_STATE,
-            "batch: eigensolver job has no eigenvectors");
-  EDH_CHECK(index >= 0 && index < e.nev, EDH_ERR_ARG, "batch: eigenpair index out of range");
-  *job_id = B.submit_gf(e.sec, e.evecs + (size_t)index * e.n, orb, annihilate != 0, dst_sec, nl, norm);
+  EDH_CHECK(count >= 0, EDH_ERR_ARG, "negative count");
+  EDH_CHECK(count == 0 || (eig_jobs && indices && orbs && annihilate && dst_secs && norms && job_ids), EDH_ERR_ARG,
+            "null argument");
+  std::vector<edh::GfRequest> req(count);
+  for (int r = 0; r < count; ++r) req[r] = {eig_jobs[r], indices[r], orbs[r], annihilate[r] ? 1 : 0, dst_secs[r]};
+  std::vector<int> ids;
+  std::vector<double> nr;
+  B.submit_gf(req, nl, ids, nr);
+  for (int r = 0; r < count; ++r) {
+    job_ids[r] = ids[r];
+    norms[r] = nr[r];
+  }
   API_END
 }
 

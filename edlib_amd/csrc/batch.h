@@ -1,7 +1,9 @@
 // Batched Krylov engine (grouped launches over many sectors and processes).
 #pragma once
 
+#include <map>
 #include <memory>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -11,7 +13,6 @@ namespace edh {
 
 enum JobKind { JOB_EIG = 0, JOB_LZ = 1 };
 enum JobState { JOB_ACTIVE = 0, JOB_DONE = 1, JOB_FAILED = 2 };
-enum OpPhase { PH_PROJECT = 0, PH_NORM = 1, PH_SCALE = 2 };
 
 // One (model, sector) resident on the device.
 struct BSector {
@@ -21,37 +22,64 @@ struct BSector {
   size_t bytes = 0;
 };
 
-// One Krylov process.
+// One Krylov process.  sc is its device scalar block: projection
+// coefficients, norms and flags, then (eigensolver) the projected-matrix
+// columns or (Lanczos) the coefficient history and stop flags.
 struct BJob {
   int kind = JOB_EIG, state = JOB_ACTIVE, sec = -1;
   int64_t n = 0;
   std::vector<void*> allocs;
   size_t bytes = 0;
-  double* dsc = nullptr;  // device scalars: h[kMaxBasis], nrm, projected-matrix columns
+  double* sc = nullptr;
   // eigensolver run
   int nev = 0, m = 0, k = 0, col = 0, iter = 0, maxitr = 0, nconv = 0, n_av = 0;
-  double tol = 0.0, nr_last = 0.0;
+  double tol = 0.0;
   bool pending_start_norm = false;
   double *V = nullptr, *Valt = nullptr, *evecs = nullptr, *Cbuf = nullptr;
-  std::vector<double> T, Tc, evals;
-  // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w
+  std::vector<double> T, evals;
+  // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w; step = next step
   int nl = 0, step = 0, nlanc = 0;
-  double last_beta = 0.0;
   double* lz[3] = {nullptr, nullptr, nullptr};
+  double *d_alpha = nullptr, *d_beta = nullptr;
+  int* d_flags = nullptr;  // [0] stopped (|beta| < 1e-10), [1] steps taken
   std::vector<double> alpha, beta;
+  // scalars read back at a restart-cycle end or at the end of a Lanczos run:
+  // a pinned buffer and the event after its copy (the process sits out the
+  // sweeps until the event has completed)
+  double* h_read = nullptr;
+  size_t h_read_count = 0;
+  hipEvent_t ev_read = nullptr;
+  bool awaiting = false;
 };
 
-// Host description of one grouped vector operation (see BOp in batch.hip).
+// Host description of one grouped vector operation on one process.
 struct BOpHost {
-  const double* X;
-  int64_t ldx;
-  int cnt;
-  double* w;
-  int64_t n;
-  double* h;
-  double* hsum;
-  double* nrm;
-  double scale;
+  const double* X = nullptr;  // vectors to project on (cnt of them, stride ldx)
+  int64_t ldx = 0;
+  int cnt = 0;
+  double* w = nullptr;  // operand
+  int64_t n = 0;
+  double* sc = nullptr;      // scalar block
+  double* hsum = nullptr;    // projected-matrix column (accumulated) or null
+  double* hist_a = nullptr;  // Lanczos alpha_k output or null
+  double* hist_b = nullptr;  // Lanczos beta_k output or null
+  int* flags = nullptr;      // Lanczos stop flags or null
+  int kstep = 0;
+  int mode = 0;  // OP_* bits
+  double scale = 1.0;
+  double* nrm_out = nullptr;  // extra copy of the norm, or null
+};
+
+enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8 };
+enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2 };
+
+struct OpGroup {
+  int kind;
+  std::vector<BOpHost> ops;
+};
+
+struct GfRequest {
+  int eig_job, index, orb, annihilate, dst_sec;
 };
 
 struct Batch {
@@ -59,10 +87,13 @@ struct Batch {
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<BSector>> secs;
   std::vector<std::unique_ptr<BJob>> jobs;
-  DevBuf d_hv, d_ops, d_cjob, d_partial, d_scal, gtab;
-  std::vector<char> h_hv;
-  void* h_pin = nullptr;
-  size_t h_pin_bytes = 0;
+  DevBuf d_arena, d_partial, d_scal, d_norms;
+  void* h_stage[2] = {nullptr, nullptr};
+  size_t h_stage_bytes[2] = {0, 0};
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
+  bool ev_used[2] = {false, false};
+  int stage_i = 0;
+  std::map<std::tuple<int, int, int>, std::vector<void*>> maps;  // (sector, orbital, annihilate) -> gather maps
   size_t bytes_in_use = 0;
   int hv_block = 256;
   long long sweeps = 0;
@@ -74,20 +105,24 @@ struct Batch {
   BJob& job(int id);
   int submit_eig(int sec, int nev, int ncv, double tol, int maxitr, uint64_t seed);
   int submit_lanczos(int sec, const double* v0_dev, int nl);
-  int submit_gf(int src_sec, const double* psi, int orb, bool annihilate, int dst_sec, int nl, double* norm);
+  // Green's-function start vectors and their Lanczos runs; ids[i] = -1 when
+  // vector i vanishes; norms[i] = its squared norm.
+  void submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms);
   void free_job(int id);
   int step();
-  void adag(int src, const double* in, int orb, bool annihilate, int dst, double* out);
   int autotune(const std::vector<int>& sec_ids);
 
  private:
   double* dalloc(BJob& j, size_t count);
   void release_job(BJob& j);
+  void read_back(BJob& j, const double* src, size_t count);
+  void complete(BJob& j);
   void finish_dense(BJob& j);
   void end_cycle(BJob& j);
-  void run_ops(std::vector<BOpHost>& ops, int phase);
-  void run_hv(std::vector<std::pair<int, HvArgs>>& hv);
+  void finish_lanczos(BJob& j);
+  void issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv, std::vector<OpGroup>& post);
   void fill_dev(double* p, size_t n);
+  void gather_maps(int src, int orb, bool annihilate, std::vector<void*>& out);
 };
 
 }  // namespace edh

@@ -7,14 +7,21 @@
 // advances every active process by one Krylov step and each phase of a
 // sweep is ONE launch over all of them: the grouped H*v over the
 // concatenated up-rows of every operand, then grouped projections, updates,
-// norms and scalings over fixed chunks of every vector.  Processes join and
-// leave between sweeps (continuous batching).
+// norms and scalings over fixed chunks of every vector.  All descriptors of
+// a sweep go to the device in one copy from pinned memory, the Lanczos
+// coefficients and stop flags stay on the device, and the host waits only
+// when an eigensolver run ends a restart cycle or a Lanczos run ends.
+// Processes join and leave between sweeps (continuous batching).
+//
+// Re-orthogonalisation follows ARPACK's dsaitr: one classical Gram-Schmidt
+// pass, and a second one only when the first removed more than 1 - 0.717
+// of the norm (the DGKS test), decided on the device per process.
 //
 // Determinism: a process's arithmetic depends only on its own operands.  Its
 // H*v elements do not depend on the grouping, its reductions use its own
-// chunks (kChunk elements each) summed in a fixed order, and its host-side
-// logic (projected matrix, restarts, breakdown test) sees only its own
-// numbers.  A result is therefore bit-identical alone and in any mix.
+// chunks (kChunk elements each) reduced in a fixed order, and its host-side
+// logic (projected matrix, restarts) sees only its own numbers.  A result
+// is therefore bit-identical alone and in any mix.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -27,25 +34,42 @@ namespace edh {
 
 namespace {
 
-constexpr int kChunk = 16384;  // elements per reduction chunk
+constexpr int kChunk = 2048;   // elements per reduction chunk
 constexpr int kOpBlock = 256;  // threads per chunk kernel block
 constexpr int kHGroup = 8;     // projections accumulated per pass over a chunk
+constexpr double kDgks = 0.717;
 
-// One grouped vector operation on one process: project w on the cnt
-// vectors X (stride ldx), subtract, take norms, scale.
+// scalar block layout (doubles)
+constexpr int S_NRM = kMaxBasis;         // current norm
+constexpr int S_ONE = kMaxBasis + 1;     // 1.0  } H*v scales of a Lanczos step:
+constexpr int S_NEGNRM = kMaxBasis + 2;  // -nrm }  v scale, w scale
+constexpr int S_NB = kMaxBasis + 3;      // norm before the first projection pass
+constexpr int S_GATE = kMaxBasis + 4;    // 1 when the second pass is needed
+constexpr int S_T = 2 * kMaxBasis;       // projected-matrix columns / Lanczos history
+
 struct BOp {
   const double* X;
   int64_t ldx;
   int cnt;
   double* w;
   int64_t n;
-  int chunk0;   // first global chunk of this process
-  int nch;      // chunks of this process
-  double* h;    // device: projection coefficients (kMaxBasis)
-  double* hsum; // device: accumulated coefficients (projected matrix column) or null
-  double* nrm;  // device: norm output
-  double scale; // bscale: factor when nrm is null
+  int chunk0, nch;
+  double* sc;
+  double* hsum;
+  double* hist_a;
+  double* hist_b;
+  int* flags;
+  int kstep;
+  int mode;
+  double scale;
+  double* nrm_out;
 };
+
+__device__ __forceinline__ bool op_skip(const BOp& o) {
+  if (o.flags && o.flags[0]) return true;
+  if ((o.mode & OP_GATED) && o.sc[S_GATE] == 0.0) return true;
+  return false;
+}
 
 __device__ __forceinline__ double block_sum256(double x, double* s_red) {
   for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
@@ -59,61 +83,100 @@ __device__ __forceinline__ double block_sum256(double x, double* s_red) {
   return t;  // valid on thread 0
 }
 
-// partial[c * kMaxBasis + i] = sum over chunk c of X_i[e] * w[e], i < cnt
+// partial[c * kMaxBasis + i] = chunk c's sum of X_i[e] * w[e], i < cnt, and
+// (OP_NB) slot cnt = chunk c's sum of w[e]^2
 __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
                                                          double* __restrict__ partial) {
   __shared__ double s_red[8];
   const int c = blockIdx.x;
   const BOp& o = ops[cjob[c]];
+  if (op_skip(o)) return;
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * kChunk;
   const int64_t e1 = min(o.n, e0 + kChunk);
-  for (int i0 = 0; i0 < o.cnt; i0 += kHGroup) {
+  const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
+  for (int i0 = 0; i0 < tot; i0 += kHGroup) {
     double acc[kHGroup];
 #pragma unroll
     for (int g = 0; g < kHGroup; ++g) acc[g] = 0.0;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
       const double we = o.w[e];
 #pragma unroll
-      for (int g = 0; g < kHGroup; ++g)
-        if (i0 + g < o.cnt) acc[g] = fma(o.X[(int64_t)(i0 + g) * o.ldx + e], we, acc[g]);
+      for (int g = 0; g < kHGroup; ++g) {
+        const int i = i0 + g;
+        if (i < o.cnt) acc[g] = fma(o.X[(int64_t)i * o.ldx + e], we, acc[g]);
+        else if (i < tot) acc[g] = fma(we, we, acc[g]);
+      }
     }
 #pragma unroll
     for (int g = 0; g < kHGroup; ++g) {
-      if (i0 + g >= o.cnt) break;
+      if (i0 + g >= tot) break;
       const double t = block_sum256(acc[g], s_red);
       if (threadIdx.x == 0) partial[(size_t)c * kMaxBasis + i0 + g] = t;
     }
   }
 }
 
-// One block per process: h[i] = sum of its chunks' partials in chunk order;
-// hsum[i] += h[i] when given.  mode 1: nrm = sqrt(sum of partial slot 0).
-__global__ __launch_bounds__(64) void breduce_kernel(const BOp* __restrict__ ops, const double* __restrict__ partial,
-                                                     int mode) {
+// One block (four wavefronts) per process.  mode 0: h = its chunks'
+// partials, each coefficient summed by one wavefront (lane l takes chunks l,
+// l+64, ...; then a fixed shuffle tree), into hsum too, alpha history, norm
+// before.  mode 1: norm of the updated vector, DGKS gate, Lanczos beta
+// history and stop test.
+__device__ __forceinline__ double wave_chunk_sum(const BOp& o, const double* partial, int i, int lane) {
+  double t = 0.0;
+  for (int lc = lane; lc < o.nch; lc += 64) t += partial[(size_t)(o.chunk0 + lc) * kMaxBasis + i];
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  return t;  // valid on lane 0
+}
+
+__global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ ops, const double* __restrict__ partial,
+                                                      int mode) {
   const BOp& o = ops[blockIdx.x];
+  if (op_skip(o)) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (mode == 0) {
-    for (int i = threadIdx.x; i < o.cnt; i += blockDim.x) {
-      double t = 0.0;
-      for (int lc = 0; lc < o.nch; ++lc) t += partial[(size_t)(o.chunk0 + lc) * kMaxBasis + i];
-      o.h[i] = t;
-      if (o.hsum) o.hsum[i] += t;
+    const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
+    for (int i = wave; i < tot; i += 4) {
+      const double t = wave_chunk_sum(o, partial, i, lane);
+      if (lane == 0) {
+        if (i < o.cnt) {
+          o.sc[i] = t;
+          if (o.hsum) o.hsum[i] += t;
+          if (i == 0 && o.hist_a) *o.hist_a = t;
+        } else {
+          o.sc[S_NB] = sqrt(t);
+        }
+      }
     }
-  } else if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int lc = 0; lc < o.nch; ++lc) t += partial[(size_t)(o.chunk0 + lc) * kMaxBasis];
-    *o.nrm = sqrt(t);
+  } else if (wave == 0) {
+    const double t = wave_chunk_sum(o, partial, 0, lane);
+    if (lane == 0) {
+      const double nrm = sqrt(t);
+      o.sc[S_NRM] = nrm;
+      o.sc[S_ONE] = 1.0;
+      o.sc[S_NEGNRM] = -nrm;
+      if (o.mode & OP_NB) o.sc[S_GATE] = nrm < kDgks * o.sc[S_NB] ? 1.0 : 0.0;
+      if (o.nrm_out) *o.nrm_out = nrm;
+      if (o.mode & OP_LZ) {  // Lanczos.h:61-65
+        if (o.hist_b) *o.hist_b = nrm;
+        o.flags[1] = o.kstep;
+        if (fabs(nrm) < 1e-10) o.flags[0] = 1;
+      }
+    }
   }
 }
 
-// w[e] -= sum_i h[i] X_i[e]; partial[c * kMaxBasis] = sum of the new w[e]^2
+// w[e] -= sum_i h[i] X_i[e]; partial[c * kMaxBasis] = sum of the new w[e]^2.
+// The projections are applied eight at a time, so eight loads per element
+// are in flight together.
 __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
                                                            double* __restrict__ partial) {
   __shared__ double s_red[8];
-  __shared__ double s_h[kMaxBasis];
+  __shared__ double s_h[kMaxBasis + kHGroup];
   const int c = blockIdx.x;
   const BOp& o = ops[cjob[c]];
-  for (int i = threadIdx.x; i < o.cnt; i += blockDim.x) s_h[i] = o.h[i];
+  if (op_skip(o)) return;
+  for (int i = threadIdx.x; i < kMaxBasis + kHGroup; i += blockDim.x) s_h[i] = i < o.cnt ? o.sc[i] : 0.0;
   __syncthreads();
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * kChunk;
@@ -121,19 +184,26 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
   double nsq = 0.0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
     double x = o.w[e];
-    for (int i = 0; i < o.cnt; ++i) x = fma(-s_h[i], o.X[(int64_t)i * o.ldx + e], x);
-    o.w[e] = x;
+    for (int i0 = 0; i0 < o.cnt; i0 += kHGroup) {
+      double xv[kHGroup];
+#pragma unroll
+      for (int g = 0; g < kHGroup; ++g) xv[g] = i0 + g < o.cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
+#pragma unroll
+      for (int g = 0; g < kHGroup; ++g) x = fma(-s_h[i0 + g], xv[g], x);
+    }
+    if (o.cnt) o.w[e] = x;
     nsq = fma(x, x, nsq);
   }
   const double t = block_sum256(nsq, s_red);
   if (threadIdx.x == 0) partial[(size_t)c * kMaxBasis] = t;
 }
 
-// w[e] *= 1 / nrm (or *= scale when nrm is null)
+// w[e] *= 1 / nrm (OP_SCALE_NRM) or *= scale
 __global__ __launch_bounds__(kOpBlock) void bscale_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob) {
   const int c = blockIdx.x;
   const BOp& o = ops[cjob[c]];
-  const double f = o.nrm ? 1.0 / *o.nrm : o.scale;
+  if (op_skip(o)) return;
+  const double f = (o.mode & OP_SCALE_NRM) ? 1.0 / o.sc[S_NRM] : o.scale;
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * kChunk;
   const int64_t e1 = min(o.n, e0 + kChunk);
@@ -153,16 +223,24 @@ uint64_t splitmix64(uint64_t& x) {
 
 int nchunks(int64_t n) { return (int)((n + kChunk - 1) / kChunk); }
 
+size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
 }  // namespace
 
 Batch::~Batch() {
   if (device >= 0) (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
   for (auto& j : jobs)
     if (j) release_job(*j);
+  for (auto& kv : maps)
+    for (void* p : kv.second) (void)hipFree(p);
   for (auto& s : secs)
     if (s) s->dev.release();
-  for (DevBuf* b : {&d_hv, &d_ops, &d_cjob, &d_partial, &d_scal, &gtab}) b->release();
-  if (h_pin) (void)hipHostFree(h_pin);
+  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms}) b->release();
+  for (int i = 0; i < 2; ++i) {
+    if (h_stage[i]) (void)hipHostFree(h_stage[i]);
+    if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
+  }
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -182,6 +260,33 @@ void Batch::release_job(BJob& j) {
   j.allocs.clear();
   bytes_in_use -= j.bytes;
   j.bytes = 0;
+  if (j.h_read) (void)hipHostFree(j.h_read);
+  j.h_read = nullptr;
+  j.h_read_count = 0;
+  if (j.ev_read) (void)hipEventDestroy(j.ev_read);
+  j.ev_read = nullptr;
+}
+
+// Queues the copy of a process's scalars into its pinned buffer and marks
+// the process as waiting for them.
+void Batch::read_back(BJob& j, const double* src, size_t count) {
+  if (count > j.h_read_count) {
+    if (j.h_read) EDH_HIP(hipHostFree(j.h_read));
+    j.h_read = nullptr;
+    EDH_HIP(hipHostMalloc((void**)&j.h_read, count * sizeof(double)));
+    j.h_read_count = count;
+  }
+  if (!j.ev_read) EDH_HIP(hipEventCreateWithFlags(&j.ev_read, hipEventDisableTiming));
+  EDH_HIP(hipMemcpyAsync(j.h_read, src, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipEventRecord(j.ev_read, stream));
+  j.awaiting = true;
+}
+
+// Host part of a process once its scalars have arrived.
+void Batch::complete(BJob& j) {
+  j.awaiting = false;
+  if (j.kind == JOB_EIG) end_cycle(j);
+  else finish_lanczos(j);
 }
 
 int Batch::add_sector(const Model& m, int nup, int ndn) {
@@ -209,18 +314,27 @@ void Batch::remove_sector(int id) {
   BSector& s = sector(id);
   for (auto& j : jobs)
     EDH_CHECK(!j || j->sec != id || j->state != JOB_ACTIVE, EDH_ERR_STATE, "batch: sector has an active job");
+  EDH_HIP(hipStreamSynchronize(stream));
+  for (auto it = maps.begin(); it != maps.end();) {
+    if (std::get<0>(it->first) == id) {
+      for (void* p : it->second) (void)hipFree(p);
+      it = maps.erase(it);
+    } else {
+      ++it;
+    }
+  }
   bytes_in_use -= s.bytes;
   s.dev.release();
   secs[id].reset();
 }
 
 // Dense solution of a sector whose Krylov space is the whole sector (the
-// same rule as the single-sector eigensolver: ncv >= n or n <= 64).
+// single-sector eigensolver's rule: ncv >= n or n <= 64).
 void Batch::finish_dense(BJob& j) {
   BSector& s = sector(j.sec);
   const int64_t n = j.n;
   std::vector<double> w;
-  std::vector<double> H(n * n, 0.0);
+  std::vector<double> H;
   dense_sector_matrix(*s.model, s.tabs, H);
   std::vector<double> Hs(H.size());
   for (int64_t a = 0; a < n; ++a)
@@ -241,6 +355,7 @@ void Batch::finish_dense(BJob& j) {
 
 int Batch::submit_eig(int sec, int nev_req, int ncv_req, double tol, int maxitr, uint64_t seed) {
   BSector& s = sector(sec);
+  EDH_CHECK(nev_req >= 1 && ncv_req >= 2, EDH_ERR_ARG, "batch: bad nev/ncv");
   auto jp = std::make_unique<BJob>();
   BJob& j = *jp;
   j.kind = JOB_EIG;
@@ -248,7 +363,6 @@ int Batch::submit_eig(int sec, int nev_req, int ncv_req, double tol, int maxitr,
   j.n = s.tabs.N;
   j.tol = tol;
   j.maxitr = maxitr;
-  EDH_CHECK(nev_req >= 1 && ncv_req >= 2, EDH_ERR_ARG, "batch: bad nev/ncv");
   jobs.push_back(std::move(jp));
   const int id = (int)jobs.size() - 1;
   const int64_t n = j.n;
@@ -278,7 +392,8 @@ int Batch::submit_eig(int sec, int nev_req, int ncv_req, double tol, int maxitr,
   EDH_CHECK(j.nev < j.m - 1, EDH_ERR_LIMIT, "batch: NEV too large for the device Krylov basis (max 46)");
   j.V = dalloc(j, (size_t)(j.m + 1) * n);
   j.Valt = dalloc(j, (size_t)(j.m + 1) * n);
-  j.dsc = dalloc(j, (size_t)(j.m + 1) * (j.m + 1) + 2 * kMaxBasis);  // T columns, norms, h scratch
+  j.sc = dalloc(j, S_T + (size_t)(j.m + 1) * (j.m + 1));
+  fill_dev(j.sc, S_T + (size_t)(j.m + 1) * (j.m + 1));
   j.T.assign((size_t)j.m * j.m, 0.0);
   // random start vector, uniform in [-1, 1) (the single-sector eigensolver's generator)
   std::vector<double> x(n);
@@ -286,10 +401,9 @@ int Batch::submit_eig(int sec, int nev_req, int ncv_req, double tol, int maxitr,
   for (int64_t i = 0; i < n; ++i) x[i] = (double)(splitmix64(st) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
   EDH_HIP(hipMemcpyAsync(j.V, x.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
   EDH_HIP(hipStreamSynchronize(stream));
-  // normalise on the device with the process's own chunked reduction
   j.k = 0;
   j.col = 0;
-  j.pending_start_norm = true;
+  j.pending_start_norm = true;  // normalised by the next sweep
   j.state = JOB_ACTIVE;
   return id;
 }
@@ -312,260 +426,122 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
     return id;
   }
   for (int q = 0; q < 3; ++q) j.lz[q] = dalloc(j, (size_t)j.n);
-  j.dsc = dalloc(j, 2 * kMaxBasis);
-  EDH_HIP(hipMemcpyAsync(j.lz[0], v0_dev, (size_t)j.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  j.sc = dalloc(j, S_T + 2 * (size_t)nl + 2);
+  fill_dev(j.sc, S_T + 2 * (size_t)nl + 2);
+  j.d_alpha = j.sc + S_T;
+  j.d_beta = j.d_alpha + nl;
+  j.d_flags = (int*)(j.d_beta + nl + 1);
+  if (v0_dev)
+    EDH_HIP(hipMemcpyAsync(j.lz[0], v0_dev, (size_t)j.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   j.step = 1;
   j.state = JOB_ACTIVE;
   return id;
 }
 
-// Green's-function start vector c(+)_orb psi in sector dst, its squared
-// norm, and (when the norm exceeds 1e-10) a Lanczos run from the normalised
-// vector (GreensFunction.h:395-416,437-458).  Returns the job id, or -1
-// when the vector vanishes.
-int Batch::submit_gf(int src_sec, const double* psi, int orb, bool annihilate, int dst_sec, int nl, double* norm) {
-  BSector& b = sector(dst_sec);
-  EDH_CHECK(nl >= 1, EDH_ERR_ARG, "batch: nl must be >= 1");
-  *norm = 0.0;
-  if (b.tabs.N == 0) return -1;
-  auto jp = std::make_unique<BJob>();
-  BJob& j = *jp;
-  j.kind = JOB_LZ;
-  j.sec = dst_sec;
-  j.n = b.tabs.N;
-  j.nl = nl;
-  j.alpha.assign(nl, 0.0);
-  j.beta.assign(nl + 1, 0.0);
-  for (int q = 0; q < 3; ++q) j.lz[q] = dalloc(j, (size_t)j.n);
-  j.dsc = dalloc(j, 2 * kMaxBasis);
-  adag(src_sec, psi, orb, annihilate, dst_sec, j.lz[0]);
-  std::vector<BOpHost> ops{{nullptr, 0, 0, j.lz[0], j.n, j.dsc, nullptr, j.dsc + kMaxBasis, 1.0}};
-  run_ops(ops, PH_NORM);
-  double nrm = 0.0;
-  EDH_HIP(hipMemcpyAsync(&nrm, j.dsc + kMaxBasis, sizeof(double), hipMemcpyDeviceToHost, stream));
-  EDH_HIP(hipStreamSynchronize(stream));
-  *norm = nrm * nrm;
-  if (!(std::fabs(*norm) > 1e-10)) {
-    release_job(j);
-    return -1;
+// Device gather maps of c / c^+ on `orb` out of sector src (the single-sector
+// a_adag maps), cached per (sector, orbital, operator).
+void Batch::gather_maps(int src, int orb, bool annihilate, std::vector<void*>& out) {
+  auto key = std::make_tuple(src, orb, annihilate ? 1 : 0);
+  auto it = maps.find(key);
+  if (it != maps.end()) {
+    out = it->second;
+    return;
   }
-  run_ops(ops, PH_SCALE);
-  j.step = 1;
-  j.state = JOB_ACTIVE;
-  jobs.push_back(std::move(jp));
-  return (int)jobs.size() - 1;
+  BSector& a = sector(src);
+  const int Ns = a.model->Ns;
+  int nup2, ndn2;
+  std::vector<int32_t> mu, md;
+  std::vector<int8_t> su, sd;
+  adag_gather_maps(Ns, a.tabs.nup, a.tabs.ndn, orb, annihilate, nup2, ndn2, mu, md, su, sd);
+  std::vector<void*> p(4, nullptr);
+  const size_t sz[4] = {mu.size() * 4, md.size() * 4, su.size(), sd.size()};
+  const void* src_p[4] = {mu.data(), md.data(), su.data(), sd.data()};
+  for (int q = 0; q < 4; ++q) {
+    EDH_HIP(hipMalloc(&p[q], std::max<size_t>(sz[q], 8)));
+    if (sz[q]) EDH_HIP(hipMemcpyAsync(p[q], src_p[q], sz[q], hipMemcpyHostToDevice, stream));
+  }
+  EDH_HIP(hipStreamSynchronize(stream));
+  maps[key] = p;
+  out = p;
+}
+
+// Green's-function start vectors c(+)_orb |psi> (GreensFunction.h:395-416,
+// 437-458, one orbital each) gathered on the device, their squared norms
+// from one grouped reduction, and a Lanczos run from each normalised vector
+// whose squared norm exceeds 1e-10.
+void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms) {
+  EDH_CHECK(nl >= 1, EDH_ERR_ARG, "batch: nl must be >= 1");
+  ids.assign(req.size(), -1);
+  norms.assign(req.size(), 0.0);
+  std::vector<int> made(req.size(), -1);
+  OpGroup g{G_NORM, {}};
+  d_norms.ensure(std::max<size_t>(req.size(), 1) * sizeof(double));
+  double* dn = d_norms.as<double>();
+  for (size_t r = 0; r < req.size(); ++r) {
+    const GfRequest& q = req[r];
+    BJob& e = job(q.eig_job);
+    EDH_CHECK(e.kind == JOB_EIG && e.state == JOB_DONE && e.evecs, EDH_ERR_STATE,
+              "batch: eigensolver job has no eigenvectors");
+    EDH_CHECK(q.index >= 0 && q.index < e.nev, EDH_ERR_ARG, "batch: eigenpair index out of range");
+    BSector& a = sector(e.sec);
+    BSector& b = sector(q.dst_sec);
+    const int Ns = a.model->Ns;
+    EDH_CHECK(q.orb >= 0 && q.orb < 2 * Ns, EDH_ERR_ARG, "batch: orbital out of range");
+    const int spin = q.orb >= Ns ? 1 : 0;
+    const int d = q.annihilate ? -1 : 1;
+    EDH_CHECK(b.tabs.nup == a.tabs.nup + (spin == 0 ? d : 0) && b.tabs.ndn == a.tabs.ndn + (spin == 1 ? d : 0),
+              EDH_ERR_ARG, "batch: target sector does not match the operator");
+    if (b.tabs.N == 0) continue;
+    std::vector<void*> mp;
+    gather_maps(e.sec, q.orb, q.annihilate != 0, mp);
+    const int id = submit_lanczos(q.dst_sec, nullptr, nl);
+    BJob& j = job(id);
+    j.state = JOB_FAILED;  // not active until its norm is known
+    dev_separable_gather(e.evecs + (size_t)q.index * e.n, a.tabs.dn.D, (const int32_t*)mp[0], (const int32_t*)mp[1],
+                         (const int8_t*)mp[2], (const int8_t*)mp[3], j.lz[0], b.tabs.up.D, b.tabs.dn.D, stream);
+    BOpHost o;
+    o.w = j.lz[0];
+    o.n = j.n;
+    o.sc = j.sc;
+    o.nrm_out = dn + r;
+    g.ops.push_back(o);
+    made[r] = id;
+  }
+  std::vector<OpGroup> pre{g}, post;
+  issue(pre, nullptr, post);
+  std::vector<double> nr(req.size(), 0.0);
+  if (!req.empty())
+    EDH_HIP(hipMemcpyAsync(nr.data(), dn, req.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipStreamSynchronize(stream));
+  OpGroup sg{G_SCALE, {}};
+  for (size_t r = 0; r < req.size(); ++r) {
+    if (made[r] < 0) continue;
+    BJob& j = job(made[r]);
+    norms[r] = nr[r] * nr[r];
+    if (!(std::fabs(norms[r]) > 1e-10)) {
+      release_job(j);
+      jobs[made[r]].reset();
+      continue;
+    }
+    BOpHost o;
+    o.w = j.lz[0];
+    o.n = j.n;
+    o.sc = j.sc;
+    o.mode = OP_SCALE_NRM;
+    sg.ops.push_back(o);
+    j.state = JOB_ACTIVE;
+    ids[r] = made[r];
+  }
+  std::vector<OpGroup> pre2{sg};
+  issue(pre2, nullptr, post);
 }
 
 void Batch::free_job(int id) {
   BJob& j = job(id);
   EDH_CHECK(j.state != JOB_ACTIVE, EDH_ERR_STATE, "batch: job still active");
+  EDH_HIP(hipStreamSynchronize(stream));
   release_job(j);
   jobs[id].reset();
-}
-
-// ---------------------------------------------------------------------------
-// Descriptor upload and the grouped launches of one phase.
-// ---------------------------------------------------------------------------
-void Batch::run_ops(std::vector<BOpHost>& ops, int phase) {
-  if (ops.empty()) return;
-  std::vector<BOp> d(ops.size());
-  std::vector<int> cj;
-  int c0 = 0;
-  for (size_t q = 0; q < ops.size(); ++q) {
-    const BOpHost& h = ops[q];
-    BOp& o = d[q];
-    o.X = h.X;
-    o.ldx = h.ldx;
-    o.cnt = h.cnt;
-    o.w = h.w;
-    o.n = h.n;
-    o.chunk0 = c0;
-    o.nch = nchunks(h.n);
-    o.h = h.h;
-    o.hsum = h.hsum;
-    o.nrm = h.nrm;
-    o.scale = h.scale;
-    for (int c = 0; c < o.nch; ++c) cj.push_back((int)q);
-    c0 += o.nch;
-  }
-  const int nch = c0;
-  d_ops.ensure(d.size() * sizeof(BOp));
-  d_cjob.ensure(cj.size() * sizeof(int));
-  d_partial.ensure((size_t)std::max(nch, 1) * kMaxBasis * sizeof(double));
-  EDH_HIP(hipMemcpyAsync(d_ops.p, d.data(), d.size() * sizeof(BOp), hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipMemcpyAsync(d_cjob.p, cj.data(), cj.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-  const BOp* dops = d_ops.as<BOp>();
-  const int* dcj = d_cjob.as<int>();
-  double* part = d_partial.as<double>();
-  const int nops = (int)ops.size();
-  if (phase == PH_PROJECT) {
-    // h = X^T w (into hsum too); w -= X h; nrm = |w|
-    hipLaunchKernelGGL(bdots_kernel, dim3(nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-    hipLaunchKernelGGL(breduce_kernel, dim3(nops), dim3(64), 0, stream, dops, part, 0);
-    hipLaunchKernelGGL(bupdate_kernel, dim3(nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-    hipLaunchKernelGGL(breduce_kernel, dim3(nops), dim3(64), 0, stream, dops, part, 1);
-  } else if (phase == PH_NORM) {
-    // nrm = |w| (cnt = 0: the update only accumulates the squares)
-    hipLaunchKernelGGL(bupdate_kernel, dim3(nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-    hipLaunchKernelGGL(breduce_kernel, dim3(nops), dim3(64), 0, stream, dops, part, 1);
-  } else {
-    hipLaunchKernelGGL(bscale_kernel, dim3(nch), dim3(kOpBlock), 0, stream, dops, dcj);
-  }
-  EDH_HIP(hipGetLastError());
-}
-
-void Batch::run_hv(std::vector<std::pair<int, HvArgs>>& hv) {
-  if (hv.empty()) return;
-  const size_t jb = hv_batch_job_bytes();
-  h_hv.resize(hv.size() * jb);
-  int64_t row0 = 0, maxd = 0;
-  for (size_t q = 0; q < hv.size(); ++q) {
-    const DevSector& ds = sector(hv[q].first).dev;
-    hv_batch_fill(ds, hv[q].second, row0, h_hv.data() + q * jb);
-    row0 += ds.up.D;
-    maxd = std::max<int64_t>(maxd, ds.dn.D);
-  }
-  d_hv.ensure(h_hv.size());
-  EDH_HIP(hipMemcpyAsync(d_hv.p, h_hv.data(), h_hv.size(), hipMemcpyHostToDevice, stream));
-  const int grid = (int)std::min<int64_t>(row0, 1 << 20);
-  launch_hv_batch(d_hv.p, (int)hv.size(), row0, maxd, hv_block, grid, stream);
-}
-
-// One sweep: every active process advances one Krylov step.
-int Batch::step() {
-  std::vector<BJob*> eig_start, eig, lz;
-  for (auto& jp : jobs) {
-    if (!jp || jp->state != JOB_ACTIVE) continue;
-    BJob& j = *jp;
-    if (j.kind == JOB_EIG) (j.pending_start_norm ? eig_start : eig).push_back(&j);
-    else lz.push_back(&j);
-  }
-  if (eig_start.empty() && eig.empty() && lz.empty()) return 0;
-  std::vector<BOpHost> ops;
-  // (0) newly submitted eigensolver runs: normalise the start vector
-  if (!eig_start.empty()) {
-    ops.clear();
-    for (BJob* j : eig_start) ops.push_back({j->V, j->n, 0, j->V, j->n, j->dsc, nullptr, j->dsc + kMaxBasis, 1.0});
-    run_ops(ops, PH_NORM);
-    run_ops(ops, PH_SCALE);
-    for (BJob* j : eig_start) {
-      j->pending_start_norm = false;
-      eig.push_back(j);
-      fill_dev(j->dsc + 2 * kMaxBasis, (size_t)(j->m + 1) * (j->m + 1));
-    }
-  }
-  // (1) Lanczos runs past their first step: v_k = w / beta_{k-1}, in place,
-  // then rotate (prev, cur, w) <- (cur, w, prev)
-  ops.clear();
-  for (BJob* j : lz)
-    if (j->step > 1) ops.push_back({nullptr, 0, 0, j->lz[2], j->n, nullptr, nullptr, nullptr, 1.0 / j->last_beta});
-  run_ops(ops, PH_SCALE);
-  for (BJob* j : lz)
-    if (j->step > 1) {
-      double* t = j->lz[1];
-      j->lz[1] = j->lz[0];
-      j->lz[0] = j->lz[2];
-      j->lz[2] = t;
-    }
-  // (2) grouped H*v: eigensolver w = V_{col+1} = H V_col ; Lanczos
-  // w = H v_k - beta_{k-1} v_{k-1} (Lanczos.h:47-55)
-  std::vector<std::pair<int, HvArgs>> hv;
-  for (BJob* j : eig) {
-    HvArgs a{};
-    a.v = j->V + (size_t)j->col * j->n;
-    a.w = j->V + (size_t)(j->col + 1) * j->n;
-    a.wprev = a.w;
-    a.vscale = 1.0;
-    a.wscale = 0.0;
-    a.mode = 0;
-    hv.push_back({j->sec, a});
-  }
-  for (BJob* j : lz) {
-    HvArgs a{};
-    a.v = j->lz[0];
-    a.w = j->lz[2];
-    a.wprev = j->lz[1];
-    a.vscale = 1.0;
-    a.wscale = j->step > 1 ? -j->last_beta : 0.0;
-    a.mode = j->step > 1 ? 1 : 0;
-    hv.push_back({j->sec, a});
-  }
-  run_hv(hv);
-  // (3) first projection pass: eigensolver against V_0..V_col (into the
-  // projected-matrix column); Lanczos alpha = <v, w>, w -= alpha v, beta = |w|
-  ops.clear();
-  for (BJob* j : eig) {
-    double* Tcol = j->dsc + 2 * kMaxBasis + (size_t)j->col * (j->m + 1);
-    ops.push_back({j->V, j->n, j->col + 1, j->V + (size_t)(j->col + 1) * j->n, j->n, j->dsc, Tcol,
-                   j->dsc + kMaxBasis, 0.0});
-  }
-  for (BJob* j : lz) ops.push_back({j->lz[0], j->n, 1, j->lz[2], j->n, j->dsc, nullptr, j->dsc + kMaxBasis, 0.0});
-  run_ops(ops, PH_PROJECT);
-  // (4) second pass (eigensolver only) and normalisation of the new basis vector
-  if (!eig.empty()) {
-    ops.resize(eig.size());
-    run_ops(ops, PH_PROJECT);
-    run_ops(ops, PH_SCALE);
-  }
-  // (5) scalars back to the host: one copy per process into pinned memory
-  const size_t per = 2 + kMaxBasis;
-  const size_t need = (eig.size() + lz.size()) * per * sizeof(double);
-  if (need > h_pin_bytes) {
-    if (h_pin) EDH_HIP(hipHostFree(h_pin));
-    h_pin = nullptr;
-    EDH_HIP(hipHostMalloc(&h_pin, need));
-    h_pin_bytes = need;
-  }
-  double* hp = (double*)h_pin;
-  size_t slot = 0;
-  for (BJob* j : lz) {
-    EDH_HIP(hipMemcpyAsync(hp + slot * per, j->dsc, sizeof(double), hipMemcpyDeviceToHost, stream));
-    EDH_HIP(hipMemcpyAsync(hp + slot * per + 1, j->dsc + kMaxBasis, sizeof(double), hipMemcpyDeviceToHost, stream));
-    ++slot;
-  }
-  std::vector<BJob*> cycle_end;
-  for (BJob* j : eig)
-    if (j->col + 1 == j->m) cycle_end.push_back(j);
-  for (BJob* j : cycle_end) {
-    j->Tc.resize((size_t)j->m * (j->m + 1));
-    j->nr_last = 0.0;
-    EDH_HIP(hipMemcpyAsync(j->Tc.data(), j->dsc + 2 * kMaxBasis, j->Tc.size() * sizeof(double),
-                           hipMemcpyDeviceToHost, stream));
-    EDH_HIP(hipMemcpyAsync(&j->nr_last, j->dsc + kMaxBasis, sizeof(double), hipMemcpyDeviceToHost, stream));
-  }
-  EDH_HIP(hipStreamSynchronize(stream));
-  ++sweeps;
-  // (6) host bookkeeping per process
-  slot = 0;
-  for (BJob* j : lz) {
-    const double al = hp[slot * per], be = hp[slot * per + 1];
-    ++slot;
-    const int k = j->step;
-    j->alpha[k - 1] = al;
-    if (k != j->nl) j->beta[k] = be;  // Lanczos.h:64
-    j->nlanc = k;
-    if (std::fabs(be) < 1e-10 || k == j->nl) {  // Lanczos.h:65
-      j->state = JOB_DONE;
-      release_job(*j);  // the coefficients are on the host
-      j->lz[0] = j->lz[1] = j->lz[2] = nullptr;
-      j->dsc = nullptr;
-      continue;
-    }
-    j->last_beta = be;
-    j->step = k + 1;
-  }
-  for (BJob* j : eig) {
-    ++j->n_av;
-    if (j->col + 1 < j->m) {
-      ++j->col;
-      continue;
-    }
-    end_cycle(*j);
-  }
-  int active = 0;
-  for (auto& jp : jobs)
-    if (jp && jp->state == JOB_ACTIVE) ++active;
-  return active;
 }
 
 void Batch::fill_dev(double* p, size_t n) {
@@ -575,18 +551,309 @@ void Batch::fill_dev(double* p, size_t n) {
   EDH_HIP(hipGetLastError());
 }
 
+// Packs every descriptor of one launch sequence (groups before the H*v, the
+// grouped H*v, groups after it) into one pinned staging buffer, copies it to
+// the device in one transfer and launches the sequence.
+void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv,
+                  std::vector<OpGroup>& post) {
+  struct Placed {
+    int kind, nops, nch;
+    size_t ops_off, cj_off;
+  };
+  std::vector<Placed> pl_pre, pl_post;
+  size_t off = 0;
+  auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
+    for (OpGroup& g : gs) {
+      if (g.ops.empty()) continue;
+      int nch = 0;
+      for (const BOpHost& h : g.ops) nch += nchunks(h.n);
+      Placed p{g.kind, (int)g.ops.size(), nch, 0, 0};
+      p.ops_off = off;
+      off = align16(off + g.ops.size() * sizeof(BOp));
+      p.cj_off = off;
+      off = align16(off + (size_t)nch * sizeof(int));
+      out.push_back(p);
+    }
+  };
+  place(pre, pl_pre);
+  const size_t hv_off = off;
+  int64_t hv_rows = 0, hv_maxd = 0;
+  const size_t jb = hv_batch_job_bytes();
+  const bool has_hv = hv && !hv->empty();
+  if (has_hv) off = align16(off + hv->size() * jb);
+  place(post, pl_post);
+  if (off == 0) return;
+  // pinned staging, double-buffered: a buffer is rewritten only after the
+  // copy that last read it has completed
+  const int si = stage_i;
+  stage_i ^= 1;
+  if (ev_used[si]) EDH_HIP(hipEventSynchronize(ev_stage[si]));
+  if (off > h_stage_bytes[si]) {
+    if (h_stage[si]) EDH_HIP(hipHostFree(h_stage[si]));
+    h_stage[si] = nullptr;
+    const size_t cap = std::max<size_t>(off * 2, 1 << 20);
+    EDH_HIP(hipHostMalloc(&h_stage[si], cap));
+    h_stage_bytes[si] = cap;
+  }
+  if (!ev_stage[si]) EDH_HIP(hipEventCreateWithFlags(&ev_stage[si], hipEventDisableTiming));
+  char* hb = (char*)h_stage[si];
+  auto fill = [&](std::vector<OpGroup>& gs, std::vector<Placed>& pls) {
+    size_t q = 0;
+    for (OpGroup& g : gs) {
+      if (g.ops.empty()) continue;
+      const Placed& p = pls[q++];
+      int* cj = (int*)(hb + p.cj_off);
+      int c0 = 0;
+      for (size_t i = 0; i < g.ops.size(); ++i) {
+        const BOpHost& h = g.ops[i];
+        BOp o;
+        o.X = h.X;
+        o.ldx = h.ldx;
+        o.cnt = h.cnt;
+        o.w = h.w;
+        o.n = h.n;
+        o.chunk0 = c0;
+        o.nch = nchunks(h.n);
+        o.sc = h.sc;
+        o.hsum = h.hsum;
+        o.hist_a = h.hist_a;
+        o.hist_b = h.hist_b;
+        o.flags = h.flags;
+        o.kstep = h.kstep;
+        o.mode = h.mode;
+        o.scale = h.scale;
+        o.nrm_out = h.nrm_out;
+        std::memcpy(hb + p.ops_off + i * sizeof(BOp), &o, sizeof(BOp));
+        for (int c = 0; c < o.nch; ++c) cj[c0 + c] = (int)i;
+        c0 += o.nch;
+      }
+    }
+  };
+  fill(pre, pl_pre);
+  if (has_hv) {
+    for (size_t q = 0; q < hv->size(); ++q) {
+      const DevSector& ds = sector((*hv)[q].first).dev;
+      hv_batch_fill(ds, (*hv)[q].second, hv_rows, hb + hv_off + q * jb);
+      hv_rows += ds.up.D;
+      hv_maxd = std::max<int64_t>(hv_maxd, ds.dn.D);
+    }
+  }
+  fill(post, pl_post);
+  int max_ch = 1;
+  for (auto* v : {&pl_pre, &pl_post})
+    for (const Placed& p : *v) max_ch = std::max(max_ch, p.nch);
+  d_arena.ensure(off);
+  d_partial.ensure((size_t)max_ch * kMaxBasis * sizeof(double));
+  EDH_HIP(hipMemcpyAsync(d_arena.p, hb, off, hipMemcpyHostToDevice, stream));
+  EDH_HIP(hipEventRecord(ev_stage[si], stream));
+  ev_used[si] = true;
+  char* db = d_arena.as<char>();
+  double* part = d_partial.as<double>();
+  auto launch = [&](const std::vector<Placed>& pls) {
+    for (const Placed& p : pls) {
+      const BOp* dops = (const BOp*)(db + p.ops_off);
+      const int* dcj = (const int*)(db + p.cj_off);
+      if (p.kind == G_PROJECT) {
+        hipLaunchKernelGGL(bdots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1);
+      } else if (p.kind == G_NORM) {
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1);
+      } else {
+        hipLaunchKernelGGL(bscale_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
+      }
+    }
+  };
+  launch(pl_pre);
+  if (has_hv)
+    launch_hv_batch(db + hv_off, (int)hv->size(), hv_rows, hv_maxd, hv_block,
+                    (int)std::min<int64_t>(hv_rows, 1 << 20), stream);
+  launch(pl_post);
+  EDH_HIP(hipGetLastError());
+}
+
+// One sweep: every active process advances one Krylov step.
+int Batch::step() {
+  // processes whose scalars have arrived go on; when every active process is
+  // waiting, wait for the oldest copies
+  {
+    std::vector<BJob*> wait;
+    bool any_running = false;
+    for (auto& jp : jobs) {
+      if (!jp || jp->state != JOB_ACTIVE) continue;
+      if (!jp->awaiting) {
+        any_running = true;
+        continue;
+      }
+      if (hipEventQuery(jp->ev_read) == hipSuccess) complete(*jp);
+      else wait.push_back(jp.get());
+    }
+    if (!any_running)
+      for (BJob* j : wait) {
+        EDH_HIP(hipEventSynchronize(j->ev_read));
+        complete(*j);
+      }
+  }
+  std::vector<BJob*> eig, lz, starting;
+  for (auto& jp : jobs) {
+    if (!jp || jp->state != JOB_ACTIVE || jp->awaiting) continue;
+    if (jp->kind == JOB_EIG) {
+      eig.push_back(jp.get());
+      if (jp->pending_start_norm) starting.push_back(jp.get());
+    } else {
+      lz.push_back(jp.get());
+    }
+  }
+  if (eig.empty() && lz.empty()) {
+    int active = 0;
+    for (auto& jp : jobs)
+      if (jp && jp->state == JOB_ACTIVE) ++active;
+    return active;
+  }
+  std::vector<OpGroup> pre, post;
+  // (0) newly submitted eigensolver runs: normalise the start vector
+  OpGroup g_norm{G_NORM, {}}, g_scale{G_SCALE, {}};
+  for (BJob* j : starting) {
+    BOpHost o;
+    o.w = j->V;
+    o.n = j->n;
+    o.sc = j->sc;
+    g_norm.ops.push_back(o);
+    o.mode = OP_SCALE_NRM;
+    g_scale.ops.push_back(o);
+    j->pending_start_norm = false;
+  }
+  // (1) Lanczos runs past their first step: v_k = w / beta_{k-1} in place,
+  // then rotate (prev, cur, w) <- (cur, w, prev)  (Lanczos.h:47-51)
+  for (BJob* j : lz)
+    if (j->step > 1) {
+      BOpHost o;
+      o.w = j->lz[2];
+      o.n = j->n;
+      o.sc = j->sc;
+      o.flags = j->d_flags;
+      o.mode = OP_SCALE_NRM;
+      g_scale.ops.push_back(o);
+      double* t = j->lz[1];
+      j->lz[1] = j->lz[0];
+      j->lz[0] = j->lz[2];
+      j->lz[2] = t;
+    }
+  pre.push_back(g_norm);
+  pre.push_back(g_scale);
+  // (2) grouped H*v: eigensolver V_{col+1} = H V_col; Lanczos
+  // w = H v_k - beta_{k-1} v_{k-1} (Lanczos.h:55)
+  std::vector<std::pair<int, HvArgs>> hv;
+  for (BJob* j : eig) {
+    HvArgs a{};
+    a.v = j->V + (size_t)j->col * j->n;
+    a.w = j->V + (size_t)(j->col + 1) * j->n;
+    a.wprev = a.w;
+    a.vscale = 1.0;
+    a.mode = 0;
+    hv.push_back({j->sec, a});
+  }
+  for (BJob* j : lz) {
+    HvArgs a{};
+    a.v = j->lz[0];
+    a.w = j->lz[2];
+    a.wprev = j->lz[1];
+    a.vscale = 1.0;
+    a.mode = j->step > 1 ? 1 : 0;
+    a.dscale = j->step > 1 ? j->sc + S_ONE : nullptr;  // {1, -beta_{k-1}}
+    a.done = j->d_flags;
+    hv.push_back({j->sec, a});
+  }
+  // (3) first projection pass: eigensolver against V_0..V_col (into the
+  // projected-matrix column, with the norm before for the DGKS test);
+  // Lanczos alpha = <v, w>, w -= alpha v, beta = |w| (Lanczos.h:56-65)
+  OpGroup p1{G_PROJECT, {}}, p2{G_PROJECT, {}}, sc{G_SCALE, {}};
+  for (BJob* j : eig) {
+    BOpHost o;
+    o.X = j->V;
+    o.ldx = j->n;
+    o.cnt = j->col + 1;
+    o.w = j->V + (size_t)(j->col + 1) * j->n;
+    o.n = j->n;
+    o.sc = j->sc;
+    o.hsum = j->sc + S_T + (size_t)j->col * (j->m + 1);
+    o.mode = OP_NB;
+    p1.ops.push_back(o);
+    o.mode = OP_GATED;  // (4) second pass only where the DGKS test asks for it
+    p2.ops.push_back(o);
+    BOpHost s;
+    s.w = o.w;
+    s.n = j->n;
+    s.sc = j->sc;
+    s.mode = OP_SCALE_NRM;
+    sc.ops.push_back(s);
+  }
+  for (BJob* j : lz) {
+    BOpHost o;
+    o.X = j->lz[0];
+    o.ldx = j->n;
+    o.cnt = 1;
+    o.w = j->lz[2];
+    o.n = j->n;
+    o.sc = j->sc;
+    o.hist_a = j->d_alpha + (j->step - 1);
+    o.hist_b = j->step != j->nl ? j->d_beta + j->step : nullptr;  // Lanczos.h:64
+    o.flags = j->d_flags;
+    o.kstep = j->step;
+    o.mode = OP_LZ;
+    p1.ops.push_back(o);
+  }
+  post.push_back(p1);
+  post.push_back(p2);
+  post.push_back(sc);
+  issue(pre, &hv, post);
+  ++sweeps;
+  // (5) restart-cycle ends and finished Lanczos runs queue the copy of
+  // their scalars and sit out until it has arrived
+  for (BJob* j : eig) {
+    ++j->n_av;
+    if (j->col + 1 < j->m) ++j->col;
+    else read_back(*j, j->sc + S_NRM, (size_t)(S_T - S_NRM) + (size_t)j->m * (j->m + 1));
+  }
+  for (BJob* j : lz) {
+    if (j->step == j->nl) read_back(*j, j->d_alpha, 2 * (size_t)j->nl + 2);
+    else ++j->step;
+  }
+  int active = 0;
+  for (auto& jp : jobs)
+    if (jp && jp->state == JOB_ACTIVE) ++active;
+  return active;
+}
+
+void Batch::finish_lanczos(BJob& j) {
+  const int nl = j.nl;
+  const double* hist = j.h_read;
+  int fl[2];
+  std::memcpy(fl, hist + 2 * nl + 1, sizeof(fl));
+  j.nlanc = fl[1];
+  for (int q = 0; q < nl; ++q) j.alpha[q] = q < j.nlanc ? hist[q] : 0.0;
+  for (int q = 0; q <= nl; ++q) j.beta[q] = (q >= 1 && q <= j.nlanc && q != nl) ? hist[nl + q] : 0.0;
+  j.state = JOB_DONE;
+  release_job(j);  // the coefficients are on the host now
+  j.lz[0] = j.lz[1] = j.lz[2] = nullptr;
+  j.sc = nullptr;
+}
+
 // End of a restart cycle of one eigensolver run: Ritz values of the
 // projected matrix, convergence test, then a thick restart or the result
 // (the single-sector eigensolver's rules).
 void Batch::end_cycle(BJob& j) {
   const int m = j.m, nev = j.nev;
   const int64_t n = j.n;
+  const double* Tcol = j.h_read + (S_T - S_NRM);
   for (int c = j.k; c < m; ++c)
     for (int i = 0; i <= c; ++i) {
-      j.T[(size_t)i * m + c] = j.Tc[(size_t)c * (m + 1) + i];
-      j.T[(size_t)c * m + i] = j.Tc[(size_t)c * (m + 1) + i];
+      j.T[(size_t)i * m + c] = Tcol[(size_t)c * (m + 1) + i];
+      j.T[(size_t)c * m + i] = Tcol[(size_t)c * (m + 1) + i];
     }
-  const double bres = j.nr_last;
+  const double bres = j.h_read[0];
   std::vector<double> A = j.T, theta;
   sym_eig(m, A, theta);
   const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
@@ -640,36 +907,9 @@ void Batch::end_cycle(BJob& j) {
     j.T[(size_t)i * m + kk] = si;
     j.T[(size_t)kk * m + i] = si;
   }
-  fill_dev(j.dsc + 2 * kMaxBasis, (size_t)(m + 1) * (m + 1));
+  fill_dev(j.sc + S_T, (size_t)(m + 1) * (m + 1));
   j.k = kk;
   j.col = kk;
-}
-
-// Gather map of c / c^+ between two sectors of the batch (the single-sector
-// a_adag maps) applied on the device.
-void Batch::adag(int src, const double* in, int orb, bool annihilate, int dst, double* out) {
-  BSector& a = sector(src);
-  BSector& b = sector(dst);
-  const int Ns = a.model->Ns;
-  EDH_CHECK(orb >= 0 && orb < 2 * Ns, EDH_ERR_ARG, "batch: orbital out of range");
-  int nup2, ndn2;
-  std::vector<int32_t> mu, md;
-  std::vector<int8_t> su, sd;
-  adag_gather_maps(Ns, a.tabs.nup, a.tabs.ndn, orb, annihilate, nup2, ndn2, mu, md, su, sd);
-  EDH_CHECK(nup2 == b.tabs.nup && ndn2 == b.tabs.ndn, EDH_ERR_ARG, "batch: target sector does not match the operator");
-  const size_t bytes = (mu.size() + md.size()) * 4 + su.size() + sd.size() + 64;
-  gtab.ensure(bytes);
-  char* g = gtab.as<char>();
-  int32_t* dmu = (int32_t*)g;
-  int32_t* dmd = dmu + mu.size();
-  int8_t* dsu = (int8_t*)(dmd + md.size());
-  int8_t* dsd = dsu + su.size();
-  EDH_HIP(hipMemcpyAsync(dmu, mu.data(), mu.size() * 4, hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipMemcpyAsync(dmd, md.data(), md.size() * 4, hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipMemcpyAsync(dsu, su.data(), su.size(), hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipMemcpyAsync(dsd, sd.data(), sd.size(), hipMemcpyHostToDevice, stream));
-  dev_separable_gather(in, a.tabs.dn.D, dmu, dmd, dsu, dsd, out, b.tabs.up.D, b.tabs.dn.D, stream);
-  EDH_HIP(hipStreamSynchronize(stream));
 }
 
 // Picks the grouped H*v block size for the sectors of the batch by timing
@@ -696,6 +936,7 @@ int Batch::autotune(const std::vector<int>& sec_ids) {
     hv.push_back({id, a});
     off += secs[id]->tabs.N;
   }
+  std::vector<OpGroup> none;
   hipEvent_t e0, e1;
   EDH_HIP(hipEventCreate(&e0));
   EDH_HIP(hipEventCreate(&e1));
@@ -703,9 +944,9 @@ int Batch::autotune(const std::vector<int>& sec_ids) {
   int choice = hv_block;
   for (int b : {128, 256, 512}) {
     hv_block = b;
-    run_hv(hv);  // warm-up
+    issue(none, &hv, none);  // warm-up
     EDH_HIP(hipEventRecord(e0, stream));
-    for (int r = 0; r < 3; ++r) run_hv(hv);
+    for (int r = 0; r < 3; ++r) issue(none, &hv, none);
     EDH_HIP(hipEventRecord(e1, stream));
     EDH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -717,6 +958,7 @@ int Batch::autotune(const std::vector<int>& sec_ids) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  EDH_HIP(hipStreamSynchronize(stream));
   (void)hipFree(scratch);
   hv_block = choice;
   return choice;

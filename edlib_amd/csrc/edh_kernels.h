@@ -15,8 +15,11 @@ void dev_norm(const double* a, int64_t n, double* partial, double* out, hipStrea
 void dev_scale_inv(const double* x, const double* nrm, double* out, int64_t n, hipStream_t s);
 // w <- w - V V^T w (one classical Gram-Schmidt pass over m columns); h = V^T w
 // (device, m), hsum += h when non-null; *nrm = |w| afterwards (device scalar).
+// nb: also *nb = |w| before the pass; gate: the pass does nothing when *gate
+// is 0; gate_out: *gate_out = 1 when |w| after < 0.717 |w| before (DGKS).
 void dev_orthogonalize(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial, double* h,
-                       double* hsum, double* nrm, hipStream_t s);
+                       double* hsum, double* nrm, hipStream_t s, double* nb = nullptr, const double* gate = nullptr,
+                       double* gate_out = nullptr);
 // Y[:, j] = sum_i C[i + j*m] V[:, i]  (C on the device, column-major m x k)
 void dev_combine(const double* V, int64_t ldv, int m, const double* C, int k, double* Y, int64_t ldy, int64_t n,
                  hipStream_t s);

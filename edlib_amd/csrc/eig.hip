@@ -1,7 +1,8 @@
 // Eigensolver replacing ARPACK's dsaupd/dseupd in Storage::diag
-// (Storage.h:43-146): thick-restart Lanczos with full (two-pass classical
-// Gram-Schmidt) reorthogonalisation, Krylov basis resident on the device, the
-// projected matrix diagonalised on the host.  Parameters follow the
+// (Storage.h:43-146): thick-restart Lanczos with full reorthogonalisation
+// (classical Gram-Schmidt, a second pass when the DGKS test of ARPACK's
+// dsaitr asks for it), Krylov basis resident on the device, the projected
+// matrix diagonalised on the host.  Parameters follow the
 // reference: ncv = min(NCV, ntot), nev = min(NEV, ncv-1), which = "SA",
 // convergence when the Ritz residual bound <= tol * max(|theta|, eps^(2/3)).
 #include <algorithm>
@@ -227,11 +228,12 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
   c.krylov[1].ensure((size_t)(m + 1) * vbytes);
   c.partial.ensure((size_t)kMaxPartials * kMaxBasis * sizeof(double));
   // device scalars: hsum columns (m x (m+1)), norms (m+1), h scratch
-  c.small.ensure((size_t)(m * (m + 1) + (m + 1) + kMaxBasis + kMaxBasis * kMaxBasis) * sizeof(double));
+  c.small.ensure((size_t)(m * (m + 1) + (m + 1) + kMaxBasis + kMaxBasis * kMaxBasis + 2) * sizeof(double));
   double* Tcol = c.small.as<double>();             // column j at Tcol + j*(m+1)
   double* norms = Tcol + (size_t)m * (m + 1);      // norm after orthogonalising column j
   double* hs = norms + (m + 1);                    // scratch h
   double* Cdev = hs + kMaxBasis;                   // combine coefficients
+  double* dgks = Cdev + kMaxBasis * kMaxBasis;     // norm before the first pass, second-pass gate
   double* V = c.krylov[0].as<double>();
   double* Valt = c.krylov[1].as<double>();
   double* partial = c.partial.as<double>();
@@ -262,8 +264,9 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
       launch_hv(c.dev, a, s);
       ++*n_av;
       double* hcol = Tcol + (size_t)j * (m + 1);
-      dev_orthogonalize(V, n, j + 1, w, n, partial, hs, hcol, norms + j, s);
-      dev_orthogonalize(V, n, j + 1, w, n, partial, hs, hcol, norms + j, s);
+      // classical Gram-Schmidt, repeated only when the DGKS test asks for it
+      dev_orthogonalize(V, n, j + 1, w, n, partial, hs, hcol, norms + j, s, dgks, nullptr, dgks + 1);
+      dev_orthogonalize(V, n, j + 1, w, n, partial, hs, hcol, norms + j, s, nullptr, dgks + 1, nullptr);
       dev_scale_inv(w, norms + j, w, n, s);
     }
     EDH_HIP(hipMemcpyAsync(Tc.data(), Tcol, Tc.size() * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -482,9 +482,11 @@ __global__ __launch_bounds__(512) void hv_batch_kernel(const HvBatchJob* __restr
     }
     const HvBatchJob& J = jobs[lo];
     const HvParams& p = J.p;
+    if (p.done && *p.done) continue;  // a stopped Lanczos run (uniform over the block)
     const int u = (int)(g - J.row0);
     const int Ddn = p.Ddn;
-    const double vscale = p.vscale, wscale = p.wscale;
+    const double vscale = p.dscale ? p.dscale[0] : p.vscale;
+    const double wscale = p.dscale ? p.dscale[1] : p.wscale;
     const double* __restrict__ V = p.v;
     const size_t rowoff = (size_t)u * Ddn;
     if (LDS) {

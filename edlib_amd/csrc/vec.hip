@@ -96,10 +96,19 @@ __global__ void lanczos_init_kernel(double* sc, int* st) {
 }
 
 // ---- Gram-Schmidt against a block of basis vectors (eigensolver) ----------
-// h_partial[blk * m + i] = sum_e V[i][e] w[e]  for i < m (m <= kMaxBasis)
-__global__ void multi_dot_kernel(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial) {
-  __shared__ double s_acc[kMaxBasis][16];
+// A pass is skipped when a gate is given and reads 0 (ARPACK dsaitr's DGKS
+// rule: the second pass runs only when the first one removed more than
+// 1 - 0.717 of the norm).
+__device__ __forceinline__ bool gated_off(const double* gate) { return gate && *gate == 0.0; }
+
+// h_partial[blk * ld + i] = sum_e V[i][e] w[e] for i < m (m <= kMaxBasis);
+// with_nb: slot m of each block holds sum_e w[e]^2 (ld = m + 1, else m)
+__global__ void multi_dot_kernel(const double* V, int64_t ldv, int m, const double* w, int64_t n, double* partial,
+                                 int with_nb, const double* gate) {
+  __shared__ double s_acc[kMaxBasis + 1][16];
+  if (gated_off(gate)) return;
   double acc[kMaxBasis];
+  double nb = 0.0;
 #pragma unroll
   for (int i = 0; i < kMaxBasis; ++i) acc[i] = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
@@ -107,6 +116,7 @@ __global__ void multi_dot_kernel(const double* V, int64_t ldv, int m, const doub
 #pragma unroll
     for (int i = 0; i < kMaxBasis; ++i)
       if (i < m) acc[i] = fma(V[i * ldv + e], we, acc[i]);
+    if (with_nb) nb = fma(we, we, nb);
   }
   const int wid = threadIdx.x >> 6;
 #pragma unroll
@@ -116,24 +126,38 @@ __global__ void multi_dot_kernel(const double* V, int64_t ldv, int m, const doub
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
     if ((threadIdx.x & 63) == 0) s_acc[i][wid] = x;
   }
+  if (with_nb) {
+    double x = nb;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if ((threadIdx.x & 63) == 0) s_acc[m][wid] = x;
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+  const int ld = m + (with_nb ? 1 : 0);
+  for (int i = threadIdx.x; i < ld; i += blockDim.x) {
     double t = 0.0;
     for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += s_acc[i][q];
-    partial[blockIdx.x * (int64_t)m + i] = t;
+    partial[blockIdx.x * (int64_t)ld + i] = t;
   }
 }
 
-// h[i] = sum over blocks of partial[blk*m + i]; also accumulates into hsum if given.
-__global__ void multi_reduce_kernel(const double* partial, int nblk, int m, double* h, double* hsum) {
+// h[i] = sum over blocks of partial[blk*ld + i]; also accumulates into hsum
+// if given; with_nb: *nb = sqrt of the sum of slot m.
+__global__ void multi_reduce_kernel(const double* partial, int nblk, int m, double* h, double* hsum, int with_nb,
+                                    double* nb, const double* gate) {
   __shared__ double s_red[16];
-  for (int i = 0; i < m; ++i) {
+  if (gated_off(gate)) return;
+  const int ld = m + (with_nb ? 1 : 0);
+  for (int i = 0; i < ld; ++i) {
     double acc = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partial[b * (int64_t)m + i];
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) acc += partial[b * (int64_t)ld + i];
     double t = block_sum(acc, s_red);
     if (threadIdx.x == 0) {
-      h[i] = t;
-      if (hsum) hsum[i] += t;
+      if (i < m) {
+        h[i] = t;
+        if (hsum) hsum[i] += t;
+      } else {
+        *nb = sqrt(t);
+      }
     }
     __syncthreads();
   }
@@ -141,9 +165,10 @@ __global__ void multi_reduce_kernel(const double* partial, int nblk, int m, doub
 
 // w -= sum_i h[i] V[i]; optional partial of |w|^2
 __global__ void multi_axpy_kernel(const double* V, int64_t ldv, int m, const double* h, double* w, int64_t n,
-                                  double* partial) {
+                                  double* partial, const double* gate) {
   __shared__ double s_h[kMaxBasis];
   __shared__ double s_red[16];
+  if (gated_off(gate)) return;
   for (int i = threadIdx.x; i < m; i += blockDim.x) s_h[i] = h[i];
   __syncthreads();
   double acc = 0.0;
@@ -156,6 +181,20 @@ __global__ void multi_axpy_kernel(const double* V, int64_t ldv, int m, const dou
   if (partial) {
     double t = block_sum(acc, s_red);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
+  }
+}
+
+// *nrm = sqrt(sum of the partials); gate_out: *gate_out = (*nrm < 0.717 *nb)
+__global__ void ortho_norm_kernel(const double* partial, int n, double* nrm, const double* nb, double* gate_out,
+                                  const double* gate) {
+  __shared__ double s_red[16];
+  if (gated_off(gate)) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  double t = block_sum(acc, s_red);
+  if (threadIdx.x == 0) {
+    *nrm = sqrt(t);
+    if (gate_out) *gate_out = *nrm < 0.717 * *nb ? 1.0 : 0.0;
   }
 }
 
@@ -217,17 +256,15 @@ void dev_scale_inv(const double* x, const double* nrm, double* out, int64_t n, h
 }
 
 void dev_orthogonalize(const double* V, int64_t ldv, int m, double* w, int64_t n, double* partial, double* h,
-                       double* hsum, double* nrm, hipStream_t s) {
-  if (m <= 0) {
-    dev_norm(w, n, partial, nrm, s);
-    return;
-  }
-  EDH_CHECK(m <= kMaxBasis, EDH_ERR_LIMIT, "Krylov basis larger than the device limit");
-  int g = reduce_grid(n);
-  hipLaunchKernelGGL(multi_dot_kernel, dim3(g), dim3(kVecBlock), 0, s, V, ldv, m, w, n, partial);
-  hipLaunchKernelGGL(multi_reduce_kernel, dim3(1), dim3(kVecBlock), 0, s, partial, g, m, h, hsum);
-  hipLaunchKernelGGL(multi_axpy_kernel, dim3(g), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n, partial);
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kVecBlock), 0, s, partial, g, nrm, 1);
+                       double* hsum, double* nrm, hipStream_t s, double* nb, const double* gate, double* gate_out) {
+  EDH_CHECK(m >= 1 && m <= kMaxBasis, EDH_ERR_LIMIT, "Krylov basis size outside the device limit");
+  EDH_CHECK(!gate_out || nb, EDH_ERR_INTERNAL, "DGKS gate needs the norm before the pass");
+  const int g = reduce_grid(n);
+  const int with_nb = nb ? 1 : 0;
+  hipLaunchKernelGGL(multi_dot_kernel, dim3(g), dim3(kVecBlock), 0, s, V, ldv, m, w, n, partial, with_nb, gate);
+  hipLaunchKernelGGL(multi_reduce_kernel, dim3(1), dim3(kVecBlock), 0, s, partial, g, m, h, hsum, with_nb, nb, gate);
+  hipLaunchKernelGGL(multi_axpy_kernel, dim3(g), dim3(kVecBlock), 0, s, V, ldv, m, h, w, n, partial, gate);
+  hipLaunchKernelGGL(ortho_norm_kernel, dim3(1), dim3(kVecBlock), 0, s, partial, g, nrm, nb, gate_out, gate);
   EDH_HIP(hipGetLastError());
 }
 

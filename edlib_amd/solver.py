@@ -184,14 +184,18 @@ class BatchedSolve:
             self.pairs = _order_pairs(pairs)
             self.e0 = self.pairs[0][0]
             self.Z = sum(math.exp(-(e - self.e0) * p.beta) for e, _, _, _ in self.pairs)
+            reqs, meta = [], []
             for e, sec, j, k in self.pairs:
                 bf = math.exp(-(e - self.e0) * p.beta)
                 if not _passes(bf, p.cutoff):
                     continue
                 for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
-                    jid, norm = self.b.submit_gf(j, k, o + s * self.Ns, ann, self.sid[nxt], p.nlanc)
-                    if jid >= 0:
-                        self.gf_jobs.append((jid, norm, e, o, s, ann))
+                    reqs.append((j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]))
+                    meta.append((e, o, s, ann))
+            ids, norms = self.b.submit_gf(reqs, p.nlanc) if reqs else ([], [])
+            for jid, norm, (e, o, s, ann) in zip(ids, norms, meta):
+                if jid >= 0:
+                    self.gf_jobs.append((jid, norm, e, o, s, ann))
             self.phase = "gf"
         if self.phase == "gf" and self._all_done([j[0] for j in self.gf_jobs]):
             wn = matsubara(p.beta, p.nomega)

@@ -199,12 +199,14 @@ int edh_batch_submit_eig(edh_batch* b, int sec_id, int nev, int ncv, double tol,
                          int* job_id);
 /* Lanczos run (Lanczos::lanczos, Lanczos.h:36-77) from the normalised v0. */
 int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl, int mem, int* job_id);
-/* Green's-function start vector c(+)_orb |psi> of eigenpair `index` of a
- * finished eigensolver job into sector dst_sec, *norm = <out|out>, and a
- * Lanczos run from out/sqrt(norm) when norm > 1e-10, else *job_id = -1
- * (GreensFunction.h:395-416,437-458 for one orbital). */
-int edh_batch_submit_gf(edh_batch* b, int eig_job, int index, int orb, int annihilate, int dst_sec, int nl,
-                        double* norm, int* job_id);
+/* Green's-function start vectors, `count` of them: vector r is
+ * c(+)_{orbs[r]} |psi> of eigenpair indices[r] of the finished eigensolver
+ * job eig_jobs[r], in sector dst_secs[r] (annihilate[r] selects c).
+ * norms[r] = <out|out>; when it exceeds 1e-10 a Lanczos run of nl steps
+ * starts from out/sqrt(norms[r]) and job_ids[r] is its id, else -1
+ * (GreensFunction.h:395-416,437-458 for one orbital each). */
+int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
+                        const int* annihilate, const int* dst_secs, int nl, double* norms, int* job_ids);
 /* Runs up to nsweeps sweeps (stops early when nothing is active). */
 int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
 /* state EDH_JOB_*, kind EDH_JOB_EIG/LANCZOS, dimension, count = nev or
