@@ -81,8 +81,10 @@ _SIGS = {
     "edh_batch_eig_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
     "edh_batch_lanczos_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
     "edh_batch_free_job": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "edh_batch_cancel": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edh_batch_memory": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "edh_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "edh_batch_cf_many": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _ip, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]),
     "edh_batch_cf_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, _dp, _dp, _dp, _dp]),
 }

@@ -106,6 +106,9 @@ class Batch:
         check(_lib.lib().edh_batch_lanczos_result(self._h, jid, _d(a), _d(b), ctypes.byref(nlanc)))
         return nlanc.value, a, b
 
+    def cancel(self, jid):
+        check(_lib.lib().edh_batch_cancel(self._h, jid))
+
     def free_job(self, jid):
         check(_lib.lib().edh_batch_free_job(self._h, jid))
 
@@ -120,6 +123,22 @@ class Batch:
         check(_lib.lib().edh_batch_autotune(self._h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                             ctypes.byref(blk)))
         return blk.value
+
+    def cf_many(self, job_ids, ev, isign, expb, excited, wn):
+        """Continued fractions of finished Lanczos runs at z = excited*isign + i wn;
+        returns a (len(job_ids), len(wn)) complex array."""
+        k, nz = len(job_ids), len(wn)
+        if k == 0:
+            return np.zeros((0, nz), dtype=complex)
+        ids = np.ascontiguousarray(job_ids, dtype=np.int32)
+        isg = np.ascontiguousarray(isign, dtype=np.int32)
+        evs, exb, exc = (np.ascontiguousarray(x, dtype=np.float64) for x in (ev, expb, excited))
+        w = np.ascontiguousarray(wn, dtype=np.float64)
+        re, im = np.zeros((k, nz)), np.zeros((k, nz))
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        check(_lib.lib().edh_batch_cf_many(self._h, k, ip(ids), _d(evs), ip(isg), _d(exb), _d(exc), nz, _d(w),
+                                           _d(re), _d(im)))
+        return re + 1j * im
 
     def cf_eval(self, z, ev, nlanc, isign, expb, alpha, beta):
         z = np.asarray(z, dtype=np.complex128).reshape(-1)

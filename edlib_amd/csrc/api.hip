@@ -772,7 +772,7 @@ int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs,
     EDH_HIP(hipMemcpyAsync(evecs, j.evecs, (size_t)j.nev * j.n * sizeof(double), hipMemcpyDeviceToHost, B.stream));
     EDH_HIP(hipStreamSynchronize(B.stream));
   }
-  EDH_CHECK(j.state == edh::JOB_DONE, EDH_ERR_NOCONV, "batch: eigensolver did not converge");
+  EDH_CHECK(j.state == edh::JOB_DONE, EDH_ERR_STATE, "batch: eigensolver job was cancelled");
   API_END
 }
 
@@ -783,6 +783,13 @@ int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* be
   if (nlanc) *nlanc = j.nlanc;
   if (alpha) std::copy(j.alpha.begin(), j.alpha.end(), alpha);
   if (beta) std::copy(j.beta.begin(), j.beta.end(), beta);
+  API_END
+}
+
+int edh_batch_cancel(edh_batch* b, int job_id) {
+  API_BEGIN
+  edh::BJob& j = use_batch(b).job(job_id);
+  if (j.state == edh::JOB_ACTIVE) j.state = edh::JOB_FAILED;
   API_END
 }
 
@@ -806,6 +813,52 @@ int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block) {
   std::vector<int> ids(sec_ids, sec_ids + nsec);
   const int r = B.autotune(ids);
   if (block) *block = r;
+  API_END
+}
+
+int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
+                      const double* expb, const double* excited, int nz, const double* wn, double* out_re,
+                      double* out_im) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(njobs >= 0 && nz >= 0, EDH_ERR_ARG, "bad argument");
+  if (njobs == 0 || nz == 0) return EDH_OK;
+  EDH_CHECK(job_ids && ev && isign && expb && excited && wn && out_re && out_im, EDH_ERR_ARG, "null argument");
+  std::vector<size_t> off(njobs);
+  size_t tot = 0;
+  for (int j = 0; j < njobs; ++j) {
+    edh::BJob& J = B.job(job_ids[j]);
+    EDH_CHECK(J.kind == edh::JOB_LZ && J.state == edh::JOB_DONE && J.nlanc >= 1, EDH_ERR_STATE,
+              "batch: continued fraction of an unfinished Lanczos run");
+    off[j] = tot;
+    tot += J.alpha.size() + J.beta.size();
+  }
+  std::vector<double> coef(tot);
+  for (int j = 0; j < njobs; ++j) {
+    edh::BJob& J = B.job(job_ids[j]);
+    std::copy(J.alpha.begin(), J.alpha.end(), coef.begin() + off[j]);
+    std::copy(J.beta.begin(), J.beta.end(), coef.begin() + off[j] + J.alpha.size());
+  }
+  const size_t need = tot * 8 + (size_t)nz * 8 + (size_t)njobs * sizeof(edh::CfJob) + 2 * (size_t)njobs * nz * 8 + 64;
+  B.d_scal.ensure(need);
+  char* base = B.d_scal.as<char>();
+  double* dcoef = (double*)base;
+  double* dwn = dcoef + tot;
+  edh::CfJob* djobs = (edh::CfJob*)(dwn + nz);
+  double* dre = (double*)(djobs + njobs);
+  double* dim = dre + (size_t)njobs * nz;
+  std::vector<edh::CfJob> cj(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    edh::BJob& J = B.job(job_ids[j]);
+    cj[j] = {ev[j], expb[j], excited[j], isign[j], J.nlanc, dcoef + off[j], dcoef + off[j] + J.alpha.size()};
+  }
+  EDH_HIP(hipMemcpyAsync(dcoef, coef.data(), tot * 8, hipMemcpyHostToDevice, B.stream));
+  EDH_HIP(hipMemcpyAsync(dwn, wn, (size_t)nz * 8, hipMemcpyHostToDevice, B.stream));
+  EDH_HIP(hipMemcpyAsync(djobs, cj.data(), cj.size() * sizeof(edh::CfJob), hipMemcpyHostToDevice, B.stream));
+  edh::dev_cf_batch(njobs, nz, dwn, djobs, dre, dim, B.stream);
+  EDH_HIP(hipMemcpyAsync(out_re, dre, (size_t)njobs * nz * 8, hipMemcpyDeviceToHost, B.stream));
+  EDH_HIP(hipMemcpyAsync(out_im, dim, (size_t)njobs * nz * 8, hipMemcpyDeviceToHost, B.stream));
+  EDH_HIP(hipStreamSynchronize(B.stream));
   API_END
 }
 

@@ -539,7 +539,7 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
 void Batch::free_job(int id) {
   BJob& j = job(id);
   EDH_CHECK(j.state != JOB_ACTIVE, EDH_ERR_STATE, "batch: job still active");
-  EDH_HIP(hipStreamSynchronize(stream));
+  if (!j.allocs.empty()) EDH_HIP(hipStreamSynchronize(stream));
   release_job(j);
   jobs[id].reset();
 }
@@ -857,25 +857,33 @@ void Batch::end_cycle(BJob& j) {
   std::vector<double> A = j.T, theta;
   sym_eig(m, A, theta);
   const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-  int nconv_now = 0;
+  std::vector<int> done_i;
   for (int i = 0; i < nev; ++i) {
     const double bound = std::fabs(bres * A[(size_t)(m - 1) * m + i]);
-    if (bound <= j.tol * std::max(std::fabs(theta[i]), eps23)) ++nconv_now;
+    if (bound <= j.tol * std::max(std::fabs(theta[i]), eps23)) done_i.push_back(i);
   }
+  const int nconv_now = (int)done_i.size();
   const bool conv = nconv_now == nev;
   ++j.iter;
   if (conv || j.iter >= j.maxitr) {
-    std::vector<double> Cm((size_t)m * nev);
-    for (int c = 0; c < nev; ++c)
-      for (int i = 0; i < m; ++i) Cm[(size_t)c * m + i] = A[(size_t)i * m + c];
-    double* Cd = dalloc(j, Cm.size());
-    EDH_HIP(hipMemcpyAsync(Cd, Cm.data(), Cm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    j.evecs = dalloc(j, (size_t)nev * n);
-    dev_combine(j.V, n, m, Cd, nev, j.evecs, n, n, stream);
+    // after maxitr the converged Ritz pairs are the result (as dseupd with
+    // info = 1, Storage.h:94-129)
+    const int nk = nconv_now;
+    std::vector<double> Cm((size_t)m * std::max(nk, 1));
+    for (int c = 0; c < nk; ++c)
+      for (int i = 0; i < m; ++i) Cm[(size_t)c * m + i] = A[(size_t)i * m + done_i[c]];
+    j.evals.clear();
+    for (int c = 0; c < nk; ++c) j.evals.push_back(theta[done_i[c]]);
+    if (nk > 0) {
+      double* Cd = dalloc(j, Cm.size());
+      EDH_HIP(hipMemcpyAsync(Cd, Cm.data(), Cm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+      j.evecs = dalloc(j, (size_t)nk * n);
+      dev_combine(j.V, n, m, Cd, nk, j.evecs, n, n, stream);
+    }
     EDH_HIP(hipStreamSynchronize(stream));
-    j.evals.assign(theta.begin(), theta.begin() + nev);
-    j.nconv = conv ? nev : 0;
-    j.state = conv ? JOB_DONE : JOB_FAILED;
+    j.nev = nk;
+    j.nconv = nk;
+    j.state = JOB_DONE;
     // the Krylov bases are no longer needed
     for (double* b : {j.V, j.Valt}) {
       auto it = std::find(j.allocs.begin(), j.allocs.end(), (void*)b);

@@ -33,6 +33,15 @@ void dev_separable_gather(const double* in, int64_t ldin, const int32_t* mu, con
                           const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s);
 void dev_separable_scale(const double* in, const double* fu, const double* fd, double* out, int64_t Dup, int64_t Ddn,
                          hipStream_t s);
+// One continued fraction of a batch (device pointers into a packed array).
+struct CfJob {
+  double ev, expb, excited;
+  int isign, nlanc;
+  const double* alpha;
+  const double* beta;
+};
+void dev_cf_batch(int njobs, int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im,
+                  hipStream_t s);
 void dev_cf_eval(int nz, const double* zre, const double* zim, double ev, int nlanc, int isign, double expb,
                  const double* alpha, const double* beta, double* out_re, double* out_im, hipStream_t s);
 

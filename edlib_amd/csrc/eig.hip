@@ -281,26 +281,31 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
     std::vector<double> A = T;
     sym_eig(m, A, theta);
     Y = A;  // columns = eigenvectors of T
-    int nconv_now = 0;
+    std::vector<int> done_i;
     for (int i = 0; i < nev; ++i) {
       const double bound = std::fabs(bres * Y[(size_t)(m - 1) * m + i]);
-      if (bound <= tol * std::max(std::fabs(theta[i]), eps23)) ++nconv_now;
+      if (bound <= tol * std::max(std::fabs(theta[i]), eps23)) done_i.push_back(i);
     }
+    const int nconv_now = (int)done_i.size();
     const bool conv = nconv_now == nev;
     if (conv || iter + 1 >= maxitr) {
-      std::vector<double> Cm((size_t)m * nev);
-      for (int j = 0; j < nev; ++j)
-        for (int i = 0; i < m; ++i) Cm[(size_t)j * m + i] = Y[(size_t)i * m + j];
-      EDH_HIP(hipMemcpyAsync(Cdev, Cm.data(), Cm.size() * sizeof(double), hipMemcpyHostToDevice, s));
-      c.evecs.ensure((size_t)nev * vbytes);
-      dev_combine(V, n, m, Cdev, nev, c.evecs.as<double>(), n, n, s);
-      if (evecs_host)
-        EDH_HIP(hipMemcpyAsync(evecs_host, c.evecs.p, (size_t)nev * vbytes, hipMemcpyDeviceToHost, s));
+      // Like dsaupd/dseupd after maxitr (info = 1, Storage.h:94-129): the
+      // converged Ritz pairs are returned and nconv says how many there are.
+      const int nk = nconv_now;
+      std::vector<double> Cm((size_t)m * std::max(nk, 1));
+      for (int q = 0; q < nk; ++q)
+        for (int i = 0; i < m; ++i) Cm[(size_t)q * m + i] = Y[(size_t)i * m + done_i[q]];
+      if (nk > 0) {
+        EDH_HIP(hipMemcpyAsync(Cdev, Cm.data(), (size_t)m * nk * sizeof(double), hipMemcpyHostToDevice, s));
+        c.evecs.ensure((size_t)nk * vbytes);
+        dev_combine(V, n, m, Cdev, nk, c.evecs.as<double>(), n, n, s);
+        if (evecs_host)
+          EDH_HIP(hipMemcpyAsync(evecs_host, c.evecs.p, (size_t)nk * vbytes, hipMemcpyDeviceToHost, s));
+      }
       EDH_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i < nev; ++i) evals[i] = theta[i];
-      c.n_evecs = nev;
-      *nconv = conv ? nev : 0;
-      EDH_CHECK(conv, EDH_ERR_NOCONV, "eigsh: maximum number of restarts reached");
+      for (int q = 0; q < nk; ++q) evals[q] = theta[done_i[q]];
+      c.n_evecs = nk;
+      *nconv = nk;
       return;
     }
     // thick restart: keep kk lowest Ritz vectors, continue from the residual

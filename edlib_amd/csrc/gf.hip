@@ -84,6 +84,52 @@ __global__ void cf_kernel(int nz, const double* zre, const double* zim, double e
   out_im[q] = res.im;
 }
 
+// Many continued fractions at once: block row j evaluates job j at the
+// points z = excited_j * isign_j + i wn[q] (the same recurrence as cf_kernel).
+__global__ void cf_batch_kernel(int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const CfJob& J = jobs[blockIdx.y];
+  if (q >= nz) return;
+  const cplx z = {J.excited * J.isign, wn[q]};
+  const double big = (double)FLT_MAX / 2.0;
+  const double shift = 1.0 / ((double)FLT_MAX / 1000.0);
+  const double* alpha = J.alpha;
+  const double* beta = J.beta;
+  const int isign = J.isign;
+  cplx res;
+  if (J.nlanc == 1) {
+    const cplx dl0 = {z.re - alpha[0] * isign, z.im};
+    res = cdiv({J.ev * J.expb, 0.0}, dl0);
+  } else {
+    const int n = J.nlanc;
+    const cplx dl1 = {z.re - alpha[n - 1] * isign, z.im};
+    const cplx dl2 = {z.re - alpha[n - 2] * isign, z.im};
+    cplx dA = dl1;
+    cplx dB = csub(cmul(dl2, dl1), {beta[n - 1] * beta[n - 1], 0.0});
+    for (int i = n - 3; i >= 0; --i) {
+      const cplx dli = {z.re - alpha[i] * isign, z.im};
+      const double b2 = beta[i + 1] * beta[i + 1];
+      cplx d = csub(cmul(dli, dB), cscale(dA, b2));
+      if (hypot(d.re, d.im) > big && i != 0) {
+        d = cscale(d, shift);
+        dB = cscale(dB, shift);
+      }
+      dA = dB;
+      dB = d;
+    }
+    res = cdiv(cscale(dA, J.ev * J.expb), dB);
+  }
+  out_re[(size_t)blockIdx.y * nz + q] = res.re;
+  out_im[(size_t)blockIdx.y * nz + q] = res.im;
+}
+
+void dev_cf_batch(int njobs, int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im,
+                  hipStream_t s) {
+  if (njobs <= 0 || nz <= 0) return;
+  hipLaunchKernelGGL(cf_batch_kernel, dim3((nz + 127) / 128, njobs), dim3(128), 0, s, nz, wn, jobs, out_re, out_im);
+  EDH_HIP(hipGetLastError());
+}
+
 void dev_separable_gather(const double* in, int64_t ldin, const int32_t* mu, const int32_t* md, const int8_t* su,
                           const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s) {
   if (Dup * Ddn == 0) return;

@@ -145,7 +145,16 @@ def solve_plain(model, p, device=0, ctx=None):
 
 class BatchedSolve:
     """One impurity solve on a shared Batch, driven phase by phase so that
-    several solves can share the batch's sweeps (see serve.py)."""
+    several solves can share the batch's sweeps (see serve.py).
+
+    Green's-function runs start while slower sectors are still converging:
+    once a sector's eigenpairs are known, every pair within the Boltzmann
+    window of the lowest energy known so far gets its start vectors.  The
+    lowest energy can only fall, so every pair that passes the final cutoff
+    has been started; runs of pairs that fall out of the window are
+    cancelled.  Contributions are summed in the pair order of the final
+    eigenpair list, so the result does not depend on when sectors finished.
+    """
 
     def __init__(self, batch, model, p):
         self.b = batch
@@ -155,9 +164,13 @@ class BatchedSolve:
         self.secs = [s for s in sector_list(self.Ns, p) if binom(self.Ns, s[0]) * binom(self.Ns, s[1]) > 0]
         self.sid = {}
         self.eig_jobs = []
-        self.gf_jobs = []
+        self.known = []      # (E, sector index, sector, eig job, k) of finished sectors
+        self.seen = set()    # finished eig jobs
+        self.gf = {}         # (eig job, k, orbital, spin, annihilate) -> (gf job or -1, norm)
         self.phase = "new"
         self.result = None
+        c = p.cutoff
+        self.window = math.inf if c <= 0 else (-math.log(c) / p.beta) * (1 + 1e-9) + 1e-9
 
     def start(self):
         for sec in self.secs:
@@ -168,53 +181,82 @@ class BatchedSolve:
                                                              self.p.maxitr, self.p.seed + i)))
         self.phase = "eig"
 
-    def _all_done(self, jobs):
-        return all(self.b.job_info(j)["state"] != 0 for j in jobs)
+    def _collect(self):
+        new = False
+        for idx, (sec, j) in enumerate(self.eig_jobs):
+            if j in self.seen or self.b.job_info(j)["state"] == 0:
+                continue
+            self.seen.add(j)
+            ev, _ = self.b.eig_result(j)
+            for k in range(len(ev)):
+                self.known.append((float(ev[k]), idx, sec, j, k))
+            new = True
+        return new
+
+    def _submit_window(self):
+        if not self.known:
+            return
+        e0 = min(q[0] for q in self.known)
+        reqs, keys = [], []
+        for e, _, sec, j, k in self.known:
+            if e - e0 > self.window:
+                continue
+            for o, s, ann, nxt in _gf_targets(self.Ns, sec, self.p.orbitals):
+                key = (j, k, o, s, ann)
+                if key not in self.gf:
+                    reqs.append((j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]))
+                    keys.append(key)
+        if reqs:
+            ids, norms = self.b.submit_gf(reqs, self.p.nlanc)
+            for key, jid, norm in zip(keys, ids, norms):
+                self.gf[key] = (jid, norm)
 
     def advance(self):
-        """Moves to the next phase when the current one has finished;
-        returns True when the solve is complete."""
+        """Moves the solve on; returns True when it is complete."""
         p = self.p
-        if self.phase == "eig" and self._all_done([j for _, j in self.eig_jobs]):
-            pairs = []
-            for sec, j in self.eig_jobs:
-                ev, _ = self.b.eig_result(j)
-                for k in range(len(ev)):
-                    pairs.append((float(ev[k]), sec, j, k))
-            self.pairs = _order_pairs(pairs)
-            self.e0 = self.pairs[0][0]
-            self.Z = sum(math.exp(-(e - self.e0) * p.beta) for e, _, _, _ in self.pairs)
-            reqs, meta = [], []
-            for e, sec, j, k in self.pairs:
-                bf = math.exp(-(e - self.e0) * p.beta)
-                if not _passes(bf, p.cutoff):
-                    continue
-                for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
-                    reqs.append((j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]))
-                    meta.append((e, o, s, ann))
-            ids, norms = self.b.submit_gf(reqs, p.nlanc) if reqs else ([], [])
-            for jid, norm, (e, o, s, ann) in zip(ids, norms, meta):
-                if jid >= 0:
-                    self.gf_jobs.append((jid, norm, e, o, s, ann))
-            self.phase = "gf"
-        if self.phase == "gf" and self._all_done([j[0] for j in self.gf_jobs]):
-            wn = matsubara(p.beta, p.nomega)
-            norb = len(p.orbitals)
-            Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
-            Gl = np.zeros_like(Gg)
-            for jid, norm, e, o, s, ann in self.gf_jobs:
-                nlanc, a, bb = self.b.lanczos_result(jid, p.nlanc)
-                isign = -1 if ann else 1
-                g = self.b.cf_eval(1j * wn + e * isign, norm, nlanc, isign, _expb(p.beta, e, self.e0), a, bb)
-                (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g
-                self.b.free_job(jid)
-            for _, j in self.eig_jobs:
-                self.b.free_job(j)
-            for sid in self.sid.values():
-                self.b.remove_sector(sid)
-            self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
-                           "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(self.gf_jobs)}
-            self.phase = "done"
+        if self.phase == "eig":
+            if self._collect():
+                self._submit_window()
+            if len(self.seen) == len(self.eig_jobs):
+                # final eigenpair order: energy, ties in sector order
+                pairs = _order_pairs([(e, idx, sec, j, k) for e, idx, sec, j, k in
+                                      sorted(self.known, key=lambda q: (q[1], q[4]))])
+                self.pairs = pairs
+                self.e0 = pairs[0][0]
+                self.Z = sum(math.exp(-(q[0] - self.e0) * p.beta) for q in pairs)
+                self.used = []
+                for e, _, sec, j, k in pairs:
+                    if not _passes(math.exp(-(e - self.e0) * p.beta), p.cutoff):
+                        continue
+                    for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
+                        self.used.append(((j, k, o, s, ann), e))
+                needed = {key for key, _ in self.used}
+                for key, (jid, _) in self.gf.items():
+                    if key not in needed and jid >= 0:
+                        self.b.cancel(jid)
+                        self.b.free_job(jid)
+                self.phase = "gf"
+        if self.phase == "gf":
+            live = [self.gf[key][0] for key, _ in self.used if self.gf[key][0] >= 0]
+            if all(self.b.job_info(j)["state"] != 0 for j in live):
+                wn = matsubara(p.beta, p.nomega)
+                norb = len(p.orbitals)
+                Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
+                Gl = np.zeros_like(Gg)
+                rows = [(self.gf[key], key, e) for key, e in self.used if self.gf[key][0] >= 0]
+                g = self.b.cf_many([r[0][0] for r in rows], [r[0][1] for r in rows],
+                                   [-1 if r[1][4] else 1 for r in rows], [_expb(p.beta, r[2], self.e0) for r in rows],
+                                   [r[2] for r in rows], wn)
+                for q, ((jid, _), (_, _, o, s, ann), _) in enumerate(rows):
+                    (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
+                    self.b.free_job(jid)
+                for _, j in self.eig_jobs:
+                    self.b.free_job(j)
+                for sid in self.sid.values():
+                    self.b.remove_sector(sid)
+                self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
+                               "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows)}
+                self.phase = "done"
         return self.phase == "done"
 
 

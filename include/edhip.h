@@ -162,7 +162,9 @@ int edh_cf_eval(edh_ctx* ctx, int nz, const double* zre, const double* zim, doub
  * on the device (which="SA"); ncv Krylov vectors, convergence when the Ritz
  * residual <= tol*max(|theta|, eps^(2/3)), at most maxitr restarts.  evals
  * (host, nev) ascending; evecs (host, n*nev column-major) may be NULL.
- * Special cases of Storage::diag: n == 1 -> (diag, [1]). */
+ * Special cases of Storage::diag: n == 1 -> (diag, [1]).  After maxitr
+ * restarts the converged pairs are returned and *nconv counts them (ARPACK
+ * info = 1). */
 int edh_eigsh(edh_ctx* ctx, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals,
               double* evecs, int* nconv, int* n_av);
 
@@ -212,16 +214,28 @@ int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
 /* state EDH_JOB_*, kind EDH_JOB_EIG/LANCZOS, dimension, count = nev or
  * Lanczos steps, n_av = H*v applications. */
 int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t* n, int* count, int* n_av);
-/* evals (nev) ascending; evecs (host, n*nev column-major) may be NULL. */
+/* evals (count of edh_batch_job_info, ascending); evecs (host, n*count
+ * column-major) may be NULL.  After maxitr restarts only the converged pairs
+ * are returned (nconv < the requested nev), as Storage::diag does. */
 int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs, int* nconv);
 /* alpha[nlanc], beta[nl+1] (beta[0] unused), as edh_lanczos. */
 int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc);
+/* Stops a process (it takes no further sweeps; its state becomes
+ * EDH_JOB_FAILED) so that it can be freed. */
+int edh_batch_cancel(edh_batch* b, int job_id);
 int edh_batch_free_job(edh_batch* b, int job_id);
 /* Device bytes held by the batch's sectors and processes. */
 int edh_batch_memory(edh_batch* b, size_t* bytes);
 /* Autotuner: times the grouped H*v over the given sectors for each block
  * size and keeps the fastest (returned in *block). */
 int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block);
+/* Continued fractions of many finished Lanczos runs in one launch
+ * (Lanczos::compute_continued_fraction, Lanczos.h:82-100, for each): run j
+ * at z = excited[j]*isign[j] + i*wn[q], q < nz, weight ev[j]*expb[j];
+ * out_re/out_im are njobs x nz row-major. */
+int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
+                      const double* expb, const double* excited, int nz, const double* wn, double* out_re,
+                      double* out_im);
 /* Lanczos::get_frac_point on the batch's stream (as edh_cf_eval). */
 int edh_batch_cf_eval(edh_batch* b, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign,
                       double expb, const double* alpha, const double* beta, double* out_re, double* out_im);

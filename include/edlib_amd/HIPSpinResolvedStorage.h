@@ -108,11 +108,9 @@ class HIPSpinResolvedStorage {
     int nconv = 0, nav = 0;
     int rc = edh_eigsh(ctx_.get(), nev_, ncv_, 1e-14, 1000, 0x45444c6962ull, ev.data(), eval_only_ ? nullptr : X.data(),
                        &nconv, &nav);
-    if (rc == EDH_ERR_NOCONV) {
-      std::cout << "Maximum number of iterations reached." << std::endl;
-      return 1;
-    }
     edh_throw(rc);
+    if (nconv < std::min(nev_, (int)std::min<int64_t>(ncv_, n_) - 1) && n_ > 1)
+      std::cout << "Maximum number of iterations reached." << std::endl;  // Storage.h:122-124
     evals_.assign(ev.begin(), ev.begin() + nconv);
     if (!eval_only_) {
       evecs_.resize(nconv);
