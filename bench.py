@@ -11,6 +11,13 @@ per-spin hopping tables T); the diagonal and interaction terms are generated
 on the fly.  value = B / (wall time per step); roofline.achieved = B / (mean
 kernel time from HIP events on the launch stream).
 
+Also reported (same JSON line):
+* pcie_inclusive_gbs: the same H*v called with host vectors (staged over
+  PCIe both ways, as the reference's host-pointer av() contract does);
+* solves: impurity solves/s of the batched engine and of the continuous-
+  batching server against the plain per-sector GPU solve on the same
+  problems, with p50/p99 latency (see edlib_amd/solver.py, serve.py).
+
 Prints ONE JSON line.  --gpus N>1 exits: this build runs on one GPU.
 """
 import argparse
@@ -68,12 +75,96 @@ def cpu_baseline(t, U, xmu, v, w, B):
             "ms_per_call": dt * 1e3}
 
 
+def siam_model(Ns, ml, seed, U=4.0, J=0.7):
+    """Anderson impurity with ml orbitals and Ns - ml bath sites: bath
+    energies U[-2, 2], hybridisations U[0.1, 0.8] (spin-symmetric), impurity
+    levels U[-0.3, 0.3]; Kanamori interaction (U, J, U' = U - 2J) for ml > 1,
+    mu = U/2 (ml = 1) or (5U - 10J)/2 (ml = 3, half filling)."""
+    import edlib_amd as ea
+    rng = np.random.default_rng(seed)
+    nb = Ns - ml
+    eps = np.repeat(rng.uniform(-2, 2, nb)[:, None], 2, axis=1)
+    Vk = np.zeros((ml, nb, 2))
+    for im in range(ml):
+        Vk[im] = rng.uniform(0.1, 0.8, nb)[:, None]
+    H0 = np.zeros((ml, ml, 2))
+    for im in range(ml):
+        H0[im, im, :] = rng.uniform(-0.3, 0.3)
+    if ml > 1:
+        Ut = np.zeros((ml, ml, ml, ml))
+        for i in range(ml):
+            Ut[i, i, i, i] = U
+            for j in range(ml):
+                if i != j:
+                    Ut[i, j, i, j] = U - 2 * J
+                    Ut[i, j, j, i] = J
+                    Ut[i, i, j, j] = J
+        Ut = np.broadcast_to(Ut, (2, 2) + Ut.shape).copy()
+        mu = (5 * U - 10 * J) / 2
+    else:
+        Ut = np.full((2, 2, 1, 1, 1, 1), U)
+        mu = U / 2
+    return ea.SingleImpurityAndersonModel(Ns, ml, eps, Vk, H0, Ut, mu)
+
+
+def solves_section(args):
+    """Batched and served impurity solves against the plain per-sector solve."""
+    from edlib_amd.serve import Server, latency_stats
+    from edlib_amd.solver import SolveParams, solve_batched, solve_plain
+    out = {}
+    # batched: one Anderson impurity + 11 bath sites, all 169 sectors
+    sp = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0,), seed=SEED)
+    m = siam_model(12, 1, SEED + 4)
+    solve_batched(m, sp)  # warm-up (code paths, allocator)
+    t = time.perf_counter()
+    rb = solve_batched(m, sp)
+    tb = time.perf_counter() - t
+    t = time.perf_counter()
+    rp = solve_plain(m, sp)
+    tp = time.perf_counter() - t
+    out["batched"] = {
+        "workload": "AIM 1 orbital + 11 bath sites, U=4, all 169 (N_up, N_dn) sectors, NEV=4, NCV=11, tol 1e-14; "
+                    "G of the impurity orbital from 4 start vectors per pair (spin x particle/hole), beta=10, "
+                    "256 Matsubara points, NLANC=60",
+        "plain_solves_per_s": round(1.0 / tp, 4), "batched_solves_per_s": round(1.0 / tb, 4),
+        "speedup": round(tp / tb, 3), "plain_s": round(tp, 3), "batched_s": round(tb, 3),
+        "max_abs_dG": float(np.abs(rb["G"] - rp["G"]).max()),
+        "max_rel_dE": float(np.max(np.abs(rb["energies"] - rp["energies"]) /
+                                   np.maximum(np.abs(rp["energies"]), 1.0))),
+        "n_gf": rb["n_gf"]}
+    # served: Kanamori 3 orbitals + 6..8 bath sites, Poisson arrivals
+    sp4 = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0, 1, 2), seed=SEED)
+    rng = np.random.default_rng(SEED + 7)
+    reqs = [(r, siam_model(3 + int(rng.integers(6, 9)), 3, SEED + 1000 + r), sp4) for r in range(args.requests)]
+    t = time.perf_counter()
+    plain_sample = [solve_plain(reqs[r][1], sp4) for r in range(2)]
+    tps = (time.perf_counter() - t) / 2
+    srv = Server(0, sweeps_per_tick=8, max_active=args.max_active)
+    done, wall = srv.run_stream(reqs, rate=args.rate, seed=SEED + 7)
+    srv.close()
+    st = latency_stats(done)
+    same = max(float(np.abs(done[r]["G"] - plain_sample[r]["G"]).max()) for r in range(2))
+    out["served"] = {
+        "workload": "AIM 3 orbitals (Kanamori U=4, J=0.7, U'=2.6) + 6..8 bath sites (9..11 sites), %d requests, "
+                    "Poisson arrivals at %.2f/s, NEV=4, G of the 3 impurity orbitals, beta=10, 256 Matsubara "
+                    "points, NLANC=60" % (args.requests, args.rate),
+        "solves_per_s": round(len(done) / wall, 4), "p50_s": round(st["p50_s"], 3), "p99_s": round(st["p99_s"], 3),
+        "plain_solves_per_s": round(1.0 / tps, 4), "speedup": round((len(done) / wall) * tps, 3),
+        "plain_sample": "mean of 2 requests run alone by the plain per-sector solve",
+        "max_abs_dG_vs_plain": same}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--max-active", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -116,6 +207,15 @@ def main():
     ctx.hv_variant = chosen
     achieved = B / (kern_ms * 1e-3) / 1e9
     value = B / wall / 1e9
+    # host vectors: staged over PCIe both ways (the reference's av() contract)
+    import ctypes
+    from edlib_amd import _lib
+    wh = w.copy()
+    t0 = time.perf_counter()
+    _lib.check(_lib.lib().edh_av(ctx.handle, v.ctypes.data_as(ctypes.c_void_p), wh.ctypes.data_as(ctypes.c_void_p),
+                                 n, 0, ea.EDH_MEM_HOST))
+    pcie_gbs = B / (time.perf_counter() - t0) / 1e9
+    del wh
 
     out = {
         "metric": METRIC,
@@ -140,10 +240,13 @@ def main():
                                 "edh::hv_rows_kernel (no up term) + edh::hv_upslab_kernel (two passes)"),
                      "kernel_ms": round(kern_ms, 4), "variants": per_variant},
     }
+    out["pcie_inclusive_gbs"] = round(pcie_gbs, 2)
+    ctx.close()
     if not args.no_cpu_baseline:
         wcpu = w.copy()
         out["cpu_baseline"] = cpu_baseline(t, U, xmu, v, wcpu, B)
-    ctx.close()
+    if not args.no_solves:
+        out["solves"] = solves_section(args)
     print(json.dumps(out), flush=True)
 
 
