@@ -1,6 +1,7 @@
 // C ABI of libedhip (see include/edhip.h).  Every entry catches all C++
 // exceptions and turns them into a status code plus a per-thread message.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "batch.h"
@@ -455,22 +456,63 @@ int lanczos_device(Ctx& c, int nl, double* alpha, double* beta, double* step_ms)
     EDH_HIP(hipEventCreate(&e1));
     EDH_HIP(hipEventRecord(e0, s));
   }
-  for (int k = 1; k <= nl; ++k) {
+  // One Lanczos step = H*v with the alpha partial sums, the alpha reduction,
+  // then axpy + norm and the beta reduction; every scalar and the step number
+  // live on the device.  The step is captured once per buffer rotation
+  // (period 3) as a hipGraph and replayed; EDH_LANCZOS_GRAPH=0 launches the
+  // kernels directly instead.
+  auto step = [&](double* cu, double* pr, double* ou, int k) {
     HvArgs a{};
-    a.v = cur;
-    a.w = out;
-    a.wprev = prev;
-    a.mode = k > 1 ? 1 : 0;
+    a.v = cu;
+    a.w = ou;
+    a.wprev = pr;
+    a.mode = 1;  // step 1: the w scale is 0 and the previous vector is zeroed
     a.dscale = sc;
     a.dot_partial = partial;
     a.done = st + 1;
     launch_hv(c.dev, a, s);
     dev_lanczos_alpha(partial, hv_num_blocks(c.dev), dalpha, k, sc, st, s);
-    dev_lanczos_passB(out, cur, sc + 2, n, partial, st + 1, dbeta, k, nl, sc, st, s);
-    double* t = prev;
-    prev = cur;
-    cur = out;
-    out = t;
+    dev_lanczos_passB(ou, cu, sc + 2, n, partial, st + 1, dbeta, k, nl, sc, st, s);
+  };
+  dev_fill(prev, n, 0.0, s);
+  const char* genv = getenv("EDH_LANCZOS_GRAPH");
+  const bool use_graph = nl >= 4 && !(genv && genv[0] == '0');
+  bool graphed = false;
+  if (use_graph) {
+    hipGraphExec_t ex[3] = {nullptr, nullptr, nullptr};
+    double* rot[3] = {cur, prev, out};
+    bool ok = true;
+    for (int ph = 0; ph < 3 && ok; ++ph) {
+      hipGraph_t g = nullptr;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        ok = false;
+        break;
+      }
+      // phase ph: (cur, prev, out) = rotation^ph of the three buffers
+      step(rot[(3 - ph) % 3], rot[(4 - ph) % 3], rot[(5 - ph) % 3], 0);
+      if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
+        ok = false;
+        break;
+      }
+      if (hipGraphInstantiate(&ex[ph], g, nullptr, nullptr, 0) != hipSuccess) ok = false;
+      (void)hipGraphDestroy(g);
+    }
+    if (ok) {
+      for (int k = 1; k <= nl; ++k) EDH_HIP(hipGraphLaunch(ex[(k - 1) % 3], s));
+      graphed = true;
+    }
+    for (auto& e : ex)
+      if (e) (void)hipGraphExecDestroy(e);
+    (void)hipGetLastError();
+  }
+  if (!graphed) {
+    for (int k = 1; k <= nl; ++k) {
+      step(cur, prev, out, k);
+      double* t = prev;
+      prev = cur;
+      cur = out;
+      out = t;
+    }
   }
   if (step_ms) {
     EDH_HIP(hipEventRecord(e1, s));

@@ -71,7 +71,7 @@ struct BOpHost {
 };
 
 enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8 };
-enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2 };
+enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2, G_REORTH = 3 };
 
 struct OpGroup {
   int kind;

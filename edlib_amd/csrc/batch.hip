@@ -19,7 +19,7 @@
 //
 // Determinism: a process's arithmetic depends only on its own operands.  Its
 // H*v elements do not depend on the grouping, its reductions use its own
-// chunks (kChunk elements each) reduced in a fixed order, and its host-side
+// chunks (a size fixed by its own dimension) reduced in a fixed order, and its host-side
 // logic (projected matrix, restarts) sees only its own numbers.  A result
 // is therefore bit-identical alone and in any mix.
 #include <algorithm>
@@ -34,7 +34,8 @@ namespace edh {
 
 namespace {
 
-constexpr int kChunk = 2048;   // elements per reduction chunk
+constexpr int kChunkMax = 2048;  // elements per reduction chunk (large operands)
+constexpr int kChunkMin = 256;   // ... (small operands: more blocks in flight)
 constexpr int kOpBlock = 256;  // threads per chunk kernel block
 constexpr int kHGroup = 8;     // projections accumulated per pass over a chunk
 constexpr double kDgks = 0.717;
@@ -53,7 +54,7 @@ struct BOp {
   int cnt;
   double* w;
   int64_t n;
-  int chunk0, nch;
+  int chunk0, nch, chunk;
   double* sc;
   double* hsum;
   double* hist_a;
@@ -83,17 +84,39 @@ __device__ __forceinline__ double block_sum256(double x, double* s_red) {
   return t;  // valid on thread 0
 }
 
+// Reduces `cnt` per-thread values over the block (fixed order: shuffle
+// tree per wavefront, then the four wavefronts in order) and writes them to
+// out[0..cnt); one barrier for the whole set.
+template <int K>
+__device__ __forceinline__ void block_store_many(const double (&v)[K], int cnt, double (*s_part)[K], double* out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int g = 0; g < K; ++g) {
+    if (g >= cnt) break;
+    double x = v[g];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) s_part[wave][g] = x;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < cnt) {
+    double t = 0.0;
+    for (int q = 0; q < kOpBlock / 64; ++q) t += s_part[q][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
 // partial[c * kMaxBasis + i] = chunk c's sum of X_i[e] * w[e], i < cnt, and
 // (OP_NB) slot cnt = chunk c's sum of w[e]^2
 __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
                                                          double* __restrict__ partial) {
-  __shared__ double s_red[8];
+  __shared__ double s_part[kOpBlock / 64][kHGroup];
   const int c = blockIdx.x;
   const BOp& o = ops[cjob[c]];
   if (op_skip(o)) return;
   const int lc = c - o.chunk0;
-  const int64_t e0 = (int64_t)lc * kChunk;
-  const int64_t e1 = min(o.n, e0 + kChunk);
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
   const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
   for (int i0 = 0; i0 < tot; i0 += kHGroup) {
     double acc[kHGroup];
@@ -108,13 +131,48 @@ __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__
         else if (i < tot) acc[g] = fma(we, we, acc[g]);
       }
     }
-#pragma unroll
-    for (int g = 0; g < kHGroup; ++g) {
-      if (i0 + g >= tot) break;
-      const double t = block_sum256(acc[g], s_red);
-      if (threadIdx.x == 0) partial[(size_t)c * kMaxBasis + i0 + g] = t;
-    }
+    block_store_many<kHGroup>(acc, min(kHGroup, tot - i0), s_part, partial + (size_t)c * kMaxBasis + i0);
   }
+}
+
+// First pass's update fused with the second pass's projections (cnt <=
+// kFuse): w -= sum_i h[i] X_i; then, from the same registers, chunk c's sums
+// of X_i[e] * w_new[e] in slots i < cnt and of w_new[e]^2 in slot cnt.
+constexpr int kFuse = 24;
+__global__ __launch_bounds__(kOpBlock) void bupdate_dots_kernel(const BOp* __restrict__ ops,
+                                                                const int* __restrict__ cjob,
+                                                                double* __restrict__ partial) {
+  __shared__ double s_part[kOpBlock / 64][kFuse + 1];
+  __shared__ double s_h[kFuse];
+  const int c = blockIdx.x;
+  const BOp& o = ops[cjob[c]];
+  if (op_skip(o)) return;
+  if ((int)threadIdx.x < kFuse) s_h[threadIdx.x] = (int)threadIdx.x < o.cnt ? o.sc[threadIdx.x] : 0.0;
+  __syncthreads();
+  const int lc = c - o.chunk0;
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
+  const int cnt = o.cnt;
+  double acc[kFuse + 1];
+#pragma unroll
+  for (int g = 0; g <= kFuse; ++g) acc[g] = 0.0;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
+    double xv[kFuse];
+#pragma unroll
+    for (int g = 0; g < kFuse; ++g) xv[g] = g < cnt ? o.X[(int64_t)g * o.ldx + e] : 0.0;
+    double x = o.w[e];
+#pragma unroll
+    for (int g = 0; g < kFuse; ++g) x = fma(-s_h[g], xv[g], x);
+    o.w[e] = x;
+#pragma unroll
+    for (int g = 0; g < kFuse; ++g) acc[g] = fma(xv[g], x, acc[g]);
+    acc[kFuse] = fma(x, x, acc[kFuse]);
+  }
+  // slot layout: projections 0..cnt-1, squared norm at cnt
+  double v[kFuse + 1];
+#pragma unroll
+  for (int g = 0; g <= kFuse; ++g) v[g] = g < cnt ? acc[g] : (g == cnt ? acc[kFuse] : 0.0);
+  block_store_many<kFuse + 1>(v, cnt + 1, s_part, partial + (size_t)c * kMaxBasis);
 }
 
 // One block (four wavefronts) per process.  mode 0: h = its chunks'
@@ -130,10 +188,37 @@ __device__ __forceinline__ double wave_chunk_sum(const BOp& o, const double* par
 }
 
 __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ ops, const double* __restrict__ partial,
-                                                      int mode) {
+                                                      int mode, int gated) {
   const BOp& o = ops[blockIdx.x];
-  if (op_skip(o)) return;
+  if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (mode == 2) {
+    // after the fused update: norm in slot cnt; DGKS gate; when the gate is
+    // set, the second pass's projections go to h and into hsum
+    __shared__ int s_gate;
+    if (wave == 0) {
+      const double t = wave_chunk_sum(o, partial, o.cnt, lane);
+      if (lane == 0) {
+        const double nrm = sqrt(t);
+        o.sc[S_NRM] = nrm;
+        o.sc[S_ONE] = 1.0;
+        o.sc[S_NEGNRM] = -nrm;
+        const int g = nrm < kDgks * o.sc[S_NB] ? 1 : 0;
+        o.sc[S_GATE] = g;
+        s_gate = g;
+      }
+    }
+    __syncthreads();
+    if (!s_gate) return;
+    for (int i = wave; i < o.cnt; i += 4) {
+      const double t = wave_chunk_sum(o, partial, i, lane);
+      if (lane == 0) {
+        o.sc[i] = t;
+        if (o.hsum) o.hsum[i] += t;
+      }
+    }
+    return;
+  }
   if (mode == 0) {
     const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
     for (int i = wave; i < tot; i += 4) {
@@ -170,17 +255,17 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
 // The projections are applied eight at a time, so eight loads per element
 // are in flight together.
 __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
-                                                           double* __restrict__ partial) {
+                                                           double* __restrict__ partial, int gated) {
   __shared__ double s_red[8];
   __shared__ double s_h[kMaxBasis + kHGroup];
   const int c = blockIdx.x;
   const BOp& o = ops[cjob[c]];
-  if (op_skip(o)) return;
+  if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
   for (int i = threadIdx.x; i < kMaxBasis + kHGroup; i += blockDim.x) s_h[i] = i < o.cnt ? o.sc[i] : 0.0;
   __syncthreads();
   const int lc = c - o.chunk0;
-  const int64_t e0 = (int64_t)lc * kChunk;
-  const int64_t e1 = min(o.n, e0 + kChunk);
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
   double nsq = 0.0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
     double x = o.w[e];
@@ -205,8 +290,8 @@ __global__ __launch_bounds__(kOpBlock) void bscale_kernel(const BOp* __restrict_
   if (op_skip(o)) return;
   const double f = (o.mode & OP_SCALE_NRM) ? 1.0 / o.sc[S_NRM] : o.scale;
   const int lc = c - o.chunk0;
-  const int64_t e0 = (int64_t)lc * kChunk;
-  const int64_t e1 = min(o.n, e0 + kChunk);
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) o.w[e] *= f;
 }
 
@@ -221,7 +306,15 @@ uint64_t splitmix64(uint64_t& x) {
   return z ^ (z >> 31);
 }
 
-int nchunks(int64_t n) { return (int)((n + kChunk - 1) / kChunk); }
+// Chunk size of an operand of n elements: about n / 256, a power of two in
+// [kChunkMin, kChunkMax] -- it depends on n only, so a process reduces the
+// same way whatever else is in the batch.
+int chunk_of(int64_t n) {
+  int c = kChunkMin;
+  while (c < kChunkMax && (int64_t)c * 256 < n) c *= 2;
+  return c;
+}
+int nchunks(int64_t n) { return (int)((n + chunk_of(n) - 1) / chunk_of(n)); }
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
@@ -614,6 +707,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         o.n = h.n;
         o.chunk0 = c0;
         o.nch = nchunks(h.n);
+        o.chunk = chunk_of(h.n);
         o.sc = h.sc;
         o.hsum = h.hsum;
         o.hist_a = h.hist_a;
@@ -655,12 +749,21 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const int* dcj = (const int*)(db + p.cj_off);
       if (p.kind == G_PROJECT) {
         hipLaunchKernelGGL(bdots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
+      } else if (p.kind == G_REORTH) {
+        // pass 1 projections (with the norm before) -> fused pass 1 update and
+        // pass 2 projections (with the DGKS gate) -> pass 2 update where gated
+        hipLaunchKernelGGL(bdots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
+        hipLaunchKernelGGL(bupdate_dots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 1);
       } else if (p.kind == G_NORM) {
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else {
         hipLaunchKernelGGL(bscale_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
       }
@@ -769,7 +872,7 @@ int Batch::step() {
   // (3) first projection pass: eigensolver against V_0..V_col (into the
   // projected-matrix column, with the norm before for the DGKS test);
   // Lanczos alpha = <v, w>, w -= alpha v, beta = |w| (Lanczos.h:56-65)
-  OpGroup p1{G_PROJECT, {}}, p2{G_PROJECT, {}}, sc{G_SCALE, {}};
+  OpGroup p1{G_PROJECT, {}}, p2{G_PROJECT, {}}, rf{G_REORTH, {}}, sc{G_SCALE, {}};
   for (BJob* j : eig) {
     BOpHost o;
     o.X = j->V;
@@ -780,9 +883,13 @@ int Batch::step() {
     o.sc = j->sc;
     o.hsum = j->sc + S_T + (size_t)j->col * (j->m + 1);
     o.mode = OP_NB;
-    p1.ops.push_back(o);
-    o.mode = OP_GATED;  // (4) second pass only where the DGKS test asks for it
-    p2.ops.push_back(o);
+    if (j->m <= kFuse) {
+      rf.ops.push_back(o);  // both passes in one group (fused)
+    } else {
+      p1.ops.push_back(o);
+      o.mode = OP_GATED;  // (4) second pass only where the DGKS test asks for it
+      p2.ops.push_back(o);
+    }
     BOpHost s;
     s.w = o.w;
     s.n = j->n;
@@ -806,6 +913,7 @@ int Batch::step() {
     p1.ops.push_back(o);
   }
   post.push_back(p1);
+  post.push_back(rf);
   post.push_back(p2);
   post.push_back(sc);
   issue(pre, &hv, post);

@@ -56,9 +56,12 @@ __global__ void axpy_norm_kernel(double* w, const double* x, const double* coef,
 // Lanczos bookkeeping (Lanczos.h:55-65).  sc[0] = scale of the current
 // operand (v_k = s_k W_k), sc[1] = additive scale for the next step,
 // sc[2] = pass-B coefficient (-alpha_k s_k).  st[0] = steps taken, st[1] = stop.
+// k <= 0: the step number is read on the device (st[0] + 1), so one launch
+// sequence serves every step (hipGraph replay).
 __global__ void lanczos_alpha_kernel(const double* partial, int n, double* alpha, int k, double* sc, const int* st) {
   __shared__ double s_red[16];
   if (st[1]) return;
+  if (k <= 0) k = st[0] + 1;
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
   double t = block_sum(acc, s_red);
@@ -71,6 +74,7 @@ __global__ void lanczos_alpha_kernel(const double* partial, int n, double* alpha
 __global__ void lanczos_beta_kernel(const double* partial, int n, double* beta, int k, int nl, double* sc, int* st) {
   __shared__ double s_red[16];
   if (st[1]) return;
+  if (k <= 0) k = st[0] + 1;
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
   double t = block_sum(acc, s_red);
@@ -247,6 +251,16 @@ void dev_norm(const double* a, int64_t n, double* partial, double* out, hipStrea
   int g = reduce_grid(n);
   hipLaunchKernelGGL(dot_partial_kernel, dim3(g), dim3(kVecBlock), 0, s, a, a, n, partial);
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kVecBlock), 0, s, partial, g, out, 1);
+  EDH_HIP(hipGetLastError());
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double x) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) p[e] = x;
+}
+
+void dev_fill(double* p, int64_t n, double x, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(reduce_grid(n)), dim3(kVecBlock), 0, s, p, n, x);
   EDH_HIP(hipGetLastError());
 }
 

@@ -265,3 +265,26 @@ def test_gf_chi_golden_via_device(ctx):
     ref = load_dat("gom1.dat")
     for s in range(2):
         assert np.max(np.abs(G[:, s] - (ref[:, 1] + 1j * ref[:, 2]))) < 1e-10
+
+
+def test_lanczos_graph_replay_matches_eager():
+    """The Lanczos step replayed as hipGraphs gives the directly launched
+    recurrence's coefficients, call after call with eager work in between."""
+    import os
+    Ns = 12
+    ctx = ea.Context(0)
+    ctx.set_model(ea.HubbardModel(Ns, models.torus_hopping(3, 4), np.full(Ns, 8.0), np.full(Ns, 4.0)))
+    n = ctx.set_sector(6, 6)
+    v0 = np.random.default_rng(20261202 + 11).uniform(-1, 1, n)
+    v0 /= np.linalg.norm(v0)
+    os.environ["EDH_LANCZOS_GRAPH"] = "0"
+    try:
+        ref = ctx.lanczos(v0, 40)
+    finally:
+        os.environ.pop("EDH_LANCZOS_GRAPH", None)
+    for rep in range(3):
+        ctx.av(v0)  # eager work between the replays
+        got = ctx.lanczos(v0, 40)
+        assert got[0] == ref[0]
+        assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), rep
+    ctx.close()
