@@ -90,6 +90,14 @@ class Batch:
                                             ctypes.byref(cnt), ctypes.byref(nav)))
         return {"state": st.value, "kind": kind.value, "n": n.value, "count": cnt.value, "n_av": nav.value}
 
+    def job_states(self, jids):
+        """States of many jobs in one call (array of JOB_*)."""
+        ids = np.ascontiguousarray(jids, dtype=np.int32)
+        st = np.zeros(max(len(ids), 1), dtype=np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        check(_lib.lib().edh_batch_job_states(self._h, len(ids), ip(ids), ip(st)))
+        return st[:len(ids)]
+
     def eig_result(self, jid, vectors=False):
         info = self.job_info(jid)
         nev, n = info["count"], info["n"]

@@ -802,6 +802,14 @@ int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t*
   API_END
 }
 
+int edh_batch_job_states(edh_batch* b, int count, const int* job_ids, int* states) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(count >= 0 && (count == 0 || (job_ids && states)), EDH_ERR_ARG, "bad argument");
+  for (int q = 0; q < count; ++q) states[q] = B.job(job_ids[q]).state;
+  API_END
+}
+
 int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs, int* nconv) {
   API_BEGIN
   edh::Batch& B = use_batch(b);

@@ -670,10 +670,27 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   };
   place(pre, pl_pre);
   const size_t hv_off = off;
-  int64_t hv_rows = 0, hv_maxd = 0;
+  int64_t hv_maxd = 0;
   const size_t jb = hv_batch_job_bytes();
   const bool has_hv = hv && !hv->empty();
-  if (has_hv) off = align16(off + hv->size() * jb);
+  std::vector<HvBatchUnit> units;
+  if (has_hv) {
+    // rows per unit: one while the rows of the sweep are few (parallelism),
+    // up to ~4K elements per unit when there are many (less scheduling)
+    int64_t total_rows = 0;
+    for (size_t q = 0; q < hv->size(); ++q) total_rows += sector((*hv)[q].first).dev.up.D;
+    const int64_t spread = std::max<int64_t>(1, total_rows / 8192);
+    for (size_t q = 0; q < hv->size(); ++q) {
+      const DevSector& ds = sector((*hv)[q].first).dev;
+      const int ru = (int)std::min<int64_t>(hv_batch_unit_rows(ds.dn.D), spread);
+      for (int64_t r = 0; r < ds.up.D; r += ru)
+        units.push_back({(int)q, (int)r, (int)std::min<int64_t>(ru, ds.up.D - r), 0});
+      hv_maxd = std::max<int64_t>(hv_maxd, ds.dn.D);
+    }
+    off = align16(off + hv->size() * jb);
+  }
+  const size_t units_off = off;
+  off = align16(off + units.size() * sizeof(HvBatchUnit));
   place(post, pl_post);
   if (off == 0) return;
   // pinned staging, double-buffered: a buffer is rewritten only after the
@@ -725,12 +742,9 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   };
   fill(pre, pl_pre);
   if (has_hv) {
-    for (size_t q = 0; q < hv->size(); ++q) {
-      const DevSector& ds = sector((*hv)[q].first).dev;
-      hv_batch_fill(ds, (*hv)[q].second, hv_rows, hb + hv_off + q * jb);
-      hv_rows += ds.up.D;
-      hv_maxd = std::max<int64_t>(hv_maxd, ds.dn.D);
-    }
+    for (size_t q = 0; q < hv->size(); ++q)
+      hv_batch_fill(sector((*hv)[q].first).dev, (*hv)[q].second, 0, hb + hv_off + q * jb);
+    if (!units.empty()) std::memcpy(hb + units_off, units.data(), units.size() * sizeof(HvBatchUnit));
   }
   fill(post, pl_post);
   int max_ch = 1;
@@ -771,8 +785,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   };
   launch(pl_pre);
   if (has_hv)
-    launch_hv_batch(db + hv_off, (int)hv->size(), hv_rows, hv_maxd, hv_block,
-                    (int)std::min<int64_t>(hv_rows, 1 << 20), stream);
+    launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_maxd, hv_block, stream);
   launch(pl_post);
   EDH_HIP(hipGetLastError());
 }

@@ -219,12 +219,17 @@ int hv_num_blocks(const DevSector& ds);
 int hv_autotune(DevSector& ds, hipStream_t s);
 
 // Grouped H*v over many operands (sectors of any models): one launch covers
-// the concatenated up-rows of all of them.  A job record is filled on the
-// host (hv_batch_fill, row0 = first global row of the operand) into a
-// buffer of hv_batch_job_bytes() per job, then copied to the device.
+// the up-rows of all of them, cut into work units.  A job record is filled
+// on the host (hv_batch_fill) into a buffer of hv_batch_job_bytes() per job
+// and copied to the device with the unit list.
+// Work unit of the grouped H*v: rows [row0, row0 + nrows) of job `job`.
+struct HvBatchUnit {
+  int job, row0, nrows, pad;
+};
 size_t hv_batch_job_bytes();
 void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out);
-void launch_hv_batch(const void* djobs, int njobs, int64_t total_rows, int64_t max_ddn, int block, int grid,
+int hv_batch_unit_rows(int64_t ddn);  // rows per unit for a down dimension
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int64_t max_ddn, int block,
                      hipStream_t s);
 
 }  // namespace edh

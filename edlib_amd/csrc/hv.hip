@@ -455,12 +455,13 @@ __global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* cou
 
 
 // ---------------------------------------------------------------------------
-// Grouped H*v: every operand of a batch in one launch.  Work-group slot g is
-// global up-row g of the concatenation of all operands; the operand is found
-// by binary search over the jobs' first rows.  A row is staged in LDS when
-// the largest down dimension of the batch fits, and each thread takes
-// columns tid, tid + T, ...; the down-spin hopping reads the 32-bit sliced
-// ELL (present for every sector), so any model and value set is handled.
+// Grouped H*v: every operand of a batch in one launch.  The up-rows of all
+// operands are cut into work units (operand, first row, row count; about
+// 4K elements each) listed on the host, and work-group b takes units b,
+// b + grid, ...  A row is staged in LDS when the largest down dimension of
+// the batch fits, and each thread takes columns tid, tid + T, ...; the
+// down-spin hopping reads the 32-bit sliced ELL (present for every sector),
+// so any model and value set is handled.
 struct HvBatchJob {
   HvParams p;
   int64_t row0;
@@ -468,71 +469,67 @@ struct HvBatchJob {
 };
 
 template <bool LDS>
-__global__ __launch_bounds__(512) void hv_batch_kernel(const HvBatchJob* __restrict__ jobs, int njobs,
-                                                        int64_t total_rows) {
+__global__ __launch_bounds__(512) void hv_batch_kernel(const HvBatchJob* __restrict__ jobs,
+                                                        const HvBatchUnit* __restrict__ units, int nunits) {
   extern __shared__ double s_v[];
   const int tid = threadIdx.x;
   const int T = blockDim.x;
-  for (int64_t g = blockIdx.x; g < total_rows; g += gridDim.x) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].row0 <= g) lo = mid;
-      else hi = mid - 1;
-    }
-    const HvBatchJob& J = jobs[lo];
+  for (int ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
+    const HvBatchUnit U = units[ui];
+    const HvBatchJob& J = jobs[U.job];
     const HvParams& p = J.p;
     if (p.done && *p.done) continue;  // a stopped Lanczos run (uniform over the block)
-    const int u = (int)(g - J.row0);
     const int Ddn = p.Ddn;
     const double vscale = p.dscale ? p.dscale[0] : p.vscale;
     const double wscale = p.dscale ? p.dscale[1] : p.wscale;
     const double* __restrict__ V = p.v;
-    const size_t rowoff = (size_t)u * Ddn;
-    if (LDS) {
-      for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
-      __syncthreads();
-    }
-    const int bu = p.up_bits[u];
-    const double eu = p.up_e[u];
-    const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
     const int na = 1 << (p.Ns - p.nb);
     const unsigned lowmask = (1u << p.nb) - 1u;
     const int cross = J.cross;
-    for (int d = tid; d < Ddn; d += T) {
-      const int bd = p.dn_bits[d];
-      const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
-      double x;
-      if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
-      else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
-      else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
-      else x = 0.0;
-      double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
-      acc = fma(eu + p.dn_e[d] + x, vd, acc);
-      const int sl = d >> 6;
-      const int len = p.dn_sllen[sl];
-      const int base = p.dn_slptr[sl] + (d & 63);
-      for (int k = 0; k < len; ++k) {
-        const uint32_t e = p.dn_ell[base + k * kSlice];
-        const int c = (int)(e & kColMask);
-        acc = fma(p.values[e >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc);
+    for (int u = U.row0; u < U.row0 + U.nrows; ++u) {
+      const size_t rowoff = (size_t)u * Ddn;
+      if (LDS) {
+        for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
+        __syncthreads();
       }
-      for (int q = hb; q < he; ++q) {
-        const uint32_t e = p.up_packed[q];
-        acc = fma(p.values[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
-      }
-      if (J.kan) {
-        const int a = bu >> p.nb, b = bd >> p.nb;
-        const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-        for (int q = kb; q < ke; ++q) {
-          const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
-          const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
-          acc = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
+      const int bu = p.up_bits[u];
+      const double eu = p.up_e[u];
+      const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+      for (int d = tid; d < Ddn; d += T) {
+        const int bd = p.dn_bits[d];
+        const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
+        double x;
+        if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
+        else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
+        else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
+        else x = 0.0;
+        double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
+        acc = fma(eu + p.dn_e[d] + x, vd, acc);
+        const int sl = d >> 6;
+        const int len = p.dn_sllen[sl];
+        const int base = p.dn_slptr[sl] + (d & 63);
+        for (int k = 0; k < len; ++k) {
+          const uint32_t e = p.dn_ell[base + k * kSlice];
+          const int c = (int)(e & kColMask);
+          acc = fma(p.values[e >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc);
         }
+        for (int q = hb; q < he; ++q) {
+          const uint32_t e = p.up_packed[q];
+          acc = fma(p.values[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
+        }
+        if (J.kan) {
+          const int a = bu >> p.nb, b = bd >> p.nb;
+          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+          for (int q = kb; q < ke; ++q) {
+            const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
+            const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
+            acc = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
+          }
+        }
+        p.w[rowoff + d] = acc;
       }
-      p.w[rowoff + d] = acc;
+      if (LDS) __syncthreads();
     }
-    if (LDS) __syncthreads();
   }
 }
 
@@ -767,19 +764,22 @@ void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out
   std::memcpy(out, &j, sizeof(j));
 }
 
-void launch_hv_batch(const void* djobs, int njobs, int64_t total_rows, int64_t max_ddn, int block, int grid,
+int hv_batch_unit_rows(int64_t ddn) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, 4096 / std::max<int64_t>(ddn, 1))); }
+
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int64_t max_ddn, int block,
                      hipStream_t s) {
-  if (njobs <= 0 || total_rows <= 0) return;
+  if (nunits <= 0) return;
   EDH_CHECK(block >= 64 && block <= 512 && block % 64 == 0, EDH_ERR_ARG, "grouped H*v: bad block size");
   const size_t smem = (size_t)max_ddn * sizeof(double);
   const HvBatchJob* jobs = (const HvBatchJob*)djobs;
+  const int grid = std::min(nunits, 1 << 20);
   if (smem <= lds_budget()) {
     if (smem > 64 * 1024)
       EDH_HIP(hipFuncSetAttribute((const void*)hv_batch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)smem));
-    hipLaunchKernelGGL(hv_batch_kernel<true>, dim3(grid), dim3(block), smem, s, jobs, njobs, total_rows);
+    hipLaunchKernelGGL(hv_batch_kernel<true>, dim3(grid), dim3(block), smem, s, jobs, dunits, nunits);
   } else {
-    hipLaunchKernelGGL(hv_batch_kernel<false>, dim3(grid), dim3(block), 0, s, jobs, njobs, total_rows);
+    hipLaunchKernelGGL(hv_batch_kernel<false>, dim3(grid), dim3(block), 0, s, jobs, dunits, nunits);
   }
   EDH_HIP(hipGetLastError());
 }

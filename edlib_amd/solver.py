@@ -183,8 +183,10 @@ class BatchedSolve:
 
     def _collect(self):
         new = False
-        for idx, (sec, j) in enumerate(self.eig_jobs):
-            if j in self.seen or self.b.job_info(j)["state"] == 0:
+        pend = [(idx, sec, j) for idx, (sec, j) in enumerate(self.eig_jobs) if j not in self.seen]
+        states = self.b.job_states([j for _, _, j in pend]) if pend else []
+        for (idx, sec, j), st in zip(pend, states):
+            if st == 0:
                 continue
             self.seen.add(j)
             ev, _ = self.b.eig_result(j)
@@ -238,7 +240,7 @@ class BatchedSolve:
                 self.phase = "gf"
         if self.phase == "gf":
             live = [self.gf[key][0] for key, _ in self.used if self.gf[key][0] >= 0]
-            if all(self.b.job_info(j)["state"] != 0 for j in live):
+            if not live or not np.any(self.b.job_states(live) == 0):
                 wn = matsubara(p.beta, p.nomega)
                 norb = len(p.orbitals)
                 Gg = np.zeros((p.nomega, norb, 2), dtype=complex)

@@ -214,6 +214,8 @@ int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
 /* state EDH_JOB_*, kind EDH_JOB_EIG/LANCZOS, dimension, count = nev or
  * Lanczos steps, n_av = H*v applications. */
 int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t* n, int* count, int* n_av);
+/* states[q] = EDH_JOB_* of job_ids[q], q < count. */
+int edh_batch_job_states(edh_batch* b, int count, const int* job_ids, int* states);
 /* evals (count of edh_batch_job_info, ascending); evecs (host, n*count
  * column-major) may be NULL.  After maxitr restarts only the converged pairs
  * are returned (nconv < the requested nev), as Storage::diag does. */
