@@ -288,3 +288,26 @@ def test_lanczos_graph_replay_matches_eager():
         assert got[0] == ref[0]
         assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2]), rep
     ctx.close()
+
+
+def test_av_parity_16site_sector():
+    """The 16-site 4x4 Hubbard sector (8,8) (N = 165,636,900) against the
+    oracle's SpinResolvedStorage::av restatement, both H*v variants."""
+    Ns = 16
+    t = models.torus_hopping(4, 4)
+    om = eo.hubbard(Ns, t, np.full(Ns, 6.0), np.full(Ns, 3.0))
+    srs = eo.SRS(om, 8, 8)
+    n = srs.n
+    rng = np.random.default_rng(20261202 + 8)
+    v = rng.uniform(-1.0, 1.0, n)
+    w0 = np.random.default_rng(20261202 + 9).uniform(-1.0, 1.0, n)
+    ref = srs.av(v, w0.copy(), clear=False, omp=True)
+    ctx = ea.Context(0)
+    ctx.set_model(ea.HubbardModel(Ns, t, np.full(Ns, 6.0), np.full(Ns, 3.0)))
+    assert ctx.set_sector(8, 8) == n
+    for var in (0, 1):
+        ctx.hv_variant = var
+        got = ctx.av(v, w0, clear=False)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < 1e-13, (var, err)
+    ctx.close()
