@@ -35,7 +35,7 @@ struct BJob {
   int nev = 0, m = 0, k = 0, col = 0, iter = 0, maxitr = 0, nconv = 0, n_av = 0;
   double tol = 0.0;
   bool pending_start_norm = false;
-  double *V = nullptr, *Valt = nullptr, *evecs = nullptr, *Cbuf = nullptr;
+  double *V = nullptr, *Valt = nullptr, *evecs = nullptr;
   std::vector<double> T, evals;
   // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w; step = next step
   int nl = 0, step = 0, nlanc = 0;
@@ -44,11 +44,10 @@ struct BJob {
   int* d_flags = nullptr;  // [0] stopped (|beta| < 1e-10), [1] steps taken
   std::vector<double> alpha, beta;
   // scalars read back at a restart-cycle end or at the end of a Lanczos run:
-  // a pinned buffer and the event after its copy (the process sits out the
-  // sweeps until the event has completed)
-  double* h_read = nullptr;
-  size_t h_read_count = 0;
-  hipEvent_t ev_read = nullptr;
+  // a place in one of the batch's pinned read slabs (the process sits out the
+  // sweeps until that slab's copy has completed)
+  const double* h_read = nullptr;
+  int slab = -1;
   bool awaiting = false;
 };
 
@@ -78,6 +77,34 @@ struct OpGroup {
   std::vector<BOpHost> ops;
 };
 
+// Krylov-basis combination of one process, queued by the host logic of a
+// restart-cycle end and launched with the next sweep: Y_j = sum_i C[j rows +
+// i] X_i (j < cols, i < rows), and `zero` cleared.
+struct CombineReq {
+  const double* X = nullptr;
+  int64_t n = 0;
+  int rows = 0, cols = 0;
+  std::vector<double> C;
+  double* Y = nullptr;
+  double* zero = nullptr;
+  int nzero = 0;
+};
+
+// Scalars of one process to be copied back with the sweep's grouped read.
+struct ReadReq {
+  BJob* j;
+  const double* src;
+  size_t count;
+};
+
+// Pinned host buffer receiving one sweep's grouped read.
+struct ReadSlab {
+  double* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  int refs = 0;
+};
+
 struct GfRequest {
   int eig_job, index, orb, annihilate, dst_sec;
 };
@@ -94,6 +121,19 @@ struct Batch {
   bool ev_used[2] = {false, false};
   int stage_i = 0;
   std::map<std::tuple<int, int, int>, std::vector<void*>> maps;  // (sector, orbital, annihilate) -> gather maps
+  DevBuf d_gather;
+  std::vector<ReadSlab> slabs;
+  std::vector<CombineReq> pending_comb;
+  // Krylov bases of finished eigensolver runs: freed once the event after
+  // the combination that last reads them has completed
+  std::vector<void*> pending_free;
+  size_t pending_free_bytes = 0;
+  struct Deferred {
+    hipEvent_t ev;
+    std::vector<void*> ptrs;
+    size_t bytes;
+  };
+  std::vector<Deferred> deferred;
   size_t bytes_in_use = 0;
   int hv_block = 256;
   long long sweeps = 0;
@@ -110,17 +150,22 @@ struct Batch {
   void submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms);
   void free_job(int id);
   int step();
+  // launches the queued basis combinations (Ritz vectors of runs that have
+  // just finished) if no sweep has done so yet
+  void flush();
   int autotune(const std::vector<int>& sec_ids);
 
  private:
   double* dalloc(BJob& j, size_t count);
   void release_job(BJob& j);
-  void read_back(BJob& j, const double* src, size_t count);
   void complete(BJob& j);
+  void release_read(BJob& j);
+  void reap_deferred(bool wait);
   void finish_dense(BJob& j);
   void end_cycle(BJob& j);
   void finish_lanczos(BJob& j);
-  void issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv, std::vector<OpGroup>& post);
+  void issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv, std::vector<OpGroup>& post,
+             const std::vector<ReadReq>* reads = nullptr);
   void fill_dev(double* p, size_t n);
   void gather_maps(int src, int orb, bool annihilate, std::vector<void*>& out);
 };

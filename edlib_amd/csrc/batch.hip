@@ -24,6 +24,7 @@
 // is therefore bit-identical alone and in any mix.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -299,6 +300,59 @@ __global__ void bfill_kernel(double* p, int64_t n, double x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = x;
 }
 
+// Grouped Krylov-basis combination (thick restarts and Ritz vectors of every
+// process whose cycle ended): per chunk, Y_j[e] = sum_i C[j rows + i] X_i[e]
+// with the X_i[e] in registers; the first chunk of a process also clears its
+// projected-matrix block.
+struct BComb {
+  const double* X;
+  double* Y;
+  const double* C;
+  double* zero;
+  int64_t n;
+  int rows, cols, nzero;
+  int chunk0, nch, chunk;
+};
+
+__global__ __launch_bounds__(kOpBlock) void bcombine_kernel(const BComb* __restrict__ ops,
+                                                            const int* __restrict__ cjob) {
+  __shared__ double s_c[kMaxBasis * kMaxBasis];
+  const int c = blockIdx.x;
+  const BComb& o = ops[cjob[c]];
+  const int lc = c - o.chunk0;
+  for (int i = threadIdx.x; i < o.rows * o.cols; i += kOpBlock) s_c[i] = o.C[i];
+  if (lc == 0 && o.zero)
+    for (int i = threadIdx.x; i < o.nzero; i += kOpBlock) o.zero[i] = 0.0;
+  __syncthreads();
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
+    double v[kMaxBasis];
+#pragma unroll
+    for (int i = 0; i < kMaxBasis; ++i) v[i] = i < o.rows ? o.X[(int64_t)i * o.n + e] : 0.0;
+    for (int jj = 0; jj < o.cols; ++jj) {
+      double y = 0.0;
+#pragma unroll
+      for (int i = 0; i < kMaxBasis; ++i)
+        if (i < o.rows) y = fma(s_c[jj * o.rows + i], v[i], y);
+      o.Y[(int64_t)jj * o.n + e] = y;
+    }
+  }
+}
+
+// Grouped read: the scalars of every process that reports this sweep,
+// packed into one device buffer (one block per process) for one copy.
+struct BGather {
+  const double* src;
+  int64_t off;
+  int count;
+};
+
+__global__ void bgather_kernel(const BGather* __restrict__ ops, double* __restrict__ dst) {
+  const BGather& o = ops[blockIdx.x];
+  for (int i = threadIdx.x; i < o.count; i += blockDim.x) dst[o.off + i] = o.src[i];
+}
+
 uint64_t splitmix64(uint64_t& x) {
   uint64_t z = (x += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -325,11 +379,17 @@ Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& j : jobs)
     if (j) release_job(*j);
+  reap_deferred(true);
+  for (void* p : pending_free) (void)hipFree(p);
+  for (ReadSlab& r : slabs) {
+    if (r.host) (void)hipHostFree(r.host);
+    if (r.ev) (void)hipEventDestroy(r.ev);
+  }
   for (auto& kv : maps)
     for (void* p : kv.second) (void)hipFree(p);
   for (auto& s : secs)
     if (s) s->dev.release();
-  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms}) b->release();
+  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms, &d_gather}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);
     if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
@@ -349,30 +409,35 @@ double* Batch::dalloc(BJob& j, size_t count) {
 }
 
 void Batch::release_job(BJob& j) {
+  release_read(j);
   for (void* p : j.allocs) (void)hipFree(p);
   j.allocs.clear();
   bytes_in_use -= j.bytes;
   j.bytes = 0;
-  if (j.h_read) (void)hipHostFree(j.h_read);
-  j.h_read = nullptr;
-  j.h_read_count = 0;
-  if (j.ev_read) (void)hipEventDestroy(j.ev_read);
-  j.ev_read = nullptr;
 }
 
-// Queues the copy of a process's scalars into its pinned buffer and marks
-// the process as waiting for them.
-void Batch::read_back(BJob& j, const double* src, size_t count) {
-  if (count > j.h_read_count) {
-    if (j.h_read) EDH_HIP(hipHostFree(j.h_read));
-    j.h_read = nullptr;
-    EDH_HIP(hipHostMalloc((void**)&j.h_read, count * sizeof(double)));
-    j.h_read_count = count;
+// Gives back a process's place in its read slab.
+void Batch::release_read(BJob& j) {
+  if (j.slab >= 0) --slabs[j.slab].refs;
+  j.slab = -1;
+  j.h_read = nullptr;
+}
+
+// Frees the bases whose last reader has completed (all of them when wait).
+void Batch::reap_deferred(bool wait) {
+  size_t keep = 0;
+  for (size_t q = 0; q < deferred.size(); ++q) {
+    Deferred& d = deferred[q];
+    if (wait) (void)hipEventSynchronize(d.ev);
+    if (wait || hipEventQuery(d.ev) == hipSuccess) {
+      for (void* p : d.ptrs) (void)hipFree(p);
+      bytes_in_use -= d.bytes;
+      (void)hipEventDestroy(d.ev);
+    } else {
+      deferred[keep++] = d;
+    }
   }
-  if (!j.ev_read) EDH_HIP(hipEventCreateWithFlags(&j.ev_read, hipEventDisableTiming));
-  EDH_HIP(hipMemcpyAsync(j.h_read, src, count * sizeof(double), hipMemcpyDeviceToHost, stream));
-  EDH_HIP(hipEventRecord(j.ev_read, stream));
-  j.awaiting = true;
+  deferred.resize(keep);
 }
 
 // Host part of a process once its scalars have arrived.
@@ -380,6 +445,7 @@ void Batch::complete(BJob& j) {
   j.awaiting = false;
   if (j.kind == JOB_EIG) end_cycle(j);
   else finish_lanczos(j);
+  release_read(j);
 }
 
 int Batch::add_sector(const Model& m, int nup, int ndn) {
@@ -629,9 +695,16 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
   issue(pre2, nullptr, post);
 }
 
+void Batch::flush() {
+  if (pending_comb.empty()) return;
+  std::vector<OpGroup> none;
+  issue(none, nullptr, none);
+}
+
 void Batch::free_job(int id) {
   BJob& j = job(id);
   EDH_CHECK(j.state != JOB_ACTIVE, EDH_ERR_STATE, "batch: job still active");
+  flush();
   if (!j.allocs.empty()) EDH_HIP(hipStreamSynchronize(stream));
   release_job(j);
   jobs[id].reset();
@@ -648,13 +721,26 @@ void Batch::fill_dev(double* p, size_t n) {
 // grouped H*v, groups after it) into one pinned staging buffer, copies it to
 // the device in one transfer and launches the sequence.
 void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv,
-                  std::vector<OpGroup>& post) {
+                  std::vector<OpGroup>& post, const std::vector<ReadReq>* reads) {
   struct Placed {
     int kind, nops, nch;
     size_t ops_off, cj_off;
   };
   std::vector<Placed> pl_pre, pl_post;
   size_t off = 0;
+  // queued basis combinations: descriptors, chunk map, coefficient matrices
+  const int ncomb = (int)pending_comb.size();
+  int comb_nch = 0;
+  for (const CombineReq& r : pending_comb) comb_nch += nchunks(r.n);
+  const size_t comb_ops_off = off;
+  off = align16(off + (size_t)ncomb * sizeof(BComb));
+  const size_t comb_cj_off = off;
+  off = align16(off + (size_t)comb_nch * sizeof(int));
+  std::vector<size_t> comb_c_off(ncomb);
+  for (int q = 0; q < ncomb; ++q) {
+    comb_c_off[q] = off;
+    off = align16(off + pending_comb[q].C.size() * sizeof(double));
+  }
   auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
     for (OpGroup& g : gs) {
       if (g.ops.empty()) continue;
@@ -670,7 +756,6 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   };
   place(pre, pl_pre);
   const size_t hv_off = off;
-  int64_t hv_maxd = 0;
   const size_t jb = hv_batch_job_bytes();
   const bool has_hv = hv && !hv->empty();
   std::vector<HvBatchUnit> units;
@@ -685,13 +770,18 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const int ru = (int)std::min<int64_t>(hv_batch_unit_rows(ds.dn.D), spread);
       for (int64_t r = 0; r < ds.up.D; r += ru)
         units.push_back({(int)q, (int)r, (int)std::min<int64_t>(ru, ds.up.D - r), 0});
-      hv_maxd = std::max<int64_t>(hv_maxd, ds.dn.D);
     }
     off = align16(off + hv->size() * jb);
   }
   const size_t units_off = off;
   off = align16(off + units.size() * sizeof(HvBatchUnit));
   place(post, pl_post);
+  // grouped read of the processes reporting this sweep
+  const int nread = reads ? (int)reads->size() : 0;
+  const size_t gat_off = off;
+  off = align16(off + (size_t)nread * sizeof(BGather));
+  size_t read_total = 0;
+  for (int q = 0; q < nread; ++q) read_total += (*reads)[q].count;
   if (off == 0) return;
   // pinned staging, double-buffered: a buffer is rewritten only after the
   // copy that last read it has completed
@@ -747,11 +837,45 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     if (!units.empty()) std::memcpy(hb + units_off, units.data(), units.size() * sizeof(HvBatchUnit));
   }
   fill(post, pl_post);
+  {
+    int c0 = 0;
+    int* cj = (int*)(hb + comb_cj_off);
+    for (int q = 0; q < ncomb; ++q) {
+      const CombineReq& r = pending_comb[q];
+      BComb b;
+      b.X = r.X;
+      b.Y = r.Y;
+      b.C = nullptr;  // the arena address, set once the arena is sized
+      b.zero = r.zero;
+      b.n = r.n;
+      b.rows = r.rows;
+      b.cols = r.cols;
+      b.nzero = r.nzero;
+      b.chunk0 = c0;
+      b.nch = nchunks(r.n);
+      b.chunk = chunk_of(r.n);
+      std::memcpy(hb + comb_ops_off + q * sizeof(BComb), &b, sizeof(BComb));
+      std::memcpy(hb + comb_c_off[q], r.C.data(), r.C.size() * sizeof(double));
+      for (int c = 0; c < b.nch; ++c) cj[c0 + c] = q;
+      c0 += b.nch;
+    }
+    int64_t roff = 0;
+    for (int q = 0; q < nread; ++q) {
+      BGather g{(*reads)[q].src, roff, (int)(*reads)[q].count};
+      std::memcpy(hb + gat_off + q * sizeof(BGather), &g, sizeof(BGather));
+      roff += (int64_t)(*reads)[q].count;
+    }
+  }
   int max_ch = 1;
   for (auto* v : {&pl_pre, &pl_post})
     for (const Placed& p : *v) max_ch = std::max(max_ch, p.nch);
   d_arena.ensure(off);
   d_partial.ensure((size_t)max_ch * kMaxBasis * sizeof(double));
+  // the coefficient pointers of the combinations are arena addresses
+  for (int q = 0; q < ncomb; ++q) {
+    const double* cdev = (const double*)(d_arena.as<char>() + comb_c_off[q]);
+    std::memcpy(hb + comb_ops_off + q * sizeof(BComb) + offsetof(BComb, C), &cdev, sizeof(cdev));
+  }
   EDH_HIP(hipMemcpyAsync(d_arena.p, hb, off, hipMemcpyHostToDevice, stream));
   EDH_HIP(hipEventRecord(ev_stage[si], stream));
   ev_used[si] = true;
@@ -783,15 +907,61 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
     }
   };
+  if (ncomb) {
+    hipLaunchKernelGGL(bcombine_kernel, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
+                       (const int*)(db + comb_cj_off));
+    pending_comb.clear();
+    if (!pending_free.empty()) {
+      Deferred d;
+      EDH_HIP(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming));
+      EDH_HIP(hipEventRecord(d.ev, stream));
+      d.ptrs.swap(pending_free);
+      d.bytes = pending_free_bytes;
+      pending_free_bytes = 0;
+      deferred.push_back(std::move(d));
+    }
+  }
   launch(pl_pre);
   if (has_hv)
-    launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_maxd, hv_block, stream);
+    launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, stream);
   launch(pl_post);
+  if (nread) {
+    d_gather.ensure(read_total * sizeof(double));
+    hipLaunchKernelGGL(bgather_kernel, dim3(nread), dim3(256), 0, stream, (const BGather*)(db + gat_off),
+                       d_gather.as<double>());
+    // a free slab (no reader left) large enough, else a new one
+    int si2 = -1;
+    for (size_t q = 0; q < slabs.size(); ++q)
+      if (slabs[q].refs == 0 && slabs[q].cap >= read_total) {
+        si2 = (int)q;
+        break;
+      }
+    if (si2 < 0) {
+      ReadSlab r;
+      r.cap = std::max<size_t>(read_total, 4096);
+      EDH_HIP(hipHostMalloc((void**)&r.host, r.cap * sizeof(double)));
+      EDH_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+      slabs.push_back(r);
+      si2 = (int)slabs.size() - 1;
+    }
+    ReadSlab& r = slabs[si2];
+    EDH_HIP(hipMemcpyAsync(r.host, d_gather.p, read_total * sizeof(double), hipMemcpyDeviceToHost, stream));
+    EDH_HIP(hipEventRecord(r.ev, stream));
+    size_t roff = 0;
+    for (const ReadReq& q : *reads) {
+      q.j->h_read = r.host + roff;
+      q.j->slab = si2;
+      q.j->awaiting = true;
+      ++r.refs;
+      roff += q.count;
+    }
+  }
   EDH_HIP(hipGetLastError());
 }
 
 // One sweep: every active process advances one Krylov step.
 int Batch::step() {
+  reap_deferred(false);
   // processes whose scalars have arrived go on; when every active process is
   // waiting, wait for the oldest copies
   {
@@ -803,12 +973,12 @@ int Batch::step() {
         any_running = true;
         continue;
       }
-      if (hipEventQuery(jp->ev_read) == hipSuccess) complete(*jp);
+      if (hipEventQuery(slabs[jp->slab].ev) == hipSuccess) complete(*jp);
       else wait.push_back(jp.get());
     }
     if (!any_running)
       for (BJob* j : wait) {
-        EDH_HIP(hipEventSynchronize(j->ev_read));
+        EDH_HIP(hipEventSynchronize(slabs[j->slab].ev));
         complete(*j);
       }
   }
@@ -823,6 +993,7 @@ int Batch::step() {
     }
   }
   if (eig.empty() && lz.empty()) {
+    flush();  // Ritz vectors of runs that just finished
     int active = 0;
     for (auto& jp : jobs)
       if (jp && jp->state == JOB_ACTIVE) ++active;
@@ -929,19 +1100,21 @@ int Batch::step() {
   post.push_back(rf);
   post.push_back(p2);
   post.push_back(sc);
-  issue(pre, &hv, post);
+  // (5) restart-cycle ends and finished Lanczos runs: their scalars come
+  // back in the sweep's grouped read, and they sit out until it has arrived
+  std::vector<ReadReq> reads;
+  for (BJob* j : eig)
+    if (j->col + 1 >= j->m) reads.push_back({j, j->sc + S_NRM, (size_t)(S_T - S_NRM) + (size_t)j->m * (j->m + 1)});
+  for (BJob* j : lz)
+    if (j->step == j->nl) reads.push_back({j, j->d_alpha, 2 * (size_t)j->nl + 2});
+  issue(pre, &hv, post, &reads);
   ++sweeps;
-  // (5) restart-cycle ends and finished Lanczos runs queue the copy of
-  // their scalars and sit out until it has arrived
   for (BJob* j : eig) {
     ++j->n_av;
     if (j->col + 1 < j->m) ++j->col;
-    else read_back(*j, j->sc + S_NRM, (size_t)(S_T - S_NRM) + (size_t)j->m * (j->m + 1));
   }
-  for (BJob* j : lz) {
-    if (j->step == j->nl) read_back(*j, j->d_alpha, 2 * (size_t)j->nl + 2);
-    else ++j->step;
-  }
+  for (BJob* j : lz)
+    if (j->step != j->nl) ++j->step;
   int active = 0;
   for (auto& jp : jobs)
     if (jp && jp->state == JOB_ACTIVE) ++active;
@@ -989,45 +1162,57 @@ void Batch::end_cycle(BJob& j) {
   if (conv || j.iter >= j.maxitr) {
     // after maxitr the converged Ritz pairs are the result (as dseupd with
     // info = 1, Storage.h:94-129)
+    // the Ritz vectors are combined with the next sweep (stream order puts
+    // every reader of them after it); the bases are freed after that
     const int nk = nconv_now;
-    std::vector<double> Cm((size_t)m * std::max(nk, 1));
-    for (int c = 0; c < nk; ++c)
-      for (int i = 0; i < m; ++i) Cm[(size_t)c * m + i] = A[(size_t)i * m + done_i[c]];
     j.evals.clear();
     for (int c = 0; c < nk; ++c) j.evals.push_back(theta[done_i[c]]);
     if (nk > 0) {
-      double* Cd = dalloc(j, Cm.size());
-      EDH_HIP(hipMemcpyAsync(Cd, Cm.data(), Cm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+      CombineReq r;
+      r.X = j.V;
+      r.n = n;
+      r.rows = m;
+      r.cols = nk;
+      r.C.assign((size_t)m * nk, 0.0);
+      for (int c = 0; c < nk; ++c)
+        for (int i = 0; i < m; ++i) r.C[(size_t)c * m + i] = A[(size_t)i * m + done_i[c]];
       j.evecs = dalloc(j, (size_t)nk * n);
-      dev_combine(j.V, n, m, Cd, nk, j.evecs, n, n, stream);
+      r.Y = j.evecs;
+      pending_comb.push_back(std::move(r));
     }
-    EDH_HIP(hipStreamSynchronize(stream));
     j.nev = nk;
     j.nconv = nk;
     j.state = JOB_DONE;
-    // the Krylov bases are no longer needed
     for (double* b : {j.V, j.Valt}) {
       auto it = std::find(j.allocs.begin(), j.allocs.end(), (void*)b);
       if (it != j.allocs.end()) {
-        (void)hipFree(*it);
-        j.allocs.erase(it);
         const size_t bb = (size_t)(m + 1) * n * sizeof(double);
+        j.allocs.erase(it);
         j.bytes -= bb;
-        bytes_in_use -= bb;
+        pending_free.push_back(b);
+        pending_free_bytes += bb;
       }
     }
     j.V = j.Valt = nullptr;
     return;
   }
+  // thick restart: Valt_c = sum_i A[i][c] V_i (c < kk) and Valt_kk = V_m (the
+  // residual direction), one combination with m + 1 rows and kk + 1 columns,
+  // which also clears the projected-matrix block
   const int kk = restart_size(nev, m, nconv_now);
-  std::vector<double> Cm((size_t)m * kk);
+  CombineReq r;
+  r.X = j.V;
+  r.n = n;
+  r.rows = m + 1;
+  r.cols = kk + 1;
+  r.C.assign((size_t)(m + 1) * (kk + 1), 0.0);
   for (int c = 0; c < kk; ++c)
-    for (int i = 0; i < m; ++i) Cm[(size_t)c * m + i] = A[(size_t)i * m + c];
-  if (!j.Cbuf) j.Cbuf = dalloc(j, (size_t)kMaxBasis * kMaxBasis);
-  EDH_HIP(hipMemcpyAsync(j.Cbuf, Cm.data(), Cm.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-  dev_combine(j.V, n, m, j.Cbuf, kk, j.Valt, n, n, stream);
-  EDH_HIP(hipMemcpyAsync(j.Valt + (size_t)kk * n, j.V + (size_t)m * n, (size_t)n * sizeof(double),
-                         hipMemcpyDeviceToDevice, stream));
+    for (int i = 0; i < m; ++i) r.C[(size_t)c * (m + 1) + i] = A[(size_t)i * m + c];
+  r.C[(size_t)kk * (m + 1) + m] = 1.0;
+  r.Y = j.Valt;
+  r.zero = j.sc + S_T;
+  r.nzero = (m + 1) * (m + 1);
+  pending_comb.push_back(std::move(r));
   std::swap(j.V, j.Valt);
   std::fill(j.T.begin(), j.T.end(), 0.0);
   for (int i = 0; i < kk; ++i) {
@@ -1036,7 +1221,6 @@ void Batch::end_cycle(BJob& j) {
     j.T[(size_t)i * m + kk] = si;
     j.T[(size_t)kk * m + i] = si;
   }
-  fill_dev(j.sc + S_T, (size_t)(m + 1) * (m + 1));
   j.k = kk;
   j.col = kk;
 }

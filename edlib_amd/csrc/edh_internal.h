@@ -229,7 +229,6 @@ struct HvBatchUnit {
 size_t hv_batch_job_bytes();
 void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out);
 int hv_batch_unit_rows(int64_t ddn);  // rows per unit for a down dimension
-void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int64_t max_ddn, int block,
-                     hipStream_t s);
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s);
 
 }  // namespace edh

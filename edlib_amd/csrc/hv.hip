@@ -458,77 +458,103 @@ __global__ __launch_bounds__(kSlabWG) void hv_upslab_kernel(HvParams p, int* cou
 // Grouped H*v: every operand of a batch in one launch.  The up-rows of all
 // operands are cut into work units (operand, first row, row count; about
 // 4K elements each) listed on the host, and work-group b takes units b,
-// b + grid, ...  A row is staged in LDS when the largest down dimension of
-// the batch fits, and each thread takes columns tid, tid + T, ...; the
-// down-spin hopping reads the 32-bit sliced ELL (present for every sector),
-// so any model and value set is handled.
+// b + grid, ...  The down-spin hopping reads the 32-bit sliced ELL (present
+// for every sector), so any model and value set is handled.
 struct HvBatchJob {
   HvParams p;
   int64_t row0;
   int cross, kan;
 };
 
-template <bool LDS>
-__global__ __launch_bounds__(512) void hv_batch_kernel(const HvBatchJob* __restrict__ jobs,
-                                                        const HvBatchUnit* __restrict__ units, int nunits) {
-  extern __shared__ double s_v[];
-  const int tid = threadIdx.x;
-  const int T = blockDim.x;
+// Wavefront-task form of the grouped H*v: no LDS and no block barrier.  A
+// unit's rows are cut into wavefront tasks: 64-column slices of one row when
+// the down dimension is 64 or more (a slice is exactly one ELL slice, so the
+// slot count is wavefront-uniform), else R = 64 / Dp whole rows per
+// wavefront (Dp = the down dimension rounded up to a power of two).  The
+// down-spin gathers read the row through the L1/L2 caches, and the loads of
+// four slots are issued before their FMAs.
+__global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __restrict__ jobs,
+                                                            const HvBatchUnit* __restrict__ units, int nunits) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
   for (int ui = blockIdx.x; ui < nunits; ui += gridDim.x) {
     const HvBatchUnit U = units[ui];
     const HvBatchJob& J = jobs[U.job];
     const HvParams& p = J.p;
     if (p.done && *p.done) continue;  // a stopped Lanczos run (uniform over the block)
     const int Ddn = p.Ddn;
+    const int lg = Ddn >= 64 ? 6 : (Ddn <= 1 ? 0 : 32 - __clz(Ddn - 1));  // log2 Dp
+    const int nsl = (Ddn + 63) >> 6;
+    const int ngr = (U.nrows + (64 >> lg) - 1) >> (6 - lg);
+    const int ntask = ngr * nsl;
     const double vscale = p.dscale ? p.dscale[0] : p.vscale;
     const double wscale = p.dscale ? p.dscale[1] : p.wscale;
     const double* __restrict__ V = p.v;
+    const double* __restrict__ vals = p.values;
     const int na = 1 << (p.Ns - p.nb);
     const unsigned lowmask = (1u << p.nb) - 1u;
     const int cross = J.cross;
-    for (int u = U.row0; u < U.row0 + U.nrows; ++u) {
+    for (int t = wave; t < ntask; t += nw) {
+      const int g = t / nsl, sl = t - g * nsl;
+      const int u = U.row0 + (g << (6 - lg)) + (lane >> lg);
+      const int d = (sl << 6) + (lane & ((1 << lg) - 1));
+      if (u >= U.row0 + U.nrows || d >= Ddn) continue;
       const size_t rowoff = (size_t)u * Ddn;
-      if (LDS) {
-        for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
-        __syncthreads();
-      }
       const int bu = p.up_bits[u];
-      const double eu = p.up_e[u];
-      const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
-      for (int d = tid; d < Ddn; d += T) {
-        const int bd = p.dn_bits[d];
-        const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
-        double x;
-        if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
-        else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
-        else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
-        else x = 0.0;
-        double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
-        acc = fma(eu + p.dn_e[d] + x, vd, acc);
-        const int sl = d >> 6;
-        const int len = p.dn_sllen[sl];
-        const int base = p.dn_slptr[sl] + (d & 63);
-        for (int k = 0; k < len; ++k) {
-          const uint32_t e = p.dn_ell[base + k * kSlice];
-          const int c = (int)(e & kColMask);
-          acc = fma(p.values[e >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc);
-        }
-        for (int q = hb; q < he; ++q) {
-          const uint32_t e = p.up_packed[q];
-          acc = fma(p.values[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
-        }
-        if (J.kan) {
-          const int a = bu >> p.nb, b = bd >> p.nb;
-          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-          for (int q = kb; q < ke; ++q) {
-            const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
-            const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
-            acc = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
-          }
-        }
-        p.w[rowoff + d] = acc;
+      const int bd = p.dn_bits[d];
+      double x;
+      if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
+      else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
+      else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
+      else x = 0.0;
+      double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
+      acc = fma(p.up_e[u] + p.dn_e[d] + x, V[rowoff + d] * vscale, acc);
+      // down-spin hopping (sliced ELL, slice = d / 64)
+      const int len = p.dn_sllen[sl];
+      const uint32_t* __restrict__ el = p.dn_ell + p.dn_slptr[sl] + (d & 63);
+      int k = 0;
+      for (; k + 4 <= len; k += 4) {
+        const uint32_t e0 = el[k * kSlice], e1 = el[(k + 1) * kSlice];
+        const uint32_t e2 = el[(k + 2) * kSlice], e3 = el[(k + 3) * kSlice];
+        const double x0 = V[rowoff + (e0 & kColMask)], x1 = V[rowoff + (e1 & kColMask)];
+        const double x2 = V[rowoff + (e2 & kColMask)], x3 = V[rowoff + (e3 & kColMask)];
+        acc = fma(vals[e0 >> kColBits], x0 * vscale, acc);
+        acc = fma(vals[e1 >> kColBits], x1 * vscale, acc);
+        acc = fma(vals[e2 >> kColBits], x2 * vscale, acc);
+        acc = fma(vals[e3 >> kColBits], x3 * vscale, acc);
       }
-      if (LDS) __syncthreads();
+      for (; k < len; ++k) {
+        const uint32_t e = el[k * kSlice];
+        acc = fma(vals[e >> kColBits], V[rowoff + (e & kColMask)] * vscale, acc);
+      }
+      // up-spin hopping: rows of the same columns (coalesced)
+      const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+      int q = hb;
+      for (; q + 4 <= he; q += 4) {
+        const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
+        const uint32_t e2 = p.up_packed[q + 2], e3 = p.up_packed[q + 3];
+        const double x0 = V[(size_t)(e0 & kColMask) * Ddn + d], x1 = V[(size_t)(e1 & kColMask) * Ddn + d];
+        const double x2 = V[(size_t)(e2 & kColMask) * Ddn + d], x3 = V[(size_t)(e3 & kColMask) * Ddn + d];
+        acc = fma(vals[e0 >> kColBits] * vscale, x0, acc);
+        acc = fma(vals[e1 >> kColBits] * vscale, x1, acc);
+        acc = fma(vals[e2 >> kColBits] * vscale, x2, acc);
+        acc = fma(vals[e3 >> kColBits] * vscale, x3, acc);
+      }
+      for (; q < he; ++q) {
+        const uint32_t e = p.up_packed[q];
+        acc = fma(vals[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
+      }
+      if (J.kan) {
+        const int a = bu >> p.nb, b = bd >> p.nb;
+        const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+        for (int r = kb; r < ke; ++r) {
+          const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[r] << p.nb)];
+          const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[r] << p.nb)];
+          acc = fma(p.kan_c[r], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
+        }
+      }
+      p.w[rowoff + d] = acc;
     }
   }
 }
@@ -766,21 +792,11 @@ void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out
 
 int hv_batch_unit_rows(int64_t ddn) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, 4096 / std::max<int64_t>(ddn, 1))); }
 
-void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int64_t max_ddn, int block,
-                     hipStream_t s) {
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s) {
   if (nunits <= 0) return;
   EDH_CHECK(block >= 64 && block <= 512 && block % 64 == 0, EDH_ERR_ARG, "grouped H*v: bad block size");
-  const size_t smem = (size_t)max_ddn * sizeof(double);
-  const HvBatchJob* jobs = (const HvBatchJob*)djobs;
   const int grid = std::min(nunits, 1 << 20);
-  if (smem <= lds_budget()) {
-    if (smem > 64 * 1024)
-      EDH_HIP(hipFuncSetAttribute((const void*)hv_batch_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)smem));
-    hipLaunchKernelGGL(hv_batch_kernel<true>, dim3(grid), dim3(block), smem, s, jobs, dunits, nunits);
-  } else {
-    hipLaunchKernelGGL(hv_batch_kernel<false>, dim3(grid), dim3(block), 0, s, jobs, dunits, nunits);
-  }
+  hipLaunchKernelGGL(hv_batch_wave_kernel, dim3(grid), dim3(block), 0, s, (const HvBatchJob*)djobs, dunits, nunits);
   EDH_HIP(hipGetLastError());
 }
 
