@@ -819,7 +819,6 @@ int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs,
   if (evals)
     for (int q = 0; q < (int)j.evals.size(); ++q) evals[q] = j.evals[q];
   if (evecs && j.evecs && j.nev > 0) {
-    B.flush();
     EDH_HIP(hipMemcpyAsync(evecs, j.evecs, (size_t)j.nev * j.n * sizeof(double), hipMemcpyDeviceToHost, B.stream));
     EDH_HIP(hipStreamSynchronize(B.stream));
   }

@@ -31,23 +31,24 @@ struct BJob {
   std::vector<void*> allocs;
   size_t bytes = 0;
   double* sc = nullptr;
-  // eigensolver run
-  int nev = 0, m = 0, k = 0, col = 0, iter = 0, maxitr = 0, nconv = 0, n_av = 0;
+  // eigensolver run (its column, restarts and convergence are kept on the
+  // device in the control block ctl; the host sees them through the status
+  // read of every sweep)
+  int nev = 0, m = 0, iter = 0, maxitr = 0, nconv = 0, n_av = 0;
   double tol = 0.0;
   bool pending_start_norm = false;
-  double *V = nullptr, *Valt = nullptr, *evecs = nullptr;
-  std::vector<double> T, evals;
+  double *V = nullptr, *Valt = nullptr, *evecs = nullptr, *ctl = nullptr;
+  std::vector<double> evals;
   // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w; step = next step
   int nl = 0, step = 0, nlanc = 0;
   double* lz[3] = {nullptr, nullptr, nullptr};
-  double *d_alpha = nullptr, *d_beta = nullptr;
+  double *d_alpha = nullptr, *d_beta = nullptr, *tdot = nullptr;
+  int64_t ntdot = 0;
   int* d_flags = nullptr;  // [0] stopped (|beta| < 1e-10), [1] steps taken
   std::vector<double> alpha, beta;
-  // scalars read back at a restart-cycle end or at the end of a Lanczos run:
-  // a place in one of the batch's pinned read slabs (the process sits out the
-  // sweeps until that slab's copy has completed)
+  // scalars of a finished Lanczos run: a place in one of the batch's pinned
+  // read slabs (the run sits out the sweeps until that copy has completed)
   const double* h_read = nullptr;
-  int slab = -1;
   bool awaiting = false;
 };
 
@@ -67,42 +68,44 @@ struct BOpHost {
   int mode = 0;  // OP_* bits
   double scale = 1.0;
   double* nrm_out = nullptr;  // extra copy of the norm, or null
+  // eigensolver operation: X, w, cnt and hsum follow the run's control block
+  // (basis buffers X and X1, basis size m)
+  const double* ctl = nullptr;
+  const double* X1 = nullptr;
+  int m = 0;
+  // Lanczos step (G_LZ): the grouped H*v's per-task shares of <v, w>
+  const double* tdot = nullptr;
+  int64_t ntdot = 0;
 };
 
-enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8 };
-enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2, G_REORTH = 3 };
+// OP_LZS: the Lanczos vectors are kept unnormalised (the next step's H*v
+// and update scale them), so the norm also sets the next step's scales.
+enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8, OP_LZS = 16 };
+enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2, G_REORTH = 3, G_LZ = 4 };
 
 struct OpGroup {
   int kind;
   std::vector<BOpHost> ops;
 };
 
-// Krylov-basis combination of one process, queued by the host logic of a
-// restart-cycle end and launched with the next sweep: Y_j = sum_i C[j rows +
-// i] X_i (j < cols, i < rows), and `zero` cleared.
-struct CombineReq {
-  const double* X = nullptr;
-  int64_t n = 0;
-  int rows = 0, cols = 0;
-  std::vector<double> C;
-  double* Y = nullptr;
-  double* zero = nullptr;
-  int nzero = 0;
-};
-
-// Scalars of one process to be copied back with the sweep's grouped read.
+// Scalars of one process to be copied back with the sweep's grouped read
+// (an eigensolver run's status every sweep, a Lanczos run's coefficients at
+// its end).
 struct ReadReq {
-  BJob* j;
+  int job;
   const double* src;
   size_t count;
 };
 
-// Pinned host buffer receiving one sweep's grouped read.
+// Pinned host buffer receiving one sweep's grouped read; its entries are
+// handled, in sweep order, once the copy has completed.
 struct ReadSlab {
   double* host = nullptr;
   size_t cap = 0;
   hipEvent_t ev = nullptr;
-  int refs = 0;
+  bool busy = false;
+  std::vector<ReadReq> entries;
+  std::vector<size_t> offs;  // entry q's place in host
 };
 
 struct GfRequest {
@@ -123,17 +126,11 @@ struct Batch {
   std::map<std::tuple<int, int, int>, std::vector<void*>> maps;  // (sector, orbital, annihilate) -> gather maps
   DevBuf d_gather;
   std::vector<ReadSlab> slabs;
-  std::vector<CombineReq> pending_comb;
-  // Krylov bases of finished eigensolver runs: freed once the event after
-  // the combination that last reads them has completed
+  std::vector<int> slab_fifo;  // slabs in flight, oldest first
+  // Krylov bases of finished eigensolver runs, freed together (a free
+  // waits for the device)
   std::vector<void*> pending_free;
   size_t pending_free_bytes = 0;
-  struct Deferred {
-    hipEvent_t ev;
-    std::vector<void*> ptrs;
-    size_t bytes;
-  };
-  std::vector<Deferred> deferred;
   size_t bytes_in_use = 0;
   int hv_block = 256;
   long long sweeps = 0;
@@ -150,22 +147,18 @@ struct Batch {
   void submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms);
   void free_job(int id);
   int step();
-  // launches the queued basis combinations (Ritz vectors of runs that have
-  // just finished) if no sweep has done so yet
-  void flush();
   int autotune(const std::vector<int>& sec_ids);
 
  private:
   double* dalloc(BJob& j, size_t count);
   void release_job(BJob& j);
-  void complete(BJob& j);
-  void release_read(BJob& j);
-  void reap_deferred(bool wait);
+  bool drain_slab(bool wait);
+  void eig_status(BJob& j, const double* h);
+  void free_bases(bool all);
   void finish_dense(BJob& j);
-  void end_cycle(BJob& j);
   void finish_lanczos(BJob& j);
   void issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv, std::vector<OpGroup>& post,
-             const std::vector<ReadReq>* reads = nullptr);
+             const std::vector<BJob*>* cycles = nullptr, const std::vector<ReadReq>* reads = nullptr);
   void fill_dev(double* p, size_t n);
   void gather_maps(int src, int orb, bool annihilate, std::vector<void*>& out);
 };

@@ -47,7 +47,26 @@ constexpr int S_ONE = kMaxBasis + 1;     // 1.0  } H*v scales of a Lanczos step:
 constexpr int S_NEGNRM = kMaxBasis + 2;  // -nrm }  v scale, w scale
 constexpr int S_NB = kMaxBasis + 3;      // norm before the first projection pass
 constexpr int S_GATE = kMaxBasis + 4;    // 1 when the second pass is needed
+constexpr int S_LZV = kMaxBasis + 5;     // } Lanczos run: scale of the stored current vector (1 / beta_{k-1})
+constexpr int S_LZW = kMaxBasis + 6;     // }   and of the previous one in w = H v_k - beta_{k-1} v_{k-1}
 constexpr int S_T = 2 * kMaxBasis;       // projected-matrix columns / Lanczos history
+
+// Eigensolver control block (after the projected-matrix block of a run's
+// scalars): the fields the device kernels keep, then the Ritz values of a
+// finished run, the Ritz values kept by the last restart and the
+// combination matrix of the last cycle end.
+constexpr int C_K = 3;        // kept Ritz vectors (columns 0..k-1 of the basis)
+constexpr int C_ITER = 4;     // restart cycles done
+constexpr int C_NCONV = 5;    // converged Ritz pairs at the last cycle end
+constexpr int C_NAV = 6;      // H*v products
+constexpr int C_PENDING = 7;  // 1: thick restart, 2: Ritz vectors to combine this sweep
+constexpr int C_SRC = 8;      // basis buffer the combination reads
+constexpr int C_ROWS = 9;     // rows and columns of the combination matrix
+constexpr int C_COLS = 10;
+constexpr int C_HDR = 16;     // header size; Ritz values follow
+__host__ __device__ inline int ctl_theta(int m) { return C_HDR + m; }
+__host__ __device__ inline int ctl_cmat(int m) { return C_HDR + 2 * m; }
+__host__ __device__ inline size_t ctl_size(int m) { return (size_t)C_HDR + 2 * m + (size_t)(m + 1) * (m + 1); }
 
 struct BOp {
   const double* X;
@@ -65,10 +84,33 @@ struct BOp {
   int mode;
   double scale;
   double* nrm_out;
+  const double* ctl;  // eigensolver control block or null (then X1, m as below)
+  const double* X1;   // second basis buffer
+  int m;              // basis size
+  const double* tdot;  // Lanczos step: per-task shares of <v, w>
+  int64_t ntdot;
 };
+
+// An eigensolver operation follows its run's control block: the operand is
+// column col + 1 of the current basis buffer, columns 0..col are projected
+// out and the projections accumulate into column col of the projected
+// matrix.
+__device__ __forceinline__ BOp resolve(const BOp& in) {
+  BOp o = in;
+  if (o.ctl) {
+    const int64_t col = (int64_t)o.ctl[kCtlCol];
+    const double* X = o.ctl[kCtlPhase] != 0.0 ? o.X1 : o.X;
+    o.X = X;
+    o.cnt = (int)col + 1;
+    o.w = const_cast<double*>(X) + (col + 1) * o.ldx;
+    if (o.hsum) o.hsum = o.sc + S_T + col * (o.m + 1);
+  }
+  return o;
+}
 
 __device__ __forceinline__ bool op_skip(const BOp& o) {
   if (o.flags && o.flags[0]) return true;
+  if (o.ctl && o.ctl[kCtlState] != 0.0) return true;
   if ((o.mode & OP_GATED) && o.sc[S_GATE] == 0.0) return true;
   return false;
 }
@@ -113,7 +155,7 @@ __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__
                                                          double* __restrict__ partial) {
   __shared__ double s_part[kOpBlock / 64][kHGroup];
   const int c = blockIdx.x;
-  const BOp& o = ops[cjob[c]];
+  const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o)) return;
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * o.chunk;
@@ -146,7 +188,7 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_dots_kernel(const BOp* __res
   __shared__ double s_part[kOpBlock / 64][kFuse + 1];
   __shared__ double s_h[kFuse];
   const int c = blockIdx.x;
-  const BOp& o = ops[cjob[c]];
+  const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o)) return;
   if ((int)threadIdx.x < kFuse) s_h[threadIdx.x] = (int)threadIdx.x < o.cnt ? o.sc[threadIdx.x] : 0.0;
   __syncthreads();
@@ -190,7 +232,7 @@ __device__ __forceinline__ double wave_chunk_sum(const BOp& o, const double* par
 
 __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ ops, const double* __restrict__ partial,
                                                       int mode, int gated) {
-  const BOp& o = ops[blockIdx.x];
+  const BOp o = resolve(ops[blockIdx.x]);
   if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (mode == 2) {
@@ -248,6 +290,12 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
         o.flags[1] = o.kstep;
         if (fabs(nrm) < 1e-10) o.flags[0] = 1;
       }
+      if (o.mode & OP_LZS) {
+        // next step: v_{k+1} = w / beta_k stays stored as w, and the term
+        // beta_k v_k = beta_k (scale of v_k) u_k
+        o.sc[S_LZW] = -nrm * o.sc[S_LZV];
+        o.sc[S_LZV] = 1.0 / nrm;
+      }
     }
   }
 }
@@ -260,7 +308,7 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
   __shared__ double s_red[8];
   __shared__ double s_h[kMaxBasis + kHGroup];
   const int c = blockIdx.x;
-  const BOp& o = ops[cjob[c]];
+  const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
   for (int i = threadIdx.x; i < kMaxBasis + kHGroup; i += blockDim.x) s_h[i] = i < o.cnt ? o.sc[i] : 0.0;
   __syncthreads();
@@ -287,7 +335,7 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
 // w[e] *= 1 / nrm (OP_SCALE_NRM) or *= scale
 __global__ __launch_bounds__(kOpBlock) void bscale_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob) {
   const int c = blockIdx.x;
-  const BOp& o = ops[cjob[c]];
+  const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o)) return;
   const double f = (o.mode & OP_SCALE_NRM) ? 1.0 / o.sc[S_NRM] : o.scale;
   const int lc = c - o.chunk0;
@@ -296,22 +344,304 @@ __global__ __launch_bounds__(kOpBlock) void bscale_kernel(const BOp* __restrict_
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) o.w[e] *= f;
 }
 
+// Lanczos step (one block per run): alpha = sum of the grouped H*v's
+// per-task shares of <v, w> (thread l takes shares l, l + 256, ... into four
+// running sums, then a fixed block tree); the update coefficient for the
+// stored (unnormalised) current vector is alpha times its scale.
+__global__ __launch_bounds__(256) void blz_alpha_kernel(const BOp* __restrict__ ops) {
+  __shared__ double s_red[8];
+  const BOp o = ops[blockIdx.x];
+  if (op_skip(o)) return;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+  int64_t q = threadIdx.x;
+  for (; q + 768 < o.ntdot; q += 1024) {
+    t0 += o.tdot[q];
+    t1 += o.tdot[q + 256];
+    t2 += o.tdot[q + 512];
+    t3 += o.tdot[q + 768];
+  }
+  for (; q < o.ntdot; q += 256) t0 += o.tdot[q];
+  const double t = block_sum256((t0 + t1) + (t2 + t3), s_red);
+  if (threadIdx.x == 0) {
+    *o.hist_a = t;
+    o.sc[0] = t * o.sc[S_LZV];
+  }
+}
+
 __global__ void bfill_kernel(double* p, int64_t n, double x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = x;
 }
 
-// Grouped Krylov-basis combination (thick restarts and Ritz vectors of every
-// process whose cycle ended): per chunk, Y_j[e] = sum_i C[j rows + i] X_i[e]
-// with the X_i[e] in registers; the first chunk of a process also clears its
-// projected-matrix block.
+// ---------------------------------------------------------------------------
+// Restart-cycle ends on the device.  After every sweep one 64-lane block per
+// eigensolver run advances its column; at a cycle end it assembles the
+// projected matrix, diagonalises it, tests convergence (ARPACK's bound
+// |beta_m s_{m,i}| <= tol max(|theta_i|, eps^(2/3))) and either plans the
+// thick restart or the Ritz vectors; the grouped combination launched next
+// carries that plan out.  The host only reads the run's state back.
+// ---------------------------------------------------------------------------
+
+__host__ __device__ inline int restart_keep(int nev, int m, int nconv) {
+  int kk = nev + (nconv < (m - nev) / 2 ? nconv : (m - nev) / 2);
+  if (kk > m - 2) kk = m - 2;
+  return kk < 1 ? 1 : kk;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;  // on every lane
+}
+
+// Value of x held by lane l (l uniform over the wavefront).
+__device__ __forceinline__ double lane_value(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Eigen-decomposition of the symmetric n x n matrix a (row-major, LDS) by
+// one 64-lane block: Householder reduction to tridiagonal form accumulated
+// in q, implicit QL with shifts, then a stable ascending sort (the host
+// sym_eig's algorithm).  The vector operations are spread over the lanes
+// (sums by shuffle trees); during QL lane i holds d_i and e_i in registers,
+// every lane evaluates the same scalar recurrence from values read across
+// lanes, and lane r rotates row r of q (stored transposed, qt[c n + r]).  On
+// return w holds the eigenvalues and column j of a the eigenvector of w[j].
+// Returns false when QL does not converge.
+__device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq, double* w, int* order) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n * n; i += 64) qt[i] = (i / n == i % n) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    const double xi = lane < m ? a[(k + 1 + lane) * n + k] : 0.0;
+    const double nx = sqrt(wave_sum(xi * xi));
+    if (nx == 0.0) continue;
+    const double x0 = lane_value(xi, 0);
+    const double alpha = x0 > 0 ? -nx : nx;
+    double vi = lane < m ? xi - (lane == 0 ? alpha : 0.0) : 0.0;
+    const double nv = sqrt(wave_sum(vi * vi));
+    if (nv == 0.0) continue;
+    vi /= nv;
+    if (lane < m) v[lane] = vi;
+    __syncthreads();
+    double pi = 0.0;
+    if (lane < m) {
+#pragma unroll 4
+      for (int j = 0; j < m; ++j) pi += a[(k + 1 + lane) * n + k + 1 + j] * v[j];
+    }
+    const double K = wave_sum(vi * pi);
+    const double qi = pi - K * vi;
+    if (lane < m) qq[lane] = qi;
+    __syncthreads();
+    if (lane < m) {
+#pragma unroll 4
+      for (int j = 0; j < m; ++j) a[(k + 1 + lane) * n + k + 1 + j] -= 2.0 * (vi * qq[j] + qi * v[j]);
+    }
+    if (lane == 0) {
+      a[(k + 1) * n + k] = alpha;
+      a[k * n + k + 1] = alpha;
+    }
+    if (lane >= 1 && lane < m) {
+      a[(k + 1 + lane) * n + k] = 0.0;
+      a[k * n + k + 1 + lane] = 0.0;
+    }
+    if (lane < n) {
+      double sum = 0.0;
+#pragma unroll 4
+      for (int j = 0; j < m; ++j) sum += qt[(k + 1 + j) * n + lane] * v[j];
+#pragma unroll 4
+      for (int j = 0; j < m; ++j) qt[(k + 1 + j) * n + lane] -= 2.0 * sum * v[j];
+    }
+    __syncthreads();
+  }
+  double dr = lane < n ? a[lane * n + lane] : 0.0;
+  double er = lane + 1 < n ? a[(lane + 1) * n + lane] : 0.0;
+  const double eps = 2.220446049250313e-16;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    for (;;) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = fabs(lane_value(dr, m)) + fabs(lane_value(dr, m + 1));
+        if (fabs(lane_value(er, m)) <= eps * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) return false;
+      const double dl = lane_value(dr, l), el = lane_value(er, l);
+      double g = (lane_value(dr, l + 1) - dl) / (2.0 * el);
+      double r = sqrt(fma(g, g, 1.0));
+      g = lane_value(dr, m) - dl + el / (g + copysign(r, g));
+      double sn = 1.0, c = 1.0, pp = 0.0;
+      bool early = false;
+      for (int i = m - 1; i >= l; --i) {
+        const double ei = lane_value(er, i);
+        const double f = sn * ei;
+        const double b = c * ei;
+        // the entries are moderate (Ritz values and couplings of H), so the
+        // plain square root cannot overflow; one reciprocal serves s and c
+        r = sqrt(fma(f, f, g * g));
+        if (lane == i + 1) er = r;
+        if (r == 0.0) {
+          if (lane == i + 1) dr -= pp;
+          if (lane == m) er = 0.0;
+          early = true;
+          break;
+        }
+        const double rinv = 1.0 / r;
+        sn = f * rinv;
+        c = g * rinv;
+        g = lane_value(dr, i + 1) - pp;
+        r = (lane_value(dr, i) - g) * sn + 2.0 * c * b;
+        pp = sn * r;
+        if (lane == i + 1) dr = g + pp;
+        g = c * r - b;
+        if (lane < n) {
+          const double qf = qt[(i + 1) * n + lane];
+          const double q0 = qt[i * n + lane];
+          qt[(i + 1) * n + lane] = sn * q0 + c * qf;
+          qt[i * n + lane] = c * q0 - sn * qf;
+        }
+      }
+      if (early) continue;
+      if (lane == l) {
+        dr -= pp;
+        er = g;
+      }
+      if (lane == m) er = 0.0;
+    }
+  }
+  // stable ascending order: lane i's rank among (d, index)
+  if (lane < n) {
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = lane_value(dr, j);
+      rk += (dj < dr || (dj == dr && j < lane)) ? 1 : 0;
+    }
+    order[rk] = lane;
+    qq[lane] = dr;
+  }
+  __syncthreads();
+  if (lane < n) {
+    w[lane] = qq[order[lane]];
+#pragma unroll 4
+    for (int j = 0; j < n; ++j) a[lane * n + j] = qt[order[j] * n + lane];
+  }
+  __syncthreads();
+  return true;
+}
+
+struct BCyc {
+  double* sc;
+  double* ctl;
+  int m, nev, maxitr;
+  double tol;
+};
+
+__global__ __launch_bounds__(64) void bcycle_kernel(const BCyc* __restrict__ ops) {
+  extern __shared__ double s_e[];
+  __shared__ int s_done[kMaxBasis];
+  __shared__ int s_nconv;
+  const BCyc o = ops[blockIdx.x];
+  double* ctl = o.ctl;
+  const int lane = threadIdx.x;
+  const int m = o.m;
+  const double state = ctl[kCtlState];
+  const int col = (int)ctl[kCtlCol];
+  __syncthreads();  // every lane has read the header before lane 0 writes it
+  if (lane == 0) ctl[C_PENDING] = 0.0;
+  if (state != 0.0) return;
+  if (lane == 0) ctl[C_NAV] += 1.0;
+  if (col + 1 < m) {
+    if (lane == 0) ctl[kCtlCol] = col + 1;
+    return;
+  }
+  // cycle end: T from the projections (columns k..m-1) and the kept Ritz values
+  const int k = (int)ctl[C_K];
+  const int phase = (int)ctl[kCtlPhase];
+  double* a = s_e;
+  double* qt = a + m * m;
+  double* v = qt + m * m;
+  double* qq = v + m;
+  double* w = qq + m;
+  int* order = (int*)(w + m);
+  const double* Tcol = o.sc + S_T;
+  const double* theta_kept = ctl + ctl_theta(m);
+  for (int i = lane; i < m * m; i += 64) {
+    const int r = i / m, c = i % m;
+    const int hi = r > c ? r : c, lo = r > c ? c : r;
+    a[i] = (hi < k) ? (r == c ? theta_kept[r] : 0.0) : Tcol[(size_t)hi * (m + 1) + lo];
+  }
+  __syncthreads();
+  if (!wave_sym_eig(m, a, qt, v, qq, w, order)) {
+    if (lane == 0) ctl[kCtlState] = 2.0;
+    return;
+  }
+  const double bres = o.sc[S_NRM];
+  if (lane == 0) {
+    const double eps23 = 3.666852862501036e-11;  // eps^(2/3)
+    int nc = 0;
+    for (int i = 0; i < o.nev; ++i) {
+      const double bound = fabs(bres * a[(m - 1) * m + i]);
+      if (bound <= o.tol * fmax(fabs(w[i]), eps23)) s_done[nc++] = i;
+    }
+    s_nconv = nc;
+  }
+  __syncthreads();
+  const int nc = s_nconv;
+  const int iter = (int)ctl[C_ITER] + 1;
+  double* C = ctl + ctl_cmat(m);
+  if (nc == o.nev || iter >= o.maxitr) {
+    // the converged Ritz pairs are the result (after maxitr: dseupd with info = 1)
+    if (lane < nc) ctl[C_HDR + lane] = w[s_done[lane]];
+    if (lane < m)
+      for (int c = 0; c < nc; ++c) C[c * m + lane] = a[lane * m + s_done[c]];
+    __syncthreads();
+    if (lane == 0) {
+      ctl[C_ITER] = iter;
+      ctl[C_NCONV] = nc;
+      ctl[C_SRC] = phase;
+      ctl[C_ROWS] = m;
+      ctl[C_COLS] = nc;
+      ctl[C_PENDING] = nc > 0 ? 2.0 : 0.0;
+      ctl[kCtlState] = 1.0;
+    }
+    return;
+  }
+  // thick restart: new basis column c < kk = sum_i A[i][c] V_i, column kk = V_m
+  const int kk = restart_keep(o.nev, m, nc);
+  if (lane <= m) {
+    for (int c = 0; c < kk; ++c) C[c * (m + 1) + lane] = lane < m ? a[lane * m + c] : 0.0;
+    C[kk * (m + 1) + lane] = lane == m ? 1.0 : 0.0;
+  }
+  if (lane < kk) ctl[ctl_theta(m) + lane] = w[lane];
+  for (int i = lane; i < (m + 1) * (m + 1); i += 64) o.sc[S_T + i] = 0.0;
+  __syncthreads();
+  if (lane == 0) {
+    ctl[C_ITER] = iter;
+    ctl[C_NCONV] = nc;
+    ctl[C_K] = kk;
+    ctl[kCtlCol] = kk;
+    ctl[C_SRC] = phase;
+    ctl[kCtlPhase] = phase ^ 1;
+    ctl[C_ROWS] = m + 1;
+    ctl[C_COLS] = kk + 1;
+    ctl[C_PENDING] = 1.0;
+  }
+}
+
+// Grouped combination planned by the cycle ends of this sweep (chunks of
+// runs with nothing pending return at once): Y_j[e] = sum_i C[j rows + i]
+// X_i[e], X = the basis buffer read, Y = the other buffer (restart) or the
+// Ritz-vector array (finished run).
 struct BComb {
-  const double* X;
-  double* Y;
-  const double* C;
-  double* zero;
+  const double* X0;
+  double* X1;
+  double* evecs;
+  const double* ctl;
   int64_t n;
-  int rows, cols, nzero;
-  int chunk0, nch, chunk;
+  int m, chunk0, nch, chunk;
 };
 
 __global__ __launch_bounds__(kOpBlock) void bcombine_kernel(const BComb* __restrict__ ops,
@@ -319,23 +649,28 @@ __global__ __launch_bounds__(kOpBlock) void bcombine_kernel(const BComb* __restr
   __shared__ double s_c[kMaxBasis * kMaxBasis];
   const int c = blockIdx.x;
   const BComb& o = ops[cjob[c]];
+  const double pend = o.ctl[C_PENDING];
+  if (pend == 0.0) return;
+  const int src = (int)o.ctl[C_SRC];
+  const double* X = src ? o.X1 : o.X0;
+  double* Y = pend == 2.0 ? o.evecs : (src ? const_cast<double*>(o.X0) : o.X1);
+  const int rows = (int)o.ctl[C_ROWS], cols = (int)o.ctl[C_COLS];
+  const double* C = o.ctl + ctl_cmat(o.m);
   const int lc = c - o.chunk0;
-  for (int i = threadIdx.x; i < o.rows * o.cols; i += kOpBlock) s_c[i] = o.C[i];
-  if (lc == 0 && o.zero)
-    for (int i = threadIdx.x; i < o.nzero; i += kOpBlock) o.zero[i] = 0.0;
+  for (int i = threadIdx.x; i < rows * cols; i += kOpBlock) s_c[i] = C[i];
   __syncthreads();
   const int64_t e0 = (int64_t)lc * o.chunk;
   const int64_t e1 = min(o.n, e0 + o.chunk);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
-    double v[kMaxBasis];
+    double xv[kMaxBasis];
 #pragma unroll
-    for (int i = 0; i < kMaxBasis; ++i) v[i] = i < o.rows ? o.X[(int64_t)i * o.n + e] : 0.0;
-    for (int jj = 0; jj < o.cols; ++jj) {
+    for (int i = 0; i < kMaxBasis; ++i) xv[i] = i < rows ? X[(int64_t)i * o.n + e] : 0.0;
+    for (int jj = 0; jj < cols; ++jj) {
       double y = 0.0;
 #pragma unroll
       for (int i = 0; i < kMaxBasis; ++i)
-        if (i < o.rows) y = fma(s_c[jj * o.rows + i], v[i], y);
-      o.Y[(int64_t)jj * o.n + e] = y;
+        if (i < rows) y = fma(s_c[jj * rows + i], xv[i], y);
+      Y[(int64_t)jj * o.n + e] = y;
     }
   }
 }
@@ -353,12 +688,6 @@ __global__ void bgather_kernel(const BGather* __restrict__ ops, double* __restri
   for (int i = threadIdx.x; i < o.count; i += blockDim.x) dst[o.off + i] = o.src[i];
 }
 
-uint64_t splitmix64(uint64_t& x) {
-  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
 
 // Chunk size of an operand of n elements: about n / 256, a power of two in
 // [kChunkMin, kChunkMax] -- it depends on n only, so a process reduces the
@@ -379,7 +708,6 @@ Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& j : jobs)
     if (j) release_job(*j);
-  reap_deferred(true);
   for (void* p : pending_free) (void)hipFree(p);
   for (ReadSlab& r : slabs) {
     if (r.host) (void)hipHostFree(r.host);
@@ -409,43 +737,78 @@ double* Batch::dalloc(BJob& j, size_t count) {
 }
 
 void Batch::release_job(BJob& j) {
-  release_read(j);
   for (void* p : j.allocs) (void)hipFree(p);
   j.allocs.clear();
   bytes_in_use -= j.bytes;
   j.bytes = 0;
-}
-
-// Gives back a process's place in its read slab.
-void Batch::release_read(BJob& j) {
-  if (j.slab >= 0) --slabs[j.slab].refs;
-  j.slab = -1;
   j.h_read = nullptr;
 }
 
-// Frees the bases whose last reader has completed (all of them when wait).
-void Batch::reap_deferred(bool wait) {
-  size_t keep = 0;
-  for (size_t q = 0; q < deferred.size(); ++q) {
-    Deferred& d = deferred[q];
-    if (wait) (void)hipEventSynchronize(d.ev);
-    if (wait || hipEventQuery(d.ev) == hipSuccess) {
-      for (void* p : d.ptrs) (void)hipFree(p);
-      bytes_in_use -= d.bytes;
-      (void)hipEventDestroy(d.ev);
-    } else {
-      deferred[keep++] = d;
-    }
-  }
-  deferred.resize(keep);
+// Frees the bases of finished eigensolver runs, several at a time (all of
+// them when `all`).
+void Batch::free_bases(bool all) {
+  if (pending_free.empty() || (!all && pending_free.size() < 16)) return;
+  for (void* p : pending_free) (void)hipFree(p);
+  pending_free.clear();
+  bytes_in_use -= pending_free_bytes;
+  pending_free_bytes = 0;
 }
 
-// Host part of a process once its scalars have arrived.
-void Batch::complete(BJob& j) {
-  j.awaiting = false;
-  if (j.kind == JOB_EIG) end_cycle(j);
-  else finish_lanczos(j);
-  release_read(j);
+// An eigensolver run's status as read back after a sweep: header of its
+// control block, then its Ritz values.
+void Batch::eig_status(BJob& j, const double* h) {
+  if (j.state != JOB_ACTIVE) return;
+  j.n_av = (int)h[C_NAV];
+  j.iter = (int)h[C_ITER];
+  j.nconv = (int)h[C_NCONV];
+  const double st = h[kCtlState];
+  if (st == 0.0) return;
+  if (st == 1.0) {
+    const int nk = (int)h[C_NCONV];
+    j.evals.assign(h + C_HDR, h + C_HDR + nk);
+    j.nev = nk;
+    j.state = JOB_DONE;
+  } else {
+    j.state = JOB_FAILED;  // the projected matrix's QL iteration did not converge
+  }
+  // the bases are no longer read (the Ritz vectors were combined before this status was)
+  for (double* b : {j.V, j.Valt}) {
+    auto it = std::find(j.allocs.begin(), j.allocs.end(), (void*)b);
+    if (it != j.allocs.end()) {
+      const size_t bb = (size_t)(j.m + 1) * j.n * sizeof(double);
+      j.allocs.erase(it);
+      j.bytes -= bb;
+      pending_free.push_back(b);
+      pending_free_bytes += bb;
+    }
+  }
+  j.V = j.Valt = nullptr;
+}
+
+// Handles the oldest grouped read if its copy has completed (waits for it
+// when `wait`); returns whether it was handled.
+bool Batch::drain_slab(bool wait) {
+  if (slab_fifo.empty()) return false;
+  ReadSlab& r = slabs[slab_fifo.front()];
+  if (wait) EDH_HIP(hipEventSynchronize(r.ev));
+  else if (hipEventQuery(r.ev) != hipSuccess) return false;
+  for (size_t q = 0; q < r.entries.size(); ++q) {
+    const int id = r.entries[q].job;
+    if (id < 0 || id >= (int)jobs.size() || !jobs[id]) continue;
+    BJob& j = *jobs[id];
+    if (j.kind == JOB_EIG) {
+      eig_status(j, r.host + r.offs[q]);
+    } else if (j.state == JOB_ACTIVE && j.awaiting) {
+      j.awaiting = false;
+      j.h_read = r.host + r.offs[q];
+      finish_lanczos(j);
+    }
+  }
+  r.busy = false;
+  r.entries.clear();
+  r.offs.clear();
+  slab_fifo.erase(slab_fifo.begin());
+  return true;
 }
 
 int Batch::add_sector(const Model& m, int nup, int ndn) {
@@ -551,17 +914,15 @@ int Batch::submit_eig(int sec, int nev_req, int ncv_req, double tol, int maxitr,
   EDH_CHECK(j.nev < j.m - 1, EDH_ERR_LIMIT, "batch: NEV too large for the device Krylov basis (max 46)");
   j.V = dalloc(j, (size_t)(j.m + 1) * n);
   j.Valt = dalloc(j, (size_t)(j.m + 1) * n);
-  j.sc = dalloc(j, S_T + (size_t)(j.m + 1) * (j.m + 1));
-  fill_dev(j.sc, S_T + (size_t)(j.m + 1) * (j.m + 1));
-  j.T.assign((size_t)j.m * j.m, 0.0);
+  // scalars: projections, norms, projected matrix, then the control block
+  // (all zero: column 0, phase 0, active)
+  const size_t nsc = S_T + (size_t)(j.m + 1) * (j.m + 1) + ctl_size(j.m);
+  j.sc = dalloc(j, nsc);
+  fill_dev(j.sc, nsc);
+  j.ctl = j.sc + S_T + (size_t)(j.m + 1) * (j.m + 1);
+  j.evecs = dalloc(j, (size_t)j.nev * n);
   // random start vector, uniform in [-1, 1) (the single-sector eigensolver's generator)
-  std::vector<double> x(n);
-  uint64_t st = seed ? seed : 0x243F6A8885A308D3ull;
-  for (int64_t i = 0; i < n; ++i) x[i] = (double)(splitmix64(st) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-  EDH_HIP(hipMemcpyAsync(j.V, x.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipStreamSynchronize(stream));
-  j.k = 0;
-  j.col = 0;
+  dev_random_uniform(j.V, n, seed, stream);
   j.pending_start_norm = true;  // normalised by the next sweep
   j.state = JOB_ACTIVE;
   return id;
@@ -587,9 +948,12 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
   for (int q = 0; q < 3; ++q) j.lz[q] = dalloc(j, (size_t)j.n);
   j.sc = dalloc(j, S_T + 2 * (size_t)nl + 2);
   fill_dev(j.sc, S_T + 2 * (size_t)nl + 2);
+  dev_fill(j.sc + S_LZV, 1, 1.0, stream);  // the start vector is normalised
   j.d_alpha = j.sc + S_T;
   j.d_beta = j.d_alpha + nl;
   j.d_flags = (int*)(j.d_beta + nl + 1);
+  j.ntdot = hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
+  j.tdot = dalloc(j, (size_t)j.ntdot);
   if (v0_dev)
     EDH_HIP(hipMemcpyAsync(j.lz[0], v0_dev, (size_t)j.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   j.step = 1;
@@ -695,16 +1059,9 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
   issue(pre2, nullptr, post);
 }
 
-void Batch::flush() {
-  if (pending_comb.empty()) return;
-  std::vector<OpGroup> none;
-  issue(none, nullptr, none);
-}
-
 void Batch::free_job(int id) {
   BJob& j = job(id);
   EDH_CHECK(j.state != JOB_ACTIVE, EDH_ERR_STATE, "batch: job still active");
-  flush();
   if (!j.allocs.empty()) EDH_HIP(hipStreamSynchronize(stream));
   release_job(j);
   jobs[id].reset();
@@ -721,26 +1078,27 @@ void Batch::fill_dev(double* p, size_t n) {
 // grouped H*v, groups after it) into one pinned staging buffer, copies it to
 // the device in one transfer and launches the sequence.
 void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv,
-                  std::vector<OpGroup>& post, const std::vector<ReadReq>* reads) {
+                  std::vector<OpGroup>& post, const std::vector<BJob*>* cycles, const std::vector<ReadReq>* reads) {
   struct Placed {
     int kind, nops, nch;
     size_t ops_off, cj_off;
   };
   std::vector<Placed> pl_pre, pl_post;
   size_t off = 0;
-  // queued basis combinations: descriptors, chunk map, coefficient matrices
-  const int ncomb = (int)pending_comb.size();
-  int comb_nch = 0;
-  for (const CombineReq& r : pending_comb) comb_nch += nchunks(r.n);
+  // eigensolver cycle ends and the combinations they plan: one cycle
+  // descriptor per run, one combination descriptor per run with its chunks
+  const int ncyc = cycles ? (int)cycles->size() : 0;
+  int comb_nch = 0, max_m = 0;
+  for (int q = 0; q < ncyc; ++q) {
+    comb_nch += nchunks((*cycles)[q]->n);
+    max_m = std::max(max_m, (*cycles)[q]->m);
+  }
+  const size_t cyc_off = off;
+  off = align16(off + (size_t)ncyc * sizeof(BCyc));
   const size_t comb_ops_off = off;
-  off = align16(off + (size_t)ncomb * sizeof(BComb));
+  off = align16(off + (size_t)ncyc * sizeof(BComb));
   const size_t comb_cj_off = off;
   off = align16(off + (size_t)comb_nch * sizeof(int));
-  std::vector<size_t> comb_c_off(ncomb);
-  for (int q = 0; q < ncomb; ++q) {
-    comb_c_off[q] = off;
-    off = align16(off + pending_comb[q].C.size() * sizeof(double));
-  }
   auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
     for (OpGroup& g : gs) {
       if (g.ops.empty()) continue;
@@ -767,7 +1125,8 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     const int64_t spread = std::max<int64_t>(1, total_rows / 8192);
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
-      const int ru = (int)std::min<int64_t>(hv_batch_unit_rows(ds.dn.D), spread);
+      const int rg = hv_batch_row_group(ds.dn.D);
+      const int ru = (int)((std::min<int64_t>(hv_batch_unit_rows(ds.dn.D), spread) + rg - 1) / rg * rg);
       for (int64_t r = 0; r < ds.up.D; r += ru)
         units.push_back({(int)q, (int)r, (int)std::min<int64_t>(ru, ds.up.D - r), 0});
     }
@@ -824,6 +1183,11 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         o.mode = h.mode;
         o.scale = h.scale;
         o.nrm_out = h.nrm_out;
+        o.ctl = h.ctl;
+        o.X1 = h.X1;
+        o.m = h.m;
+        o.tdot = h.tdot;
+        o.ntdot = h.ntdot;
         std::memcpy(hb + p.ops_off + i * sizeof(BOp), &o, sizeof(BOp));
         for (int c = 0; c < o.nch; ++c) cj[c0 + c] = (int)i;
         c0 += o.nch;
@@ -840,22 +1204,21 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   {
     int c0 = 0;
     int* cj = (int*)(hb + comb_cj_off);
-    for (int q = 0; q < ncomb; ++q) {
-      const CombineReq& r = pending_comb[q];
+    for (int q = 0; q < ncyc; ++q) {
+      const BJob& j = *(*cycles)[q];
+      BCyc y{j.sc, j.ctl, j.m, j.nev, j.maxitr, j.tol};
+      std::memcpy(hb + cyc_off + q * sizeof(BCyc), &y, sizeof(BCyc));
       BComb b;
-      b.X = r.X;
-      b.Y = r.Y;
-      b.C = nullptr;  // the arena address, set once the arena is sized
-      b.zero = r.zero;
-      b.n = r.n;
-      b.rows = r.rows;
-      b.cols = r.cols;
-      b.nzero = r.nzero;
+      b.X0 = j.V;
+      b.X1 = j.Valt;
+      b.evecs = j.evecs;
+      b.ctl = j.ctl;
+      b.n = j.n;
+      b.m = j.m;
       b.chunk0 = c0;
-      b.nch = nchunks(r.n);
-      b.chunk = chunk_of(r.n);
+      b.nch = nchunks(j.n);
+      b.chunk = chunk_of(j.n);
       std::memcpy(hb + comb_ops_off + q * sizeof(BComb), &b, sizeof(BComb));
-      std::memcpy(hb + comb_c_off[q], r.C.data(), r.C.size() * sizeof(double));
       for (int c = 0; c < b.nch; ++c) cj[c0 + c] = q;
       c0 += b.nch;
     }
@@ -871,11 +1234,6 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     for (const Placed& p : *v) max_ch = std::max(max_ch, p.nch);
   d_arena.ensure(off);
   d_partial.ensure((size_t)max_ch * kMaxBasis * sizeof(double));
-  // the coefficient pointers of the combinations are arena addresses
-  for (int q = 0; q < ncomb; ++q) {
-    const double* cdev = (const double*)(d_arena.as<char>() + comb_c_off[q]);
-    std::memcpy(hb + comb_ops_off + q * sizeof(BComb) + offsetof(BComb, C), &cdev, sizeof(cdev));
-  }
   EDH_HIP(hipMemcpyAsync(d_arena.p, hb, off, hipMemcpyHostToDevice, stream));
   EDH_HIP(hipEventRecord(ev_stage[si], stream));
   ev_used[si] = true;
@@ -899,6 +1257,11 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
         hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 1);
+      } else if (p.kind == G_LZ) {
+        // alpha from the H*v's shares, w -= alpha v_k with |w|^2, beta_k and scales
+        hipLaunchKernelGGL(blz_alpha_kernel, dim3(p.nops), dim3(256), 0, stream, dops);
+        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_NORM) {
         hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
@@ -907,32 +1270,24 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
     }
   };
-  if (ncomb) {
-    hipLaunchKernelGGL(bcombine_kernel, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
-                       (const int*)(db + comb_cj_off));
-    pending_comb.clear();
-    if (!pending_free.empty()) {
-      Deferred d;
-      EDH_HIP(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming));
-      EDH_HIP(hipEventRecord(d.ev, stream));
-      d.ptrs.swap(pending_free);
-      d.bytes = pending_free_bytes;
-      pending_free_bytes = 0;
-      deferred.push_back(std::move(d));
-    }
-  }
   launch(pl_pre);
   if (has_hv)
     launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, stream);
   launch(pl_post);
+  if (ncyc) {
+    const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);
+    hipLaunchKernelGGL(bcycle_kernel, dim3(ncyc), dim3(64), smem, stream, (const BCyc*)(db + cyc_off));
+    hipLaunchKernelGGL(bcombine_kernel, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
+                       (const int*)(db + comb_cj_off));
+  }
   if (nread) {
     d_gather.ensure(read_total * sizeof(double));
     hipLaunchKernelGGL(bgather_kernel, dim3(nread), dim3(256), 0, stream, (const BGather*)(db + gat_off),
                        d_gather.as<double>());
-    // a free slab (no reader left) large enough, else a new one
+    // a free slab large enough, else a new one
     int si2 = -1;
     for (size_t q = 0; q < slabs.size(); ++q)
-      if (slabs[q].refs == 0 && slabs[q].cap >= read_total) {
+      if (!slabs[q].busy && slabs[q].cap >= read_total) {
         si2 = (int)q;
         break;
       }
@@ -947,53 +1302,48 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     ReadSlab& r = slabs[si2];
     EDH_HIP(hipMemcpyAsync(r.host, d_gather.p, read_total * sizeof(double), hipMemcpyDeviceToHost, stream));
     EDH_HIP(hipEventRecord(r.ev, stream));
+    r.busy = true;
+    r.entries = *reads;
+    r.offs.clear();
     size_t roff = 0;
     for (const ReadReq& q : *reads) {
-      q.j->h_read = r.host + roff;
-      q.j->slab = si2;
-      q.j->awaiting = true;
-      ++r.refs;
+      r.offs.push_back(roff);
       roff += q.count;
     }
+    slab_fifo.push_back(si2);
   }
   EDH_HIP(hipGetLastError());
 }
 
 // One sweep: every active process advances one Krylov step.
 int Batch::step() {
-  reap_deferred(false);
-  // processes whose scalars have arrived go on; when every active process is
-  // waiting, wait for the oldest copies
-  {
-    std::vector<BJob*> wait;
-    bool any_running = false;
-    for (auto& jp : jobs) {
-      if (!jp || jp->state != JOB_ACTIVE) continue;
-      if (!jp->awaiting) {
-        any_running = true;
-        continue;
-      }
-      if (hipEventQuery(slabs[jp->slab].ev) == hipSuccess) complete(*jp);
-      else wait.push_back(jp.get());
-    }
-    if (!any_running)
-      for (BJob* j : wait) {
-        EDH_HIP(hipEventSynchronize(slabs[j->slab].ev));
-        complete(*j);
-      }
+  // statuses and Lanczos coefficients that have arrived, in sweep order;
+  // when no process can run, wait for the oldest read
+  while (drain_slab(false)) {
+  }
+  auto runnable = [&]() {
+    for (auto& jp : jobs)
+      if (jp && jp->state == JOB_ACTIVE && !jp->awaiting) return true;
+    return false;
+  };
+  while (!runnable() && drain_slab(true)) {
   }
   std::vector<BJob*> eig, lz, starting;
-  for (auto& jp : jobs) {
+  std::vector<int> eig_id, lz_id;
+  for (size_t id = 0; id < jobs.size(); ++id) {
+    BJob* jp = jobs[id].get();
     if (!jp || jp->state != JOB_ACTIVE || jp->awaiting) continue;
     if (jp->kind == JOB_EIG) {
-      eig.push_back(jp.get());
-      if (jp->pending_start_norm) starting.push_back(jp.get());
+      eig.push_back(jp);
+      eig_id.push_back((int)id);
+      if (jp->pending_start_norm) starting.push_back(jp);
     } else {
-      lz.push_back(jp.get());
+      lz.push_back(jp);
+      lz_id.push_back((int)id);
     }
   }
+  free_bases(eig.empty());
   if (eig.empty() && lz.empty()) {
-    flush();  // Ritz vectors of runs that just finished
     int active = 0;
     for (auto& jp : jobs)
       if (jp && jp->state == JOB_ACTIVE) ++active;
@@ -1012,17 +1362,11 @@ int Batch::step() {
     g_scale.ops.push_back(o);
     j->pending_start_norm = false;
   }
-  // (1) Lanczos runs past their first step: v_k = w / beta_{k-1} in place,
-  // then rotate (prev, cur, w) <- (cur, w, prev)  (Lanczos.h:47-51)
+  // (1) Lanczos runs past their first step: rotate (prev, cur, w) <- (cur,
+  // w, prev) (Lanczos.h:47-51); v_k = w / beta_{k-1} is not formed, the
+  // stored vector is used with its scale
   for (BJob* j : lz)
     if (j->step > 1) {
-      BOpHost o;
-      o.w = j->lz[2];
-      o.n = j->n;
-      o.sc = j->sc;
-      o.flags = j->d_flags;
-      o.mode = OP_SCALE_NRM;
-      g_scale.ops.push_back(o);
       double* t = j->lz[1];
       j->lz[1] = j->lz[0];
       j->lz[0] = j->lz[2];
@@ -1030,16 +1374,19 @@ int Batch::step() {
     }
   pre.push_back(g_norm);
   pre.push_back(g_scale);
-  // (2) grouped H*v: eigensolver V_{col+1} = H V_col; Lanczos
-  // w = H v_k - beta_{k-1} v_{k-1} (Lanczos.h:55)
+  // (2) grouped H*v: eigensolver V_{col+1} = H V_col (column and buffer
+  // from the run's control block); Lanczos w = H v_k - beta_{k-1} v_{k-1}
+  // (Lanczos.h:55)
   std::vector<std::pair<int, HvArgs>> hv;
   for (BJob* j : eig) {
     HvArgs a{};
-    a.v = j->V + (size_t)j->col * j->n;
-    a.w = j->V + (size_t)(j->col + 1) * j->n;
+    a.v = j->V;
+    a.w = j->V;
     a.wprev = a.w;
     a.vscale = 1.0;
     a.mode = 0;
+    a.ctl = j->ctl;
+    a.v_alt = j->Valt;
     hv.push_back({j->sec, a});
   }
   for (BJob* j : lz) {
@@ -1049,23 +1396,26 @@ int Batch::step() {
     a.wprev = j->lz[1];
     a.vscale = 1.0;
     a.mode = j->step > 1 ? 1 : 0;
-    a.dscale = j->step > 1 ? j->sc + S_ONE : nullptr;  // {1, -beta_{k-1}}
+    a.dscale = j->step > 1 ? j->sc + S_LZV : nullptr;  // {1 / beta_{k-1}, -beta_{k-1} / beta_{k-2}}
     a.done = j->d_flags;
+    a.tdot = j->tdot;  // alpha = <v_k, w> from the same pass
     hv.push_back({j->sec, a});
   }
   // (3) first projection pass: eigensolver against V_0..V_col (into the
   // projected-matrix column, with the norm before for the DGKS test);
   // Lanczos alpha = <v, w>, w -= alpha v, beta = |w| (Lanczos.h:56-65)
-  OpGroup p1{G_PROJECT, {}}, p2{G_PROJECT, {}}, rf{G_REORTH, {}}, sc{G_SCALE, {}};
+  OpGroup p1{G_PROJECT, {}}, p2{G_PROJECT, {}}, rf{G_REORTH, {}}, sc{G_SCALE, {}}, glz{G_LZ, {}};
   for (BJob* j : eig) {
     BOpHost o;
     o.X = j->V;
+    o.X1 = j->Valt;
+    o.ctl = j->ctl;
+    o.m = j->m;
     o.ldx = j->n;
-    o.cnt = j->col + 1;
-    o.w = j->V + (size_t)(j->col + 1) * j->n;
+    o.w = j->V;
     o.n = j->n;
     o.sc = j->sc;
-    o.hsum = j->sc + S_T + (size_t)j->col * (j->m + 1);
+    o.hsum = j->sc + S_T;
     o.mode = OP_NB;
     if (j->m <= kFuse) {
       rf.ops.push_back(o);  // both passes in one group (fused)
@@ -1074,12 +1424,10 @@ int Batch::step() {
       o.mode = OP_GATED;  // (4) second pass only where the DGKS test asks for it
       p2.ops.push_back(o);
     }
-    BOpHost s;
-    s.w = o.w;
-    s.n = j->n;
-    s.sc = j->sc;
-    s.mode = OP_SCALE_NRM;
-    sc.ops.push_back(s);
+    BOpHost t = o;
+    t.hsum = nullptr;
+    t.mode = OP_SCALE_NRM;
+    sc.ops.push_back(t);
   }
   for (BJob* j : lz) {
     BOpHost o;
@@ -1093,28 +1441,29 @@ int Batch::step() {
     o.hist_b = j->step != j->nl ? j->d_beta + j->step : nullptr;  // Lanczos.h:64
     o.flags = j->d_flags;
     o.kstep = j->step;
-    o.mode = OP_LZ;
-    p1.ops.push_back(o);
+    o.mode = OP_LZ | OP_LZS;
+    o.tdot = j->tdot;
+    o.ntdot = j->ntdot;
+    glz.ops.push_back(o);
   }
+  post.push_back(glz);
   post.push_back(p1);
   post.push_back(rf);
   post.push_back(p2);
   post.push_back(sc);
-  // (5) restart-cycle ends and finished Lanczos runs: their scalars come
-  // back in the sweep's grouped read, and they sit out until it has arrived
+  // (5) cycle ends on the device, then the grouped read: every eigensolver
+  // run's status, and the coefficients of Lanczos runs that end here (they
+  // sit out until the read has arrived)
   std::vector<ReadReq> reads;
-  for (BJob* j : eig)
-    if (j->col + 1 >= j->m) reads.push_back({j, j->sc + S_NRM, (size_t)(S_T - S_NRM) + (size_t)j->m * (j->m + 1)});
-  for (BJob* j : lz)
-    if (j->step == j->nl) reads.push_back({j, j->d_alpha, 2 * (size_t)j->nl + 2});
-  issue(pre, &hv, post, &reads);
+  for (size_t q = 0; q < eig.size(); ++q) reads.push_back({eig_id[q], eig[q]->ctl, (size_t)C_HDR + eig[q]->nev});
+  for (size_t q = 0; q < lz.size(); ++q)
+    if (lz[q]->step == lz[q]->nl) reads.push_back({lz_id[q], lz[q]->d_alpha, 2 * (size_t)lz[q]->nl + 2});
+  issue(pre, &hv, post, &eig, &reads);
   ++sweeps;
-  for (BJob* j : eig) {
-    ++j->n_av;
-    if (j->col + 1 < j->m) ++j->col;
+  for (BJob* j : lz) {
+    if (j->step == j->nl) j->awaiting = true;
+    else ++j->step;
   }
-  for (BJob* j : lz)
-    if (j->step != j->nl) ++j->step;
   int active = 0;
   for (auto& jp : jobs)
     if (jp && jp->state == JOB_ACTIVE) ++active;
@@ -1133,96 +1482,6 @@ void Batch::finish_lanczos(BJob& j) {
   release_job(j);  // the coefficients are on the host now
   j.lz[0] = j.lz[1] = j.lz[2] = nullptr;
   j.sc = nullptr;
-}
-
-// End of a restart cycle of one eigensolver run: Ritz values of the
-// projected matrix, convergence test, then a thick restart or the result
-// (the single-sector eigensolver's rules).
-void Batch::end_cycle(BJob& j) {
-  const int m = j.m, nev = j.nev;
-  const int64_t n = j.n;
-  const double* Tcol = j.h_read + (S_T - S_NRM);
-  for (int c = j.k; c < m; ++c)
-    for (int i = 0; i <= c; ++i) {
-      j.T[(size_t)i * m + c] = Tcol[(size_t)c * (m + 1) + i];
-      j.T[(size_t)c * m + i] = Tcol[(size_t)c * (m + 1) + i];
-    }
-  const double bres = j.h_read[0];
-  std::vector<double> A = j.T, theta;
-  sym_eig(m, A, theta);
-  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-  std::vector<int> done_i;
-  for (int i = 0; i < nev; ++i) {
-    const double bound = std::fabs(bres * A[(size_t)(m - 1) * m + i]);
-    if (bound <= j.tol * std::max(std::fabs(theta[i]), eps23)) done_i.push_back(i);
-  }
-  const int nconv_now = (int)done_i.size();
-  const bool conv = nconv_now == nev;
-  ++j.iter;
-  if (conv || j.iter >= j.maxitr) {
-    // after maxitr the converged Ritz pairs are the result (as dseupd with
-    // info = 1, Storage.h:94-129)
-    // the Ritz vectors are combined with the next sweep (stream order puts
-    // every reader of them after it); the bases are freed after that
-    const int nk = nconv_now;
-    j.evals.clear();
-    for (int c = 0; c < nk; ++c) j.evals.push_back(theta[done_i[c]]);
-    if (nk > 0) {
-      CombineReq r;
-      r.X = j.V;
-      r.n = n;
-      r.rows = m;
-      r.cols = nk;
-      r.C.assign((size_t)m * nk, 0.0);
-      for (int c = 0; c < nk; ++c)
-        for (int i = 0; i < m; ++i) r.C[(size_t)c * m + i] = A[(size_t)i * m + done_i[c]];
-      j.evecs = dalloc(j, (size_t)nk * n);
-      r.Y = j.evecs;
-      pending_comb.push_back(std::move(r));
-    }
-    j.nev = nk;
-    j.nconv = nk;
-    j.state = JOB_DONE;
-    for (double* b : {j.V, j.Valt}) {
-      auto it = std::find(j.allocs.begin(), j.allocs.end(), (void*)b);
-      if (it != j.allocs.end()) {
-        const size_t bb = (size_t)(m + 1) * n * sizeof(double);
-        j.allocs.erase(it);
-        j.bytes -= bb;
-        pending_free.push_back(b);
-        pending_free_bytes += bb;
-      }
-    }
-    j.V = j.Valt = nullptr;
-    return;
-  }
-  // thick restart: Valt_c = sum_i A[i][c] V_i (c < kk) and Valt_kk = V_m (the
-  // residual direction), one combination with m + 1 rows and kk + 1 columns,
-  // which also clears the projected-matrix block
-  const int kk = restart_size(nev, m, nconv_now);
-  CombineReq r;
-  r.X = j.V;
-  r.n = n;
-  r.rows = m + 1;
-  r.cols = kk + 1;
-  r.C.assign((size_t)(m + 1) * (kk + 1), 0.0);
-  for (int c = 0; c < kk; ++c)
-    for (int i = 0; i < m; ++i) r.C[(size_t)c * (m + 1) + i] = A[(size_t)i * m + c];
-  r.C[(size_t)kk * (m + 1) + m] = 1.0;
-  r.Y = j.Valt;
-  r.zero = j.sc + S_T;
-  r.nzero = (m + 1) * (m + 1);
-  pending_comb.push_back(std::move(r));
-  std::swap(j.V, j.Valt);
-  std::fill(j.T.begin(), j.T.end(), 0.0);
-  for (int i = 0; i < kk; ++i) {
-    j.T[(size_t)i * m + i] = theta[i];
-    const double si = bres * A[(size_t)(m - 1) * m + i];
-    j.T[(size_t)i * m + kk] = si;
-    j.T[(size_t)kk * m + i] = si;
-  }
-  j.k = kk;
-  j.col = kk;
 }
 
 // Picks the grouped H*v block size for the sectors of the batch by timing

@@ -212,7 +212,20 @@ struct HvArgs {
   double* dot_partial;  // optional: per-block partial sums of v*w (after scale)
   const double* dscale = nullptr;  // optional device scalars {vscale, wscale} overriding the values above
   const int* done = nullptr;       // optional device stop flag: the launch does nothing when set
+  // optional eigensolver control block (grouped H*v only): v and w are then
+  // columns col and col + 1 of the basis held in v (phase 0) or v_alt
+  // (phase 1), and a run whose state is not 0 is skipped
+  const double* ctl = nullptr;
+  const double* v_alt = nullptr;
+  // optional (grouped H*v only): per-task shares of <v, w> (v scaled), at
+  // least hv_batch_tasks(D_up, D_dn) of them
+  double* tdot = nullptr;
 };
+
+// Control block of a batched eigensolver run (doubles, on the device): the
+// basis column being extended, which of the two basis buffers holds the
+// basis, and the run state (0 = active).
+constexpr int kCtlCol = 0, kCtlPhase = 1, kCtlState = 2;
 
 void launch_hv(const DevSector& ds, const HvArgs& a, hipStream_t s, int variant = HV_AUTO);
 int hv_num_blocks(const DevSector& ds);
@@ -228,7 +241,9 @@ struct HvBatchUnit {
 };
 size_t hv_batch_job_bytes();
 void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out);
-int hv_batch_unit_rows(int64_t ddn);  // rows per unit for a down dimension
+int hv_batch_unit_rows(int64_t ddn);  // rows per unit for a down dimension (a multiple of the row group)
+int hv_batch_row_group(int64_t ddn);  // rows per wavefront task
+int64_t hv_batch_tasks(int64_t dup, int64_t ddn);  // wavefront tasks of a whole operand
 void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s);
 
 }  // namespace edh

@@ -14,6 +14,9 @@ void dev_dot(const double* a, const double* b, int64_t n, double* partial, doubl
 void dev_norm(const double* a, int64_t n, double* partial, double* out, hipStream_t s);
 void dev_scale_inv(const double* x, const double* nrm, double* out, int64_t n, hipStream_t s);
 void dev_fill(double* p, int64_t n, double x, hipStream_t s);  // a kernel, never a memset (graph-safe)
+// Random start vector: uniform [-1, 1) from splitmix64 (seed 0 -> a fixed
+// default), the same values as the sequential host generator.
+void dev_random_uniform(double* x, int64_t n, uint64_t seed, hipStream_t s);
 // w <- w - V V^T w (one classical Gram-Schmidt pass over m columns); h = V^T w
 // (device, m), hsum += h when non-null; *nrm = |w| afterwards (device scalar).
 // nb: also *nb = |w| before the pass; gate: the pass does nothing when *gate

@@ -158,17 +158,6 @@ void dense_sector_matrix(const Model& m, const SectorTables& st, std::vector<dou
     }
 }
 
-namespace {
-
-uint64_t splitmix64(uint64_t& x) {
-  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-}  // namespace
-
 // ARPACK's dsaup2 rule: keep the nev wanted Ritz vectors plus one more per
 // converged pair, at most half of the new space (this is what its implicit
 // restart with exact shifts retains).
@@ -238,13 +227,9 @@ void eigsh_run(Ctx& c, int nev_req, int ncv_req, double tol, int maxitr, uint64_
   double* Valt = c.krylov[1].as<double>();
   double* partial = c.partial.as<double>();
 
-  // random start vector, uniform in [-1, 1)
+  // random start vector, uniform in [-1, 1) (splitmix64, generated on the device)
   {
-    std::vector<double> x(n);
-    uint64_t st = seed ? seed : 0x243F6A8885A308D3ull;
-    for (int64_t i = 0; i < n; ++i) x[i] = (double)(splitmix64(st) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-    EDH_HIP(hipMemcpyAsync(V, x.data(), vbytes, hipMemcpyHostToDevice, s));
-    EDH_HIP(hipStreamSynchronize(s));
+    dev_random_uniform(V, n, seed, s);
     dev_norm(V, n, partial, norms + m, s);
     dev_scale_inv(V, norms + m, V, n, s);
   }

@@ -464,7 +464,83 @@ struct HvBatchJob {
   HvParams p;
   int64_t row0;
   int cross, kan;
+  const double* ctl;    // eigensolver control block or null
+  const double* v_alt;  // second basis buffer (with ctl)
+  double* tdot;         // per-task shares of <v, w>, or null
 };
+
+// One element of the grouped H*v: w[u][d] (written to W) from the operand
+// V scaled by vscale; returns v[u][d] * w[u][d] (scaled v).
+__device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const HvParams& p, const double* __restrict__ V,
+                                                   double* __restrict__ W, int u, int d, double vscale, double wscale,
+                                                   int na, unsigned lowmask) {
+  const int Ddn = p.Ddn;
+  const double* __restrict__ vals = p.values;
+  const int cross = J.cross;
+  const int sl = d >> 6;
+  const size_t rowoff = (size_t)u * Ddn;
+
+  const int bu = p.up_bits[u];
+  const int bd = p.dn_bits[d];
+  double x;
+  if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
+  else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
+  else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
+  else x = 0.0;
+  double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
+  acc = fma(p.up_e[u] + p.dn_e[d] + x, V[rowoff + d] * vscale, acc);
+  // down-spin hopping (sliced ELL, slice = d / 64)
+  const int len = p.dn_sllen[sl];
+  const uint32_t* __restrict__ el = p.dn_ell + p.dn_slptr[sl] + (d & 63);
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const uint32_t e0 = el[k * kSlice], e1 = el[(k + 1) * kSlice];
+    const uint32_t e2 = el[(k + 2) * kSlice], e3 = el[(k + 3) * kSlice];
+    const double x0 = V[rowoff + (e0 & kColMask)], x1 = V[rowoff + (e1 & kColMask)];
+    const double x2 = V[rowoff + (e2 & kColMask)], x3 = V[rowoff + (e3 & kColMask)];
+    acc = fma(vals[e0 >> kColBits], x0 * vscale, acc);
+    acc = fma(vals[e1 >> kColBits], x1 * vscale, acc);
+    acc = fma(vals[e2 >> kColBits], x2 * vscale, acc);
+    acc = fma(vals[e3 >> kColBits], x3 * vscale, acc);
+  }
+  for (; k < len; ++k) {
+    const uint32_t e = el[k * kSlice];
+    acc = fma(vals[e >> kColBits], V[rowoff + (e & kColMask)] * vscale, acc);
+  }
+  // up-spin hopping: rows of the same columns (coalesced)
+  const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+  int q = hb;
+  for (; q + 4 <= he; q += 4) {
+    const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
+    const uint32_t e2 = p.up_packed[q + 2], e3 = p.up_packed[q + 3];
+    const double x0 = V[(size_t)(e0 & kColMask) * Ddn + d], x1 = V[(size_t)(e1 & kColMask) * Ddn + d];
+    const double x2 = V[(size_t)(e2 & kColMask) * Ddn + d], x3 = V[(size_t)(e3 & kColMask) * Ddn + d];
+    acc = fma(vals[e0 >> kColBits] * vscale, x0, acc);
+    acc = fma(vals[e1 >> kColBits] * vscale, x1, acc);
+    acc = fma(vals[e2 >> kColBits] * vscale, x2, acc);
+    acc = fma(vals[e3 >> kColBits] * vscale, x3, acc);
+  }
+  for (; q < he; ++q) {
+    const uint32_t e = p.up_packed[q];
+    acc = fma(vals[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
+  }
+  if (J.kan) {
+    const int a = bu >> p.nb, b = bd >> p.nb;
+    const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+    for (int r = kb; r < ke; ++r) {
+      const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[r] << p.nb)];
+      const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[r] << p.nb)];
+      acc = fma(p.kan_c[r], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
+    }
+  }
+  W[rowoff + d] = acc;
+  return V[rowoff + d] * vscale * acc;
+}
+
+__device__ __forceinline__ double wave_sum64(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
 
 // Wavefront-task form of the grouped H*v: no LDS and no block barrier.  A
 // unit's rows are cut into wavefront tasks: 64-column slices of one row when
@@ -472,7 +548,9 @@ struct HvBatchJob {
 // slot count is wavefront-uniform), else R = 64 / Dp whole rows per
 // wavefront (Dp = the down dimension rounded up to a power of two).  The
 // down-spin gathers read the row through the L1/L2 caches, and the loads of
-// four slots are issued before their FMAs.
+// four slots are issued before their FMAs.  Units start at multiples of R
+// rows, so a task's (row group, slice) does not depend on the grouping; a
+// Lanczos run's share of <v, w> is stored there (tdot), one value per task.
 __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __restrict__ jobs,
                                                             const HvBatchUnit* __restrict__ units, int nunits) {
   const int lane = threadIdx.x & 63;
@@ -483,6 +561,16 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
     const HvBatchJob& J = jobs[U.job];
     const HvParams& p = J.p;
     if (p.done && *p.done) continue;  // a stopped Lanczos run (uniform over the block)
+    const double* __restrict__ V = p.v;
+    double* __restrict__ W = p.w;
+    if (J.ctl) {  // eigensolver run: its basis column and buffer
+      if (J.ctl[kCtlState] != 0.0) continue;
+      const int64_t N = (int64_t)p.Dup * p.Ddn;
+      const double* base = J.ctl[kCtlPhase] != 0.0 ? J.v_alt : p.v;
+      const int64_t col = (int64_t)J.ctl[kCtlCol];
+      V = base + col * N;
+      W = const_cast<double*>(base) + (col + 1) * N;
+    }
     const int Ddn = p.Ddn;
     const int lg = Ddn >= 64 ? 6 : (Ddn <= 1 ? 0 : 32 - __clz(Ddn - 1));  // log2 Dp
     const int nsl = (Ddn + 63) >> 6;
@@ -490,71 +578,20 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
     const int ntask = ngr * nsl;
     const double vscale = p.dscale ? p.dscale[0] : p.vscale;
     const double wscale = p.dscale ? p.dscale[1] : p.wscale;
-    const double* __restrict__ V = p.v;
-    const double* __restrict__ vals = p.values;
     const int na = 1 << (p.Ns - p.nb);
     const unsigned lowmask = (1u << p.nb) - 1u;
-    const int cross = J.cross;
     for (int t = wave; t < ntask; t += nw) {
       const int g = t / nsl, sl = t - g * nsl;
       const int u = U.row0 + (g << (6 - lg)) + (lane >> lg);
       const int d = (sl << 6) + (lane & ((1 << lg) - 1));
-      if (u >= U.row0 + U.nrows || d >= Ddn) continue;
-      const size_t rowoff = (size_t)u * Ddn;
-      const int bu = p.up_bits[u];
-      const int bd = p.dn_bits[d];
-      double x;
-      if (cross == CROSS_POPC) x = cross_term<CROSS_POPC>(p, u, bu, bd, na);
-      else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
-      else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
-      else x = 0.0;
-      double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
-      acc = fma(p.up_e[u] + p.dn_e[d] + x, V[rowoff + d] * vscale, acc);
-      // down-spin hopping (sliced ELL, slice = d / 64)
-      const int len = p.dn_sllen[sl];
-      const uint32_t* __restrict__ el = p.dn_ell + p.dn_slptr[sl] + (d & 63);
-      int k = 0;
-      for (; k + 4 <= len; k += 4) {
-        const uint32_t e0 = el[k * kSlice], e1 = el[(k + 1) * kSlice];
-        const uint32_t e2 = el[(k + 2) * kSlice], e3 = el[(k + 3) * kSlice];
-        const double x0 = V[rowoff + (e0 & kColMask)], x1 = V[rowoff + (e1 & kColMask)];
-        const double x2 = V[rowoff + (e2 & kColMask)], x3 = V[rowoff + (e3 & kColMask)];
-        acc = fma(vals[e0 >> kColBits], x0 * vscale, acc);
-        acc = fma(vals[e1 >> kColBits], x1 * vscale, acc);
-        acc = fma(vals[e2 >> kColBits], x2 * vscale, acc);
-        acc = fma(vals[e3 >> kColBits], x3 * vscale, acc);
+      const bool live = u < U.row0 + U.nrows && d < Ddn;
+      const double vw = live ? hv_batch_element(J, p, V, W, u, d, vscale, wscale, na, lowmask) : 0.0;
+      if (J.tdot) {
+        // Lanczos runs: this task's share of <v, w>, one wavefront tree,
+        // at (row group, slice) -- the same place whatever the grouping
+        const double t2 = wave_sum64(vw);
+        if (lane == 0) J.tdot[(size_t)((U.row0 >> (6 - lg)) + g) * nsl + sl] = t2;
       }
-      for (; k < len; ++k) {
-        const uint32_t e = el[k * kSlice];
-        acc = fma(vals[e >> kColBits], V[rowoff + (e & kColMask)] * vscale, acc);
-      }
-      // up-spin hopping: rows of the same columns (coalesced)
-      const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
-      int q = hb;
-      for (; q + 4 <= he; q += 4) {
-        const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
-        const uint32_t e2 = p.up_packed[q + 2], e3 = p.up_packed[q + 3];
-        const double x0 = V[(size_t)(e0 & kColMask) * Ddn + d], x1 = V[(size_t)(e1 & kColMask) * Ddn + d];
-        const double x2 = V[(size_t)(e2 & kColMask) * Ddn + d], x3 = V[(size_t)(e3 & kColMask) * Ddn + d];
-        acc = fma(vals[e0 >> kColBits] * vscale, x0, acc);
-        acc = fma(vals[e1 >> kColBits] * vscale, x1, acc);
-        acc = fma(vals[e2 >> kColBits] * vscale, x2, acc);
-        acc = fma(vals[e3 >> kColBits] * vscale, x3, acc);
-      }
-      for (; q < he; ++q) {
-        const uint32_t e = p.up_packed[q];
-        acc = fma(vals[e >> kColBits] * vscale, V[(size_t)(e & kColMask) * Ddn + d], acc);
-      }
-      if (J.kan) {
-        const int a = bu >> p.nb, b = bd >> p.nb;
-        const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-        for (int r = kb; r < ke; ++r) {
-          const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[r] << p.nb)];
-          const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[r] << p.nb)];
-          acc = fma(p.kan_c[r], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
-        }
-      }
-      p.w[rowoff + d] = acc;
     }
   }
 }
@@ -787,10 +824,29 @@ void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out
   j.row0 = row0;
   j.cross = ds.cross_mode;
   j.kan = ds.kan_ml > 0 ? 1 : 0;
+  j.ctl = a.ctl;
+  j.v_alt = a.v_alt;
+  j.tdot = a.tdot;
   std::memcpy(out, &j, sizeof(j));
 }
 
-int hv_batch_unit_rows(int64_t ddn) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, 4096 / std::max<int64_t>(ddn, 1))); }
+int hv_batch_row_group(int64_t ddn) {
+  if (ddn >= 64) return 1;
+  int dp = 1;
+  while (dp < ddn) dp *= 2;
+  return 64 / dp;
+}
+
+int hv_batch_unit_rows(int64_t ddn) {
+  const int r = (int)std::max<int64_t>(1, std::min<int64_t>(64, 4096 / std::max<int64_t>(ddn, 1)));
+  const int g = hv_batch_row_group(ddn);
+  return (r + g - 1) / g * g;
+}
+
+int64_t hv_batch_tasks(int64_t dup, int64_t ddn) {
+  const int g = hv_batch_row_group(ddn);
+  return (dup + g - 1) / g * ((ddn + 63) / 64);
+}
 
 void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s) {
   if (nunits <= 0) return;

@@ -258,6 +258,27 @@ __global__ void fill_kernel(double* p, int64_t n, double x) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) p[e] = x;
 }
 
+// x[i] = uniform [-1, 1) value number i + 1 of the splitmix64 sequence
+// started at st0 (state st0 + (i + 1) * 0x9E3779B97F4A7C15, then the
+// splitmix64 output mix), top 53 bits scaled by 2^-52, minus 1: element i of
+// the host generator's sequence, produced independently per element.
+__global__ void splitmix_uniform_kernel(double* x, int64_t n, uint64_t st0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = st0 + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+void dev_random_uniform(double* x, int64_t n, uint64_t seed, hipStream_t s) {
+  const uint64_t st0 = seed ? seed : 0x243F6A8885A308D3ull;
+  const int g = (int)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(splitmix_uniform_kernel, dim3(std::max(g, 1)), dim3(256), 0, s, x, n, st0);
+  EDH_HIP(hipGetLastError());
+}
+
 void dev_fill(double* p, int64_t n, double x, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(reduce_grid(n)), dim3(kVecBlock), 0, s, p, n, x);
