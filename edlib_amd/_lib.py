@@ -79,6 +79,7 @@ _SIGS = {
     "edh_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "edh_batch_job_info": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _i64p, _ip, _ip]),
     "edh_batch_job_states": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "edh_batch_eig_bounds": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp]),
     "edh_batch_eig_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
     "edh_batch_lanczos_result": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip]),
     "edh_batch_free_job": (ctypes.c_int, [_vp, ctypes.c_int]),

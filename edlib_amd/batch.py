@@ -98,6 +98,15 @@ class Batch:
         check(_lib.lib().edh_batch_job_states(self._h, len(ids), ip(ids), ip(st)))
         return st[:len(ids)]
 
+    def eig_bounds(self, jids):
+        """Upper bounds of the lowest eigenvalue of eigensolver jobs (NaN
+        before a running job's first restart cycle has ended)."""
+        ids = np.ascontiguousarray(jids, dtype=np.int32)
+        out = np.zeros(max(len(ids), 1))
+        check(_lib.lib().edh_batch_eig_bounds(self._h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                              _d(out)))
+        return out[:len(ids)]
+
     def eig_result(self, jid, vectors=False):
         info = self.job_info(jid)
         nev, n = info["count"], info["n"]

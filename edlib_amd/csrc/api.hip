@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 
 #include "batch.h"
 #include "edh_ctx.h"
@@ -55,7 +56,10 @@ Ctx::~Ctx() {
   evecs.release();
   small.release();
   gtab.release();
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream) {
+    (void)hipStreamSynchronize(stream);  // the tables return to the pool in stream order
+    (void)hipStreamDestroy(stream);
+  }
 }
 
 namespace {
@@ -231,6 +235,7 @@ int edh_create(edh_ctx** out, int device) {
   c->c.device = device;
   EDH_HIP(hipSetDevice(device));
   EDH_HIP(hipStreamCreate(&c->c.stream));
+  edh::configure_mem_pool(device);
   *out = c.release();
   API_END
 }
@@ -704,6 +709,7 @@ int edh_batch_create(edh_batch** out, int device) {
   std::unique_ptr<edh_batch> b(new edh_batch());
   b->b.device = device;
   EDH_HIP(hipStreamCreate(&b->b.stream));
+  edh::configure_mem_pool(device);
   *out = b.release();
   API_END
 }
@@ -807,6 +813,20 @@ int edh_batch_job_states(edh_batch* b, int count, const int* job_ids, int* state
   edh::Batch& B = use_batch(b);
   EDH_CHECK(count >= 0 && (count == 0 || (job_ids && states)), EDH_ERR_ARG, "bad argument");
   for (int q = 0; q < count; ++q) states[q] = B.job(job_ids[q]).state;
+  API_END
+}
+
+int edh_batch_eig_bounds(edh_batch* b, int count, const int* job_ids, double* bounds) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(count >= 0 && (count == 0 || (job_ids && bounds)), EDH_ERR_ARG, "bad argument");
+  for (int q = 0; q < count; ++q) {
+    const edh::BJob& j = B.job(job_ids[q]);
+    EDH_CHECK(j.kind == edh::JOB_EIG, EDH_ERR_ARG, "batch: not an eigensolver job");
+    if (j.state == edh::JOB_DONE && !j.evals.empty()) bounds[q] = j.evals[0];
+    else if (j.state == edh::JOB_ACTIVE && j.iter >= 1) bounds[q] = j.ritz0;
+    else bounds[q] = std::numeric_limits<double>::quiet_NaN();
+  }
   API_END
 }
 

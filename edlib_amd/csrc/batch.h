@@ -36,6 +36,7 @@ struct BJob {
   // read of every sweep)
   int nev = 0, m = 0, iter = 0, maxitr = 0, nconv = 0, n_av = 0;
   double tol = 0.0;
+  double ritz0 = 0.0;  // lowest Ritz value at the last cycle end (valid once iter >= 1)
   bool pending_start_norm = false;
   double *V = nullptr, *Valt = nullptr, *evecs = nullptr, *ctl = nullptr;
   std::vector<double> evals;
@@ -127,10 +128,7 @@ struct Batch {
   DevBuf d_gather;
   std::vector<ReadSlab> slabs;
   std::vector<int> slab_fifo;  // slabs in flight, oldest first
-  // Krylov bases of finished eigensolver runs, freed together (a free
-  // waits for the device)
-  std::vector<void*> pending_free;
-  size_t pending_free_bytes = 0;
+
   size_t bytes_in_use = 0;
   int hv_block = 256;
   long long sweeps = 0;
@@ -154,7 +152,6 @@ struct Batch {
   void release_job(BJob& j);
   bool drain_slab(bool wait);
   void eig_status(BJob& j, const double* h);
-  void free_bases(bool all);
   void finish_dense(BJob& j);
   void finish_lanczos(BJob& j);
   void issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, HvArgs>>* hv, std::vector<OpGroup>& post,

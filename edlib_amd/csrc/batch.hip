@@ -63,6 +63,7 @@ constexpr int C_PENDING = 7;  // 1: thick restart, 2: Ritz vectors to combine th
 constexpr int C_SRC = 8;      // basis buffer the combination reads
 constexpr int C_ROWS = 9;     // rows and columns of the combination matrix
 constexpr int C_COLS = 10;
+constexpr int C_RITZ0 = 11;   // lowest Ritz value at the last cycle end (an upper bound of the lowest eigenvalue)
 constexpr int C_HDR = 16;     // header size; Ritz values follow
 __host__ __device__ inline int ctl_theta(int m) { return C_HDR + m; }
 __host__ __device__ inline int ctl_cmat(int m) { return C_HDR + 2 * m; }
@@ -591,6 +592,7 @@ __global__ __launch_bounds__(64) void bcycle_kernel(const BCyc* __restrict__ ops
   __syncthreads();
   const int nc = s_nconv;
   const int iter = (int)ctl[C_ITER] + 1;
+  if (lane == 0) ctl[C_RITZ0] = w[0];
   double* C = ctl + ctl_cmat(m);
   if (nc == o.nev || iter >= o.maxitr) {
     // the converged Ritz pairs are the result (after maxitr: dseupd with info = 1)
@@ -708,15 +710,15 @@ Batch::~Batch() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& j : jobs)
     if (j) release_job(*j);
-  for (void* p : pending_free) (void)hipFree(p);
   for (ReadSlab& r : slabs) {
     if (r.host) (void)hipHostFree(r.host);
     if (r.ev) (void)hipEventDestroy(r.ev);
   }
   for (auto& kv : maps)
-    for (void* p : kv.second) (void)hipFree(p);
+    for (void* p : kv.second) (void)hipFreeAsync(p, stream);
   for (auto& s : secs)
     if (s) s->dev.release();
+  if (stream) (void)hipStreamSynchronize(stream);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms, &d_gather}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);
@@ -728,8 +730,10 @@ Batch::~Batch() {
 double* Batch::dalloc(BJob& j, size_t count) {
   void* p = nullptr;
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(double);
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("batch: hipMalloc failed: ") + hipGetErrorString(e));
+  // stream-ordered: every reader and writer of a job's memory runs on the
+  // batch stream, so the memory can return to the pool in stream order
+  hipError_t e = hipMallocAsync(&p, bytes, stream);
+  if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("batch: hipMallocAsync failed: ") + hipGetErrorString(e));
   j.allocs.push_back(p);
   j.bytes += bytes;
   bytes_in_use += bytes;
@@ -737,21 +741,11 @@ double* Batch::dalloc(BJob& j, size_t count) {
 }
 
 void Batch::release_job(BJob& j) {
-  for (void* p : j.allocs) (void)hipFree(p);
+  for (void* p : j.allocs) (void)hipFreeAsync(p, stream);
   j.allocs.clear();
   bytes_in_use -= j.bytes;
   j.bytes = 0;
   j.h_read = nullptr;
-}
-
-// Frees the bases of finished eigensolver runs, several at a time (all of
-// them when `all`).
-void Batch::free_bases(bool all) {
-  if (pending_free.empty() || (!all && pending_free.size() < 16)) return;
-  for (void* p : pending_free) (void)hipFree(p);
-  pending_free.clear();
-  bytes_in_use -= pending_free_bytes;
-  pending_free_bytes = 0;
 }
 
 // An eigensolver run's status as read back after a sweep: header of its
@@ -761,6 +755,7 @@ void Batch::eig_status(BJob& j, const double* h) {
   j.n_av = (int)h[C_NAV];
   j.iter = (int)h[C_ITER];
   j.nconv = (int)h[C_NCONV];
+  if (j.iter >= 1) j.ritz0 = h[C_RITZ0];
   const double st = h[kCtlState];
   if (st == 0.0) return;
   if (st == 1.0) {
@@ -771,15 +766,16 @@ void Batch::eig_status(BJob& j, const double* h) {
   } else {
     j.state = JOB_FAILED;  // the projected matrix's QL iteration did not converge
   }
-  // the bases are no longer read (the Ritz vectors were combined before this status was)
+  // the bases are no longer read (the Ritz vectors were combined before this
+  // status was; sweeps issued since skip the run before touching them)
   for (double* b : {j.V, j.Valt}) {
     auto it = std::find(j.allocs.begin(), j.allocs.end(), (void*)b);
     if (it != j.allocs.end()) {
       const size_t bb = (size_t)(j.m + 1) * j.n * sizeof(double);
       j.allocs.erase(it);
       j.bytes -= bb;
-      pending_free.push_back(b);
-      pending_free_bytes += bb;
+      bytes_in_use -= bb;
+      (void)hipFreeAsync(b, stream);
     }
   }
   j.V = j.Valt = nullptr;
@@ -836,10 +832,11 @@ void Batch::remove_sector(int id) {
   BSector& s = sector(id);
   for (auto& j : jobs)
     EDH_CHECK(!j || j->sec != id || j->state != JOB_ACTIVE, EDH_ERR_STATE, "batch: sector has an active job");
-  EDH_HIP(hipStreamSynchronize(stream));
+  // no wait: the tables and maps return to the pool after the stream's
+  // earlier work (every reader of them) has run
   for (auto it = maps.begin(); it != maps.end();) {
     if (std::get<0>(it->first) == id) {
-      for (void* p : it->second) (void)hipFree(p);
+      for (void* p : it->second) (void)hipFreeAsync(p, stream);
       it = maps.erase(it);
     } else {
       ++it;
@@ -980,7 +977,7 @@ void Batch::gather_maps(int src, int orb, bool annihilate, std::vector<void*>& o
   const size_t sz[4] = {mu.size() * 4, md.size() * 4, su.size(), sd.size()};
   const void* src_p[4] = {mu.data(), md.data(), su.data(), sd.data()};
   for (int q = 0; q < 4; ++q) {
-    EDH_HIP(hipMalloc(&p[q], std::max<size_t>(sz[q], 8)));
+    EDH_HIP(hipMallocAsync(&p[q], std::max<size_t>(sz[q], 8), stream));
     if (sz[q]) EDH_HIP(hipMemcpyAsync(p[q], src_p[q], sz[q], hipMemcpyHostToDevice, stream));
   }
   EDH_HIP(hipStreamSynchronize(stream));
@@ -1062,7 +1059,6 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
 void Batch::free_job(int id) {
   BJob& j = job(id);
   EDH_CHECK(j.state != JOB_ACTIVE, EDH_ERR_STATE, "batch: job still active");
-  if (!j.allocs.empty()) EDH_HIP(hipStreamSynchronize(stream));
   release_job(j);
   jobs[id].reset();
 }
@@ -1342,7 +1338,6 @@ int Batch::step() {
       lz_id.push_back((int)id);
     }
   }
-  free_bases(eig.empty());
   if (eig.empty() && lz.empty()) {
     int active = 0;
     for (auto& jp : jobs)

@@ -196,10 +196,16 @@ struct DevSector {
   int* counters = nullptr;       // work queues of the slab pass (8 ints)
   int64_t table_bytes = 0;
   std::vector<void*> allocs;
+  hipStream_t astream = nullptr;  // stream the tables were allocated on (stream-ordered pool)
   void release();
 };
 
 void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s);
+
+// Keeps freed stream-ordered allocations in the device's default pool (no
+// return to the driver at synchronisations), so that allocation in the
+// solve loops costs no driver call and no device-wide wait.
+void configure_mem_pool(int device);
 
 // H*v launch parameters.
 struct HvArgs {

@@ -1,5 +1,8 @@
 // Upload of one sector's tables to the device.
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <functional>
 
 #include "edh_internal.h"
 
@@ -7,34 +10,54 @@ namespace edh {
 
 namespace {
 
-template <class T>
-T* upload(DevSector& ds, const std::vector<T>& h, hipStream_t s) {
-  if (h.empty()) return nullptr;
-  void* p = nullptr;
-  hipError_t e = hipMalloc(&p, h.size() * sizeof(T));
-  if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-  ds.allocs.push_back(p);
-  EDH_HIP(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return (T*)p;
-}
+// All tables of a sector go to the device as one allocation and one copy:
+// the host arrays are first packed (256-byte aligned) into one blob, and
+// the device pointers are set once the blob has its device address.
+class Packer {
+ public:
+  template <class T>
+  void add(T*& field, const std::vector<T>& h) {
+    field = nullptr;
+    if (h.empty()) return;
+    const size_t off = (blob_.size() + 255) & ~size_t(255);
+    blob_.resize(off + h.size() * sizeof(T));
+    std::memcpy(blob_.data() + off, h.data(), h.size() * sizeof(T));
+    set_.push_back([&field, off](char* base) { field = reinterpret_cast<T*>(base + off); });
+  }
+  // The blob must stay alive until the copy has completed (the caller
+  // synchronises the stream).
+  void commit(DevSector& ds, hipStream_t s) {
+    if (blob_.empty()) return;
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, blob_.size(), s);
+    if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("hipMallocAsync failed: ") + hipGetErrorString(e));
+    ds.allocs.push_back(p);
+    EDH_HIP(hipMemcpyAsync(p, blob_.data(), blob_.size(), hipMemcpyHostToDevice, s));
+    for (auto& f : set_) f(static_cast<char*>(p));
+  }
 
-void upload_spin(DevSector& ds, const SpinTables& sp, DevSpin& d, hipStream_t s) {
+ private:
+  std::vector<char> blob_;
+  std::vector<std::function<void(char*)>> set_;
+};
+
+void upload_spin(Packer& pk, const SpinTables& sp, DevSpin& d) {
   d.D = sp.D;
-  d.bits = upload(ds, sp.bits, s);
-  d.ediag = upload(ds, sp.ediag, s);
-  d.rowp = upload(ds, sp.rowp, s);
-  d.packed = upload(ds, sp.packed, s);
-  d.sl_ptr = upload(ds, sp.sl_ptr, s);
-  d.sl_len = upload(ds, sp.sl_len, s);
-  d.ell = upload(ds, sp.ell, s);
-  d.ell16 = upload(ds, sp.ell16, s);
+  pk.add(d.bits, sp.bits);
+  pk.add(d.ediag, sp.ediag);
+  pk.add(d.rowp, sp.rowp);
+  pk.add(d.packed, sp.packed);
+  pk.add(d.sl_ptr, sp.sl_ptr);
+  pk.add(d.sl_len, sp.sl_len);
+  pk.add(d.ell, sp.ell);
+  pk.add(d.ell16, sp.ell16);
   for (int q = 0; q < 4; ++q) d.vt16[q] = sp.vt16[q];
   d.nvt16 = sp.nvt16;
-  d.q4_ptr = upload(ds, sp.q4_ptr, s);
-  d.ell_q4 = upload(ds, sp.ell_q4, s);
-  d.ell_s4 = upload(ds, sp.ell_s4, s);
+  pk.add(d.q4_ptr, sp.q4_ptr);
+  pk.add(d.ell_q4, sp.ell_q4);
+  pk.add(d.ell_s4, sp.ell_s4);
   d.s4_scale = sp.s4_scale;
-  d.rank = upload(ds, sp.rank, s);
+  pk.add(d.rank, sp.rank);
 }
 
 // Reverse Cuthill-McKee order of the graph whose adjacency is the CSR
@@ -74,43 +97,58 @@ std::vector<int32_t> rcm_order(const std::vector<int32_t>& rowp, const std::vect
 }  // namespace
 
 void DevSector::release() {
-  for (void* p : allocs) (void)hipFree(p);
+  for (void* p : allocs) (void)hipFreeAsync(p, astream);
   allocs.clear();
   *this = DevSector();
 }
 
+void configure_mem_pool(int device) {
+  static bool done[64] = {false};
+  if (device < 0 || device >= 64 || done[device]) return;
+  hipMemPool_t pool;
+  EDH_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t keep = UINT64_MAX;
+  EDH_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  done[device] = true;
+}
+
 void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s) {
   ds.release();
+  ds.astream = s;
   ds.Ns = Ns;
   ds.nup = st.nup;
   ds.ndn = st.ndn;
   ds.N = st.N;
   ds.imp_ml = st.imp_ml;
   ds.table_bytes = st.table_bytes;
-  upload_spin(ds, st.up, ds.up, s);
-  upload_spin(ds, st.dn, ds.dn, s);
-  ds.values = upload(ds, st.values, s);
+  Packer pk;
+  upload_spin(pk, st.up, ds.up);
+  upload_spin(pk, st.dn, ds.dn);
+  pk.add(ds.values, st.values);
   ds.cross_mode = st.cross_mode;
   ds.cross_u = st.cross_u;
-  ds.cross_imp = upload(ds, st.cross_imp, s);
-  ds.cross_rowtab = upload(ds, st.cross_rowtab, s);
+  pk.add(ds.cross_imp, st.cross_imp);
+  pk.add(ds.cross_rowtab, st.cross_rowtab);
   ds.kan_ml = st.kan_ml;
+  std::vector<int32_t> ka, kb;
+  std::vector<double> kc;
   if (st.kan_ml > 0) {
-    std::vector<int32_t> a(st.kan.size()), b(st.kan.size());
-    std::vector<double> c(st.kan.size());
+    ka.resize(st.kan.size());
+    kb.resize(st.kan.size());
+    kc.resize(st.kan.size());
     for (size_t q = 0; q < st.kan.size(); ++q) {
-      a[q] = st.kan[q].new_a;
-      b[q] = st.kan[q].new_b;
-      c[q] = st.kan[q].coef;
+      ka[q] = st.kan[q].new_a;
+      kb[q] = st.kan[q].new_b;
+      kc[q] = st.kan[q].coef;
     }
-    ds.kan_ptr = upload(ds, st.kan_ptr, s);
-    ds.kan_a = upload(ds, a, s);
-    ds.kan_b = upload(ds, b, s);
-    ds.kan_c = upload(ds, c, s);
+    pk.add(ds.kan_ptr, st.kan_ptr);
+    pk.add(ds.kan_a, ka);
+    pk.add(ds.kan_b, kb);
+    pk.add(ds.kan_c, kc);
   }
-  ds.counters = upload(ds, std::vector<int32_t>(64, 0), s);
-  std::vector<int32_t> order = rcm_order(st.up.rowp, st.up.col, st.up.D);
-  ds.row_order = upload(ds, order, s);
+  pk.add(ds.counters, std::vector<int32_t>(64, 0));
+  pk.add(ds.row_order, rcm_order(st.up.rowp, st.up.col, st.up.D));
+  pk.commit(ds, s);
   EDH_HIP(hipStreamSynchronize(s));
   ds.variant = hv_autotune(ds, s);
 }

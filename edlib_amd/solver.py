@@ -149,11 +149,14 @@ class BatchedSolve:
 
     Green's-function runs start while slower sectors are still converging:
     once a sector's eigenpairs are known, every pair within the Boltzmann
-    window of the lowest energy known so far gets its start vectors.  The
-    lowest energy can only fall, so every pair that passes the final cutoff
-    has been started; runs of pairs that fall out of the window are
-    cancelled.  Contributions are summed in the pair order of the final
-    eigenpair list, so the result does not depend on when sectors finished.
+    window of the current bound on the ground energy gets its start vectors.
+    The bound is the lowest of the finished sectors' eigenvalues and of the
+    running sectors' lowest Ritz values (each an upper bound of its sector's
+    lowest eigenvalue), so it can only fall towards the ground energy: every
+    pair that passes the final cutoff has been started, and few pairs are
+    started that later fall out of the window (their runs are cancelled).
+    Contributions are summed in the pair order of the final eigenpair list,
+    so the result does not depend on when sectors finished.
     """
 
     def __init__(self, batch, model, p):
@@ -199,6 +202,12 @@ class BatchedSolve:
         if not self.known:
             return
         e0 = min(q[0] for q in self.known)
+        running = [j for _, j in self.eig_jobs if j not in self.seen]
+        if running:
+            bounds = self.b.eig_bounds(running)
+            bounds = bounds[~np.isnan(bounds)]
+            if bounds.size:
+                e0 = min(e0, float(bounds.min()))
         reqs, keys = [], []
         for e, _, sec, j, k in self.known:
             if e - e0 > self.window:

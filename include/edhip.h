@@ -216,6 +216,12 @@ int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
 int edh_batch_job_info(edh_batch* b, int job_id, int* state, int* kind, int64_t* n, int* count, int* n_av);
 /* states[q] = EDH_JOB_* of job_ids[q], q < count. */
 int edh_batch_job_states(edh_batch* b, int count, const int* job_ids, int* states);
+/* Upper bounds of the lowest eigenvalue of eigensolver jobs: the lowest
+ * eigenvalue of a finished job, the lowest Ritz value of its last restart
+ * cycle for a running one, NaN before its first cycle ends.  (No
+ * counterpart in the reference; used to start Green's-function runs early
+ * only for pairs that can pass the Boltzmann cutoff.) */
+int edh_batch_eig_bounds(edh_batch* b, int count, const int* job_ids, double* bounds);
 /* evals (count of edh_batch_job_info, ascending); evecs (host, n*count
  * column-major) may be NULL.  After maxitr restarts only the converged pairs
  * are returned (nconv < the requested nev), as Storage::diag does. */
