@@ -124,7 +124,10 @@ struct Batch {
   hipEvent_t ev_stage[2] = {nullptr, nullptr};
   bool ev_used[2] = {false, false};
   int stage_i = 0;
-  std::map<std::tuple<int, int, int>, std::vector<void*>> maps;  // (sector, orbital, annihilate) -> gather maps
+  // (sector, orbital, annihilate) -> device gather maps (four arrays, then
+  // their one allocation) and the host blob they were copied from
+  std::map<std::tuple<int, int, int>, std::vector<void*>> maps;
+  std::map<std::tuple<int, int, int>, std::vector<char>> map_blobs;
   DevBuf d_gather;
   std::vector<ReadSlab> slabs;
   std::vector<int> slab_fifo;  // slabs in flight, oldest first

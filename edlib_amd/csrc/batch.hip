@@ -538,13 +538,11 @@ struct BCyc {
   double* ctl;
   int m, nev, maxitr;
   double tol;
+  double* status;  // where this sweep's status (header and Ritz values) is gathered
 };
 
-__global__ __launch_bounds__(64) void bcycle_kernel(const BCyc* __restrict__ ops) {
-  extern __shared__ double s_e[];
-  __shared__ int s_done[kMaxBasis];
-  __shared__ int s_nconv;
-  const BCyc o = ops[blockIdx.x];
+__device__ void cycle_end(const BCyc& o, double* s_e, int* s_done, int* s_nconv_p) {
+  int& s_nconv = *s_nconv_p;
   double* ctl = o.ctl;
   const int lane = threadIdx.x;
   const int m = o.m;
@@ -633,6 +631,17 @@ __global__ __launch_bounds__(64) void bcycle_kernel(const BCyc* __restrict__ ops
   }
 }
 
+__global__ __launch_bounds__(64) void bcycle_kernel(const BCyc* __restrict__ ops) {
+  extern __shared__ double s_e[];
+  __shared__ int s_done[kMaxBasis];
+  __shared__ int s_nconv;
+  const BCyc o = ops[blockIdx.x];
+  cycle_end(o, s_e, s_done, &s_nconv);
+  __syncthreads();
+  // the status read of this sweep: header and Ritz values
+  for (int i = threadIdx.x; i < C_HDR + o.nev; i += 64) o.status[i] = o.ctl[i];
+}
+
 // Grouped combination planned by the cycle ends of this sweep (chunks of
 // runs with nothing pending return at once): Y_j[e] = sum_i C[j rows + i]
 // X_i[e], X = the basis buffer read, Y = the other buffer (restart) or the
@@ -714,8 +723,7 @@ Batch::~Batch() {
     if (r.host) (void)hipHostFree(r.host);
     if (r.ev) (void)hipEventDestroy(r.ev);
   }
-  for (auto& kv : maps)
-    for (void* p : kv.second) (void)hipFreeAsync(p, stream);
+  for (auto& kv : maps) (void)hipFreeAsync(kv.second[4], stream);
   for (auto& s : secs)
     if (s) s->dev.release();
   if (stream) (void)hipStreamSynchronize(stream);
@@ -836,7 +844,8 @@ void Batch::remove_sector(int id) {
   // earlier work (every reader of them) has run
   for (auto it = maps.begin(); it != maps.end();) {
     if (std::get<0>(it->first) == id) {
-      for (void* p : it->second) (void)hipFreeAsync(p, stream);
+      (void)hipFreeAsync(it->second[4], stream);
+      map_blobs.erase(it->first);  // its copy preceded every reader the host has seen finish
       it = maps.erase(it);
     } else {
       ++it;
@@ -973,14 +982,25 @@ void Batch::gather_maps(int src, int orb, bool annihilate, std::vector<void*>& o
   std::vector<int32_t> mu, md;
   std::vector<int8_t> su, sd;
   adag_gather_maps(Ns, a.tabs.nup, a.tabs.ndn, orb, annihilate, nup2, ndn2, mu, md, su, sd);
-  std::vector<void*> p(4, nullptr);
+  // the four maps in one allocation and one copy from a host blob that the
+  // batch keeps (so nothing waits for the copy)
   const size_t sz[4] = {mu.size() * 4, md.size() * 4, su.size(), sd.size()};
   const void* src_p[4] = {mu.data(), md.data(), su.data(), sd.data()};
+  size_t off[4], tot = 0;
   for (int q = 0; q < 4; ++q) {
-    EDH_HIP(hipMallocAsync(&p[q], std::max<size_t>(sz[q], 8), stream));
-    if (sz[q]) EDH_HIP(hipMemcpyAsync(p[q], src_p[q], sz[q], hipMemcpyHostToDevice, stream));
+    off[q] = tot;
+    tot = align16(tot + std::max<size_t>(sz[q], 8));
   }
-  EDH_HIP(hipStreamSynchronize(stream));
+  std::vector<char>& blob = map_blobs[key];
+  blob.assign(tot, 0);
+  for (int q = 0; q < 4; ++q)
+    if (sz[q]) std::memcpy(blob.data() + off[q], src_p[q], sz[q]);
+  void* base = nullptr;
+  EDH_HIP(hipMallocAsync(&base, tot, stream));
+  EDH_HIP(hipMemcpyAsync(base, blob.data(), tot, hipMemcpyHostToDevice, stream));
+  std::vector<void*> p(5, nullptr);
+  for (int q = 0; q < 4; ++q) p[q] = static_cast<char*>(base) + off[q];
+  p[4] = base;  // the allocation
   maps[key] = p;
   out = p;
 }
@@ -1138,6 +1158,12 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   size_t read_total = 0;
   for (int q = 0; q < nread; ++q) read_total += (*reads)[q].count;
   if (off == 0) return;
+  // the first ncyc reads are the eigensolver runs' statuses, written into
+  // the gather buffer by the cycle-end kernel; the gather kernel copies the rest
+  EDH_CHECK(nread >= ncyc, EDH_ERR_INTERNAL, "batch: status reads missing");
+  for (int q = 0; q < ncyc; ++q)
+    EDH_CHECK((*reads)[q].src == (*cycles)[q]->ctl, EDH_ERR_INTERNAL, "batch: status read out of order");
+  if (nread) d_gather.ensure(read_total * sizeof(double));
   // pinned staging, double-buffered: a buffer is rewritten only after the
   // copy that last read it has completed
   const int si = stage_i;
@@ -1199,10 +1225,12 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   fill(post, pl_post);
   {
     int c0 = 0;
+    size_t soff = 0;
     int* cj = (int*)(hb + comb_cj_off);
     for (int q = 0; q < ncyc; ++q) {
       const BJob& j = *(*cycles)[q];
-      BCyc y{j.sc, j.ctl, j.m, j.nev, j.maxitr, j.tol};
+      BCyc y{j.sc, j.ctl, j.m, j.nev, j.maxitr, j.tol, d_gather.as<double>() + soff};
+      soff += (*reads)[q].count;
       std::memcpy(hb + cyc_off + q * sizeof(BCyc), &y, sizeof(BCyc));
       BComb b;
       b.X0 = j.V;
@@ -1220,8 +1248,10 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     }
     int64_t roff = 0;
     for (int q = 0; q < nread; ++q) {
-      BGather g{(*reads)[q].src, roff, (int)(*reads)[q].count};
-      std::memcpy(hb + gat_off + q * sizeof(BGather), &g, sizeof(BGather));
+      if (q >= ncyc) {
+        BGather g{(*reads)[q].src, roff, (int)(*reads)[q].count};
+        std::memcpy(hb + gat_off + (q - ncyc) * sizeof(BGather), &g, sizeof(BGather));
+      }
       roff += (int64_t)(*reads)[q].count;
     }
   }
@@ -1277,9 +1307,9 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
                        (const int*)(db + comb_cj_off));
   }
   if (nread) {
-    d_gather.ensure(read_total * sizeof(double));
-    hipLaunchKernelGGL(bgather_kernel, dim3(nread), dim3(256), 0, stream, (const BGather*)(db + gat_off),
-                       d_gather.as<double>());
+    if (nread > ncyc)
+      hipLaunchKernelGGL(bgather_kernel, dim3(nread - ncyc), dim3(256), 0, stream, (const BGather*)(db + gat_off),
+                         d_gather.as<double>());
     // a free slab large enough, else a new one
     int si2 = -1;
     for (size_t q = 0; q < slabs.size(); ++q)
