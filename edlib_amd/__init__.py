@@ -214,6 +214,22 @@ class Context:
                                 self.n, 1 if clear else 0, EDH_MEM_HOST))
         return w
 
+    def static_observables(self, v, norb):
+        """StaticObservables' per-eigenvector sums (StaticObservables.h:356-419)
+        for a vector of the current sector: host numpy array or a device
+        buffer.  Returns N, N_up, N_dn, M, D_occ, "M_i M_j", N_eff."""
+        out = np.zeros(5 * norb + norb * norb + 1)
+        if isinstance(v, DeviceBuffer):
+            ptr, mem = v.ptr, EDH_MEM_DEVICE
+        else:
+            v = _f64(v).reshape(-1)
+            ptr, mem = v.ctypes.data_as(ctypes.c_void_p), EDH_MEM_HOST
+        check(_lib.lib().edh_static_observables(self._h, ptr, self.n, mem, norb,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        k = norb
+        return {"N": out[0:k], "N_up": out[k:2 * k], "N_dn": out[2 * k:3 * k], "M": out[3 * k:4 * k],
+                "D_occ": out[4 * k:5 * k], "M_i M_j": out[5 * k:5 * k + k * k], "N_eff": out[5 * k + k * k:]}
+
     def av_device(self, vbuf, wbuf, clear=True):
         check(_lib.lib().edh_av(self._h, vbuf.ptr, wbuf.ptr, self.n, 1 if clear else 0, EDH_MEM_DEVICE))
 

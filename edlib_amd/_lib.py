@@ -54,6 +54,7 @@ _SIGS = {
     "edh_memcpy": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     "edh_synchronize": (ctypes.c_int, [_vp]),
     "edh_av": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
+    "edh_static_observables": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
     "edh_get_hv_variant": (ctypes.c_int, [_vp, _ip]),
     "edh_set_hv_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edh_av_time": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),

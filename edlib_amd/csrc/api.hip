@@ -56,6 +56,7 @@ Ctx::~Ctx() {
   evecs.release();
   small.release();
   gtab.release();
+  obs.release();
   if (stream) {
     (void)hipStreamSynchronize(stream);  // the tables return to the pool in stream order
     (void)hipStreamDestroy(stream);
@@ -332,6 +333,19 @@ int edh_synchronize(edh_ctx* ctx) {
   require_ctx(ctx);
   use_device(ctx->c);
   EDH_HIP(hipStreamSynchronize(ctx->c.stream));
+  API_END
+}
+
+int edh_static_observables(edh_ctx* ctx, const double* v, int64_t n, int mem, int norb, double* out) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  require_sector(c);
+  use_device(c);
+  EDH_CHECK(n == c.tabs.N, EDH_ERR_ARG, "vector size does not match the sector dimension");
+  EDH_CHECK(v && out, EDH_ERR_ARG, "null argument");
+  const double* dv = stage_in(c, c.stage_a, v, n, mem);
+  static_obs_run(c, dv, norb, out);
   API_END
 }
 

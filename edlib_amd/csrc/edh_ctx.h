@@ -46,6 +46,7 @@ struct Ctx {
   DevBuf evecs;             // eigenvectors of the last eigsh (resident)
   DevBuf small;             // small device arrays (coefficients, tables)
   DevBuf gtab;              // gather tables for a_adag / operators
+  DevBuf obs;               // static-observable row sums and column tiles
   int n_evecs = 0;
   ~Ctx();
 };
@@ -56,6 +57,11 @@ int restart_size(int nev, int m, int nconv);
 
 void eigsh_run(Ctx& c, int nev, int ncv, double tol, int maxitr, uint64_t seed, double* evals, double* evecs,
                int* nconv, int* n_av);
+
+// Static observables of one device eigenvector of the current sector
+// (StaticObservables.h:356-419): out = N, N_up, N_dn, M, D_occ (norb each),
+// M_i M_j (norb x norb, row-major), N_eff.
+void static_obs_run(Ctx& c, const double* dv, int norb, double* out);
 
 // Diagonal element of state (u, d) of the current sector from the host tables.
 double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d);

@@ -116,6 +116,16 @@ int edh_synchronize(edh_ctx* ctx);
  * w = H v + (clear ? 0 : w).  With EDH_MEM_HOST the call is synchronous. */
 int edh_av(edh_ctx* ctx, const double* v, double* w, int64_t n, int clear, int mem);
 
+/* StaticObservables::calculate_static_observables_eigenvector
+ * (StaticObservables.h:356-419) for one eigenvector v (n = sector dimension)
+ * of the current sector, with norb interacting orbitals (sites 0..norb-1):
+ * out = N[norb], N_up[norb], N_dn[norb], M[norb], D_occ[norb],
+ * M_i M_j[norb*norb] (row-major, the reference's orbital-index rule for the
+ * second down occupation), N_eff[1] -- 5*norb + norb*norb + 1 doubles.  The
+ * caller averages over eigenpairs with Boltzmann weights
+ * (StaticObservables.h:106-158).  At most 32 sites per spin. */
+int edh_static_observables(edh_ctx* ctx, const double* v, int64_t n, int mem, int norb, double* out);
+
 /* H*v kernel variant of the current sector: 0 = single pass (rows), 1 = two
  * passes (rows, then XCD-aware column slabs for the up-spin term).  The
  * sector set-up times both and keeps the faster; edh_set_hv_variant

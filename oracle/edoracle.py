@@ -577,13 +577,19 @@ def static_observables(model, pairs, params):
         bf = math.exp(-(pair.eigenvalue - e0) * beta)
         if not _passes_cutoff(bf, cutoff):
             continue
-        nu, nd = occupations(model, pair.sector, model.Ns)
-        nu = nu[:, :norb].astype(np.float64)
-        ndd = nd[:, :norb].astype(np.float64)
+        nu_all, nd_all = occupations(model, pair.sector, model.Ns)
+        nu = nu_all[:, :norb].astype(np.float64)
+        ndd = nd_all[:, :norb].astype(np.float64)
         wgt = pair.eigenvector ** 2
         m = nu - ndd
         c = dict(N=wgt @ (nu + ndd), N_up=wgt @ nu, N_dn=wgt @ ndd, M=wgt @ m, D_occ=wgt @ (nu * ndd))
-        c["M_i M_j"] = np.einsum("s,si,sj->ij", wgt, m, m).reshape(-1)
+        # StaticObservables.h:378-381: the second factor's down occupation is
+        # read at orbital orb2 + interacting_orbitals() (orbital o < Ns is the
+        # up spin of site o, o >= Ns the down spin of site o - Ns), i.e. the
+        # down spin of orb2 only when every orbital interacts
+        occ = np.concatenate([nu_all, nd_all], axis=1).astype(np.float64)
+        m2 = nu - occ[:, norb:2 * norb]
+        c["M_i M_j"] = np.einsum("s,si,sj->ij", wgt, m, m2).reshape(-1)
         c["N_eff"] = np.array([1.0 / np.sum(wgt * wgt)])
         c["E"] = np.array([pair.eigenvalue])
         for k in avg:

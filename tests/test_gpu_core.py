@@ -267,6 +267,43 @@ def test_gf_chi_golden_via_device(ctx):
         assert np.max(np.abs(G[:, s] - (ref[:, 1] + 1j * ref[:, 2]))) < 1e-10
 
 
+@pytest.mark.parametrize("name,kind,p", CASES, ids=[c[0] for c in CASES])
+def test_static_observables_match_oracle(ctx, name, kind, p):
+    """Per-eigenvector sums of StaticObservables.h:356-419 (including its
+    orbital rule for the second down occupation) from device vectors."""
+    om, pm = models.oracle_model(kind, p), models.product_model(kind, p)
+    ctx.set_model(pm)
+    Ns, norb = p["Ns"], om.interacting_orbitals()
+    rng = np.random.default_rng(13)
+    for sec in [(Ns // 2, Ns // 2), (Ns // 2 + 1, Ns // 2 - 1), (0, Ns), (2, 1)]:
+        n = ctx.set_sector(*sec)
+        v = rng.uniform(-1, 1, n)
+        v /= np.linalg.norm(v)
+        ref = eo.static_observables(om, [eo.EigenPair(0.0, v, 0, sec)], {"lanc.BETA": 1.0})
+        buf = ctx.buffer(n)
+        buf.upload(v)
+        for got in (ctx.static_observables(v, norb), ctx.static_observables(buf, norb)):
+            for key in ("N", "N_up", "N_dn", "M", "D_occ", "M_i M_j"):
+                assert np.abs(got[key] - ref[key]).max() < 1e-12, (key, sec)
+            assert abs(got["N_eff"][0] - ref["N_eff"][0]) < 1e-9 * ref["N_eff"][0]
+        buf.free()
+
+
+def test_static_observables_4ring_paramagnetic(ctx):
+    """StaticObservables_Test.cpp:63-66 from the device ground state."""
+    d = load_input("4ring_nomagfield_input.json")
+    Ns, t, U, xmu, H, J = hubbard_arrays(d)
+    ctx.set_model(ea.HubbardModel(Ns, t, U, xmu, H, J))
+    n = ctx.set_sector(2, 2)
+    evals, evecs, _ = ctx.eigsh(1)
+    obs = ctx.static_observables(evecs[:, 0], Ns)
+    for o in range(Ns):
+        assert abs(obs["N"][o] - 1.0) < 1e-8
+        assert abs(obs["N_up"][o] - 0.5) < 1e-8
+        assert abs(obs["N_dn"][o] - 0.5) < 1e-8
+        assert abs(obs["M"][o]) < 1e-7
+
+
 def test_lanczos_graph_replay_matches_eager():
     """The Lanczos step replayed as hipGraphs gives the directly launched
     recurrence's coefficients, call after call with eager work in between."""

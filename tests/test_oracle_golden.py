@@ -152,3 +152,39 @@ def test_siam_kanamori_term_count():
     m = _random_siam(6, 3, rng)
     ints, vals = m.transitions(1)
     assert len(vals) == 36
+
+
+def _literal_observables(m, sector, v):
+    """StaticObservables.h:369-388 over full state bits (checkState at bit
+    Ip - 1 - o, FermionicModel.h:35-40), independent of the occupation tables."""
+    Ns, norb, Ip = m.Ns, m.interacting_orbitals(), 2 * m.Ns
+    states = [(int(a) << Ns) | int(b) for a in eo.basis(Ns, sector[0]) for b in eo.basis(Ns, sector[1])]
+    cs = lambda st, o: (st >> (Ip - 1 - o)) & 1
+    mm = np.zeros((norb, norb))
+    dd = np.zeros(norb)
+    for st, x in zip(states, v):
+        w = x * x
+        for o in range(norb):
+            up, dn = cs(st, o), cs(st, o + Ns)
+            dd[o] += up * dn * w
+            for o2 in range(norb):
+                mm[o, o2] += (up - dn) * (cs(st, o2) - cs(st, o2 + norb)) * w
+    return mm.reshape(-1), dd
+
+
+@pytest.mark.parametrize("kind", ["hubbard", "siam"])
+def test_static_observables_follow_reference_orbital_rule(kind):
+    rng = np.random.default_rng(29)
+    m = _random_hubbard(5, rng) if kind == "hubbard" else _random_siam(5, 2, rng)
+    sector = (2, 3)
+    v = rng.normal(size=eo.cnk(5, 2) * eo.cnk(5, 3))
+    v /= np.linalg.norm(v)
+    obs = eo.static_observables(m, [eo.EigenPair(0.0, v, 0, sector)], {"lanc.BETA": 1.0})
+    mm, dd = _literal_observables(m, sector, v)
+    assert np.allclose(obs["M_i M_j"], mm, atol=1e-14)
+    assert np.allclose(obs["D_occ"], dd, atol=1e-14)
+    if kind == "siam":
+        # the rule differs from <m_i m_j> when bath sites exist
+        nu, nd = eo.occupations(m, sector, m.Ns)
+        mi = (nu[:, :2] - nd[:, :2]).astype(float)
+        assert not np.allclose(obs["M_i M_j"], np.einsum("s,si,sj->ij", v * v, mi, mi).reshape(-1))
