@@ -368,9 +368,12 @@ class StaticObservables {
     if (p["storage.EIGENVALUES_ONLY"].as<int>() == 1)
       throw std::logic_error("Eigenvectors have not been computed. Static observables can not be evaluated.");
   }
+  // StaticObservables::calculate_static_observables (StaticObservables.h:
+  // 106-158): Boltzmann-weighted average over the stored eigenpairs of the
+  // per-eigenvector sums, which the engine forms on the device
+  // (edh_static_observables, StaticObservables.h:356-419).
   std::map<std::string, std::vector<precision>> calculate_static_observables(Ham& ham) {
     const int norb = ham.model().interacting_orbitals();
-    const int Ns = ham.model().orbitals(), Ip = ham.model().max_total_electrons();
     std::map<std::string, std::vector<precision>> avg = {
         {"N", std::vector<precision>(norb, 0.0)},     {"N_up", std::vector<precision>(norb, 0.0)},
         {"N_dn", std::vector<precision>(norb, 0.0)},  {"M", std::vector<precision>(norb, 0.0)},
@@ -381,40 +384,18 @@ class StaticObservables {
     const double e0 = pairs.begin()->eigenvalue();
     double sum = 0.0;
     auto& sym = ham.model().symmetry();
+    std::vector<double> out;
+    const char* keys[5] = {"N", "N_up", "N_dn", "M", "D_occ"};
     for (const auto& pr : pairs) {
       double bf = std::exp(-(pr.eigenvalue() - e0) * beta_);
       if (!gf::passes_cutoff(bf, cutoff_)) continue;
-      std::vector<double> n(norb, 0), nu(norb, 0), nd(norb, 0), m(norb, 0), dd(norb, 0), mm(norb * norb, 0);
-      double inv_neff = 0.0;
       sym.set_sector(pr.sector());
-      const auto& vec = pr.eigenvector();
-      for (size_t i = 0; i < vec.size(); ++i) {
-        sym.next_state();
-        const long long st = sym.state();
-        const double wgt = vec[i] * vec[i];
-        for (int o = 0; o < norb; ++o) {
-          int up = (int)((st >> (Ip - 1 - o)) & 1), dn = (int)((st >> (Ip - 1 - o - Ns)) & 1);
-          n[o] += (up + dn) * wgt;
-          nu[o] += up * wgt;
-          nd[o] += dn * wgt;
-          m[o] += (up - dn) * wgt;
-          dd[o] += up * dn * wgt;
-          for (int o2 = 0; o2 < norb; ++o2) {
-            int up2 = (int)((st >> (Ip - 1 - o2)) & 1), dn2 = (int)((st >> (Ip - 1 - o2 - norb)) & 1);
-            mm[norb * o + o2] += (up - dn) * (up2 - dn2) * wgt;
-          }
-        }
-        inv_neff += wgt * wgt;
-      }
-      for (int o = 0; o < norb; ++o) {
-        avg["N"][o] += n[o] * bf;
-        avg["N_up"][o] += nu[o] * bf;
-        avg["N_dn"][o] += nd[o] * bf;
-        avg["M"][o] += m[o] * bf;
-        avg["D_occ"][o] += dd[o] * bf;
-      }
-      for (int q = 0; q < norb * norb; ++q) avg["M_i M_j"][q] += mm[q] * bf;
-      avg["N_eff"][0] += (1.0 / inv_neff) * bf;
+      ham.storage().fill();
+      ham.storage().static_observables(pr.eigenvector(), norb, out);
+      for (int k = 0; k < 5; ++k)
+        for (int o = 0; o < norb; ++o) avg[keys[k]][o] += out[(size_t)k * norb + o] * bf;
+      for (int q = 0; q < norb * norb; ++q) avg["M_i M_j"][q] += out[5 * (size_t)norb + q] * bf;
+      avg["N_eff"][0] += out[5 * (size_t)norb + (size_t)norb * norb] * bf;
       avg["E"][0] += pr.eigenvalue() * bf;
       sum += bf;
     }

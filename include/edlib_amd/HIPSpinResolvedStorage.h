@@ -137,6 +137,13 @@ class HIPSpinResolvedStorage {
     edh_throw(edh_local_op(ctx_.get(), s.nup(), s.ndown(), op, (int)orbs.size(), orbs.data(), in.data(), out.data(),
                            EDH_MEM_HOST));
   }
+  // Per-eigenvector static observables of a vector of the filled sector,
+  // on the device (StaticObservables.h:356-419): N, N_up, N_dn, M, D_occ
+  // (norb each), M_i M_j (norb x norb), N_eff.
+  void static_observables(const std::vector<prec>& v, int norb, std::vector<double>& out) {
+    out.assign(5 * (size_t)norb + (size_t)norb * norb + 1, 0.0);
+    edh_throw(edh_static_observables(ctx_.get(), v.data(), (int64_t)v.size(), EDH_MEM_HOST, norb, out.data()));
+  }
   // Continued fraction on the device (Lanczos.h:201-224).
   void cf_eval(const std::vector<double>& zre, const std::vector<double>& zim, double ev, int nlanc, int isign,
                double expb, const std::vector<prec>& alpha, const std::vector<prec>& beta, std::vector<double>& re,
