@@ -177,6 +177,13 @@ def main():
                                  n, 0, ea.EDH_MEM_HOST))
     pcie_gbs = B / (time.perf_counter() - t0) / 1e9
     del wh
+    # static observables of the (normalised) vector on the device: one read
+    # of v by rows and one by column tiles (StaticObservables.h:356-419)
+    ctx.static_observables(vb, Ns)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.static_observables(vb, Ns)
+    obs_ms = (time.perf_counter() - t0) / 3 * 1e3
 
     out = {
         "metric": METRIC,
@@ -202,6 +209,10 @@ def main():
                      "kernel_ms": round(kern_ms, 4), "variants": per_variant},
     }
     out["pcie_inclusive_gbs"] = round(pcie_gbs, 2)
+    out["static_observables"] = {"ms": round(obs_ms, 3), "sites": Ns,
+                                 "gbs": round(2 * 8 * n / (obs_ms * 1e-3) / 1e9, 1),
+                                 "what": "N, N_up, N_dn, M, D_occ, M_i M_j, N_eff of one sector vector, host call "
+                                         "including the small host reduction (2 reads of v)"}
     ctx.close()
     if not args.no_cpu_baseline:
         wcpu = w.copy()
