@@ -55,6 +55,7 @@ _SIGS = {
     "edh_synchronize": (ctypes.c_int, [_vp]),
     "edh_av": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int]),
     "edh_static_observables": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp]),
+    "edh_batch_static_observables": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "edh_get_hv_variant": (ctypes.c_int, [_vp, _ip]),
     "edh_set_hv_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edh_av_time": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),

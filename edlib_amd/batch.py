@@ -117,6 +117,15 @@ class Batch:
                                               ctypes.byref(nconv)))
         return ev[:nev], (vecs.T.copy() if vecs is not None else None)
 
+    def static_observables(self, jid, index, norb):
+        """Per-eigenvector static observables of eigenpair `index` of a
+        finished eigensolver job, as edlib_amd.Context.static_observables."""
+        out = np.zeros(5 * norb + norb * norb + 1)
+        check(_lib.lib().edh_batch_static_observables(self._h, jid, index, norb, _d(out)))
+        k = norb
+        return {"N": out[0:k], "N_up": out[k:2 * k], "N_dn": out[2 * k:3 * k], "M": out[3 * k:4 * k],
+                "D_occ": out[4 * k:5 * k], "M_i M_j": out[5 * k:5 * k + k * k], "N_eff": out[5 * k + k * k:]}
+
     def lanczos_result(self, jid, nl):
         a, b = np.zeros(nl), np.zeros(nl + 1)
         nlanc = ctypes.c_int()

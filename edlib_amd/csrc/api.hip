@@ -860,6 +860,18 @@ int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs,
   API_END
 }
 
+int edh_batch_static_observables(edh_batch* b, int job_id, int index, int norb, double* out) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  edh::BJob& j = B.job(job_id);
+  EDH_CHECK(j.kind == edh::JOB_EIG && j.state == edh::JOB_DONE && j.evecs, EDH_ERR_STATE,
+            "batch: eigensolver job has no eigenvectors");
+  EDH_CHECK(index >= 0 && index < j.nev && out, EDH_ERR_ARG, "batch: eigenpair index out of range");
+  edh::BSector& s = B.sector(j.sec);
+  edh::static_obs_sector(s.tabs, s.dev, s.model->Ns, j.evecs + (size_t)index * j.n, norb, B.d_obs, B.stream, out);
+  API_END
+}
+
 int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc) {
   API_BEGIN
   edh::BJob& j = use_batch(b).job(job_id);

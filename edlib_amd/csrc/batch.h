@@ -129,6 +129,7 @@ struct Batch {
   std::map<std::tuple<int, int, int>, std::vector<void*>> maps;
   std::map<std::tuple<int, int, int>, std::vector<char>> map_blobs;
   DevBuf d_gather;
+  DevBuf d_obs;  // static-observable scratch
   std::vector<ReadSlab> slabs;
   std::vector<int> slab_fifo;  // slabs in flight, oldest first
 

@@ -727,7 +727,7 @@ Batch::~Batch() {
   for (auto& s : secs)
     if (s) s->dev.release();
   if (stream) (void)hipStreamSynchronize(stream);
-  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms, &d_gather}) b->release();
+  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms, &d_gather, &d_obs}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);
     if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);

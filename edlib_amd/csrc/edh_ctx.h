@@ -62,6 +62,10 @@ void eigsh_run(Ctx& c, int nev, int ncv, double tol, int maxitr, uint64_t seed, 
 // (StaticObservables.h:356-419): out = N, N_up, N_dn, M, D_occ (norb each),
 // M_i M_j (norb x norb, row-major), N_eff.
 void static_obs_run(Ctx& c, const double* dv, int norb, double* out);
+// The same for a vector of any uploaded sector (tables st / ds), with its
+// scratch and stream.
+void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
+                       DevBuf& scratch, hipStream_t stream, double* out);
 
 // Diagonal element of state (u, d) of the current sector from the host tables.
 double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d);

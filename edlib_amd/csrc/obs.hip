@@ -102,30 +102,32 @@ __global__ __launch_bounds__(kObsBlock) void obs_cols_sum_kernel(const double* _
 }  // namespace
 
 void static_obs_run(Ctx& c, const double* dv, int norb, double* out) {
-  const SectorTables& st = c.tabs;
-  const int Ns = c.model->Ns;
+  static_obs_sector(c.tabs, c.dev, c.model->Ns, dv, norb, c.obs, c.stream, out);
+}
+
+void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
+                       DevBuf& scratch, hipStream_t stream, double* out) {
   EDH_CHECK(Ns <= kObsMaxSites, EDH_ERR_LIMIT, "static observables: more than 32 sites per spin");
   EDH_CHECK(norb >= 1 && norb <= Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
   const int64_t Dup = st.up.D, Ddn = st.dn.D;
   const int nout = Ns + 2;
   const int64_t ntile = (Dup + kColTile - 1) / kColTile;
   const size_t rows_n = (size_t)Dup * nout, part_n = (size_t)ntile * Ddn;
-  c.obs.ensure((rows_n + part_n + (size_t)Ddn) * sizeof(double));
-  double* rows = c.obs.as<double>();
+  scratch.ensure((rows_n + part_n + (size_t)Ddn) * sizeof(double));
+  double* rows = scratch.as<double>();
   double* part = rows + rows_n;
   double* col = part + part_n;
   const int grid_rows = (int)std::min<int64_t>(Dup, 4096);
-  hipLaunchKernelGGL(obs_rows_kernel, dim3(grid_rows), dim3(kObsBlock), 0, c.stream, dv, Dup, Ddn, c.dev.dn.bits, Ns,
-                     rows);
+  hipLaunchKernelGGL(obs_rows_kernel, dim3(grid_rows), dim3(kObsBlock), 0, stream, dv, Dup, Ddn, ds.dn.bits, Ns, rows);
   hipLaunchKernelGGL(obs_cols_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock), (unsigned)ntile),
-                     dim3(kObsBlock), 0, c.stream, dv, Dup, Ddn, part);
+                     dim3(kObsBlock), 0, stream, dv, Dup, Ddn, part);
   hipLaunchKernelGGL(obs_cols_sum_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0,
-                     c.stream, part, ntile, Ddn, col);
+                     stream, part, ntile, Ddn, col);
   EDH_HIP(hipGetLastError());
   std::vector<double> hr(rows_n), hc(Ddn);
-  EDH_HIP(hipMemcpyAsync(hr.data(), rows, rows_n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  EDH_HIP(hipMemcpyAsync(hc.data(), col, (size_t)Ddn * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  EDH_HIP(hipStreamSynchronize(c.stream));
+  EDH_HIP(hipMemcpyAsync(hr.data(), rows, rows_n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipMemcpyAsync(hc.data(), col, (size_t)Ddn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipStreamSynchronize(stream));
   // second moments of the occupation bits (bit b of a per-spin pattern is
   // site Ns - 1 - b, FermionicModel.h:35-68)
   std::vector<double> UU((size_t)Ns * Ns, 0.0), UD((size_t)Ns * Ns, 0.0), DD((size_t)Ns * Ns, 0.0);

@@ -55,6 +55,35 @@ class SolveParams:
         return self.ncv if self.ncv > 0 else 2 * self.nev + 3
 
 
+def _interacting(model):
+    """Interacting orbitals: the impurity orbitals of an Anderson model, every
+    site of a Hubbard model."""
+    ml = model.interacting_orbitals()
+    return ml if ml > 0 else model.Ns
+
+
+class _Observables:
+    """StaticObservables::calculate_static_observables (StaticObservables.h:
+    106-158): the per-eigenvector sums (from the device) averaged with the
+    Boltzmann factors of the pairs that pass the cutoff, E included."""
+
+    def __init__(self):
+        self.sum = None
+        self.z = 0.0
+
+    def add(self, obs, e, bf):
+        c = dict(obs)
+        c["E"] = np.array([e])
+        if self.sum is None:
+            self.sum = {k: np.zeros_like(v) for k, v in c.items()}
+        for k, v in c.items():
+            self.sum[k] += v * bf
+        self.z += bf
+
+    def result(self):
+        return {k: v / self.z for k, v in self.sum.items()} if self.sum else {}
+
+
 def sector_list(Ns, p):
     secs = list(p.sectors) if p.sectors else [(i, j) for i in range(Ns + 1) for j in range(Ns + 1)]
     return sorted(secs, key=lambda s: (binom(Ns, s[0]) * binom(Ns, s[1]), s[0], s[1]))
@@ -119,10 +148,13 @@ def solve_plain(model, p, device=0, ctx=None):
     Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
     Gl = np.zeros_like(Gg)
     n_gf = 0
+    obs = _Observables()
     for e, sec, vec in pairs:
         bf = math.exp(-(e - e0) * p.beta)
         if not _passes(bf, p.cutoff):
             continue
+        ctx.set_sector(*sec)
+        obs.add(ctx.static_observables(vec, _interacting(model)), e, bf)
         for q, (o, s, ann, nxt) in enumerate(_gf_targets(Ns, sec, p.orbitals)):
             _, out = ctx.adag(sec, o + s * Ns, ann, vec)
             norm = ctx.vv(out, out)
@@ -140,7 +172,7 @@ def solve_plain(model, p, device=0, ctx=None):
     if own:
         ctx.close()
     return {"E0": e0, "Z": Z, "G": G, "energies": np.array([q[0] for q in pairs]), "n_gf": n_gf, "n_av": n_av,
-            "t_eig": t_eig, "t_total": t_all}
+            "observables": obs.result(), "t_eig": t_eig, "t_total": t_all}
 
 
 class BatchedSolve:
@@ -261,12 +293,18 @@ class BatchedSolve:
                 for q, ((jid, _), (_, _, o, s, ann), _) in enumerate(rows):
                     (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
                     self.b.free_job(jid)
+                obs = _Observables()
+                for e, _, _, j, k in self.pairs:
+                    bf = math.exp(-(e - self.e0) * p.beta)
+                    if _passes(bf, p.cutoff):
+                        obs.add(self.b.static_observables(j, k, _interacting(self.model)), e, bf)
                 for _, j in self.eig_jobs:
                     self.b.free_job(j)
                 for sid in self.sid.values():
                     self.b.remove_sector(sid)
                 self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
-                               "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows)}
+                               "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows),
+                               "observables": obs.result()}
                 self.phase = "done"
         return self.phase == "done"
 

@@ -236,6 +236,9 @@ int edh_batch_eig_bounds(edh_batch* b, int count, const int* job_ids, double* bo
  * column-major) may be NULL.  After maxitr restarts only the converged pairs
  * are returned (nconv < the requested nev), as Storage::diag does. */
 int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs, int* nconv);
+/* edh_static_observables for eigenpair `index` of a finished eigensolver
+ * job (its vector stays on the device). */
+int edh_batch_static_observables(edh_batch* b, int job_id, int index, int norb, double* out);
 /* alpha[nlanc], beta[nl+1] (beta[0] unused), as edh_lanczos. */
 int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc);
 /* Stops a process (it takes no further sweeps; its state becomes

@@ -116,6 +116,13 @@ def test_batched_solve_matches_plain_and_oracle():
     go = eo.greens_function(om, pairs, prm, gf_orbs=[0])
     assert abs(pairs[0].eigenvalue - rb["E0"]) < 1e-10 * abs(rb["E0"])
     assert np.abs(rb["G"][:, 0, :] - go["G"][:, 0, :]).max() < 1e-8
+    # static observables (StaticObservables.h:106-158, 356-419): batched and
+    # plain from device sums, against the oracle's per-state loop
+    oo = eo.static_observables(om, pairs, prm)
+    for key in ("N", "N_up", "N_dn", "M", "D_occ", "M_i M_j", "E"):
+        assert np.abs(rb["observables"][key] - rp["observables"][key]).max() < 1e-10, key
+        assert np.abs(rb["observables"][key] - oo[key]).max() < 1e-8, key
+    assert abs(rb["observables"]["N_eff"][0] - oo["N_eff"][0]) < 1e-8 * oo["N_eff"][0]
 
 
 def test_served_result_same_alone_and_in_mix():
