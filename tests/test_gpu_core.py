@@ -64,7 +64,10 @@ def torus12():
     return p, om, eo.SRS(om, 6, 6)
 
 
-@pytest.mark.parametrize("variant", [0, 1], ids=["single_pass", "two_pass"])
+VARIANTS = dict(argvalues=[0, 1], ids=["single_pass", "two_pass"])
+
+
+@pytest.mark.parametrize("variant", **VARIANTS)
 def test_av_parity_12site_torus(ctx, torus12, variant):
     p, om, srs = torus12
     ctx.set_model(models.product_model("hubbard", p))
@@ -78,7 +81,7 @@ def test_av_parity_12site_torus(ctx, torus12, variant):
     assert _rel(ctx.av(v), srs.av(v, omp=True)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1], ids=["single_pass", "two_pass"])
+@pytest.mark.parametrize("variant", **VARIANTS)
 def test_lanczos_12site_torus(ctx, torus12, variant):
     p, om, srs = torus12
     ctx.set_model(models.product_model("hubbard", p))
@@ -348,3 +351,26 @@ def test_av_parity_16site_sector():
         err = np.abs(got - ref).max() / np.abs(ref).max()
         assert err < 1e-13, (var, err)
     ctx.close()
+
+
+@pytest.mark.parametrize("nup", [1, 2, 3])
+def test_av_parity_16site_long_rows(ctx, nup):
+    """Rows of 12,870 columns (four column chunks) with few rows: sectors
+    (nup, 8) of the 4x4 torus, both H*v variants against the oracle,
+    accumulate and clear modes."""
+    Ns = 16
+    t = models.torus_hopping(4, 4)
+    om = eo.hubbard(Ns, t, np.full(Ns, 6.0), np.full(Ns, 3.0))
+    srs = eo.SRS(om, nup, 8)
+    ctx.set_model(ea.HubbardModel(Ns, t, np.full(Ns, 6.0), np.full(Ns, 3.0)))
+    n = ctx.set_sector(nup, 8)
+    assert n == srs.n
+    rng = np.random.default_rng(20261202 + nup)
+    v = rng.uniform(-1, 1, n)
+    w0 = rng.uniform(-1, 1, n)
+    ref_acc = srs.av(v, w0.copy(), clear=False)
+    ref = srs.av(v)
+    for var in (0, 1):
+        ctx.hv_variant = var
+        assert _rel(ctx.av(v, w0, clear=False), ref_acc) < 1e-13, var
+        assert _rel(ctx.av(v), ref) < 1e-13, var
