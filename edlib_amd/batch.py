@@ -61,15 +61,26 @@ class Batch:
     def submit_gf(self, requests, nl):
         """Green's-function start vectors c(+)_orb |psi_index>, one per request
         (eig_job, index, orb, annihilate, dst_sid), and their Lanczos runs.
-        Returns (job ids, -1 where the vector vanishes; squared norms)."""
+        Returns the job ids (-1 where the target sector is empty); nothing
+        waits for the device.  A run whose start vector vanishes (squared
+        norm <= 1e-10) stops at once; the squared norms are known once the
+        runs have finished (lanczos_norm2)."""
         k = len(requests)
         cols = [np.ascontiguousarray([int(r[c]) for r in requests], dtype=np.int32) for c in range(5)]
-        norms = np.zeros(max(k, 1))
         ids = np.zeros(max(k, 1), dtype=np.int32)
         ip = [c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) for c in cols]
-        check(_lib.lib().edh_batch_submit_gf(self._h, k, ip[0], ip[1], ip[2], ip[3], ip[4], nl, _d(norms),
+        check(_lib.lib().edh_batch_submit_gf(self._h, k, ip[0], ip[1], ip[2], ip[3], ip[4], nl,
                                              ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-        return ids[:k].tolist(), norms[:k].tolist()
+        return ids[:k].tolist()
+
+    def lanczos_norm2(self, jids):
+        """Squared start-vector norms of finished Lanczos runs (NaN while
+        running)."""
+        ids = np.ascontiguousarray(jids, dtype=np.int32)
+        out = np.zeros(max(len(ids), 1))
+        check(_lib.lib().edh_batch_lanczos_norm2(self._h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                 _d(out)))
+        return out[:len(ids)]
 
     def step(self, nsweeps=1):
         act = ctypes.c_int()

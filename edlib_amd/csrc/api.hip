@@ -781,20 +781,28 @@ int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl,
 }
 
 int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
-                        const int* annihilate, const int* dst_secs, int nl, double* norms, int* job_ids) {
+                        const int* annihilate, const int* dst_secs, int nl, int* job_ids) {
   API_BEGIN
   edh::Batch& B = use_batch(b);
   EDH_CHECK(count >= 0, EDH_ERR_ARG, "negative count");
-  EDH_CHECK(count == 0 || (eig_jobs && indices && orbs && annihilate && dst_secs && norms && job_ids), EDH_ERR_ARG,
+  EDH_CHECK(count == 0 || (eig_jobs && indices && orbs && annihilate && dst_secs && job_ids), EDH_ERR_ARG,
             "null argument");
   std::vector<edh::GfRequest> req(count);
   for (int r = 0; r < count; ++r) req[r] = {eig_jobs[r], indices[r], orbs[r], annihilate[r] ? 1 : 0, dst_secs[r]};
   std::vector<int> ids;
-  std::vector<double> nr;
-  B.submit_gf(req, nl, ids, nr);
-  for (int r = 0; r < count; ++r) {
-    job_ids[r] = ids[r];
-    norms[r] = nr[r];
+  B.submit_gf(req, nl, ids);
+  for (int r = 0; r < count; ++r) job_ids[r] = ids[r];
+  API_END
+}
+
+int edh_batch_lanczos_norm2(edh_batch* b, int count, const int* job_ids, double* norm2) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(count >= 0 && (count == 0 || (job_ids && norm2)), EDH_ERR_ARG, "bad argument");
+  for (int q = 0; q < count; ++q) {
+    const edh::BJob& j = B.job(job_ids[q]);
+    EDH_CHECK(j.kind == edh::JOB_LZ, EDH_ERR_ARG, "batch: not a Lanczos job");
+    norm2[q] = j.state == edh::JOB_DONE ? j.norm2 : std::numeric_limits<double>::quiet_NaN();
   }
   API_END
 }

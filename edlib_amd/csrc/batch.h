@@ -1,6 +1,7 @@
 // Batched Krylov engine (grouped launches over many sectors and processes).
 #pragma once
 
+#include <limits>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -47,6 +48,7 @@ struct BJob {
   int64_t ntdot = 0;
   int* d_flags = nullptr;  // [0] stopped (|beta| < 1e-10), [1] steps taken
   std::vector<double> alpha, beta;
+  double norm2 = std::numeric_limits<double>::quiet_NaN();  // start vector's squared norm (Green's-function runs)
   // scalars of a finished Lanczos run: a place in one of the batch's pinned
   // read slabs (the run sits out the sweeps until that copy has completed)
   const double* h_read = nullptr;
@@ -81,7 +83,10 @@ struct BOpHost {
 
 // OP_LZS: the Lanczos vectors are kept unnormalised (the next step's H*v
 // and update scale them), so the norm also sets the next step's scales.
-enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8, OP_LZS = 16 };
+// OP_START: the norm of a Lanczos run's unnormalised start vector sets its
+// first scale and its squared norm (nrm_out), or stops the run when the
+// squared norm is <= 1e-10 (the vector vanishes).
+enum OpMode { OP_NB = 1, OP_GATED = 2, OP_LZ = 4, OP_SCALE_NRM = 8, OP_LZS = 16, OP_START = 32 };
 enum GroupKind { G_PROJECT = 0, G_NORM = 1, G_SCALE = 2, G_REORTH = 3, G_LZ = 4 };
 
 struct OpGroup {
@@ -118,7 +123,7 @@ struct Batch {
   hipStream_t stream = nullptr;
   std::vector<std::unique_ptr<BSector>> secs;
   std::vector<std::unique_ptr<BJob>> jobs;
-  DevBuf d_arena, d_partial, d_scal, d_norms;
+  DevBuf d_arena, d_partial, d_scal;
   void* h_stage[2] = {nullptr, nullptr};
   size_t h_stage_bytes[2] = {0, 0};
   hipEvent_t ev_stage[2] = {nullptr, nullptr};
@@ -146,7 +151,7 @@ struct Batch {
   int submit_lanczos(int sec, const double* v0_dev, int nl);
   // Green's-function start vectors and their Lanczos runs; ids[i] = -1 when
   // vector i vanishes; norms[i] = its squared norm.
-  void submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms);
+  void submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids);
   void free_job(int id);
   int step();
   int autotune(const std::vector<int>& sec_ids);

@@ -285,7 +285,20 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
       o.sc[S_ONE] = 1.0;
       o.sc[S_NEGNRM] = -nrm;
       if (o.mode & OP_NB) o.sc[S_GATE] = nrm < kDgks * o.sc[S_NB] ? 1.0 : 0.0;
-      if (o.nrm_out) *o.nrm_out = nrm;
+      if (o.mode & OP_START) {
+        // GreensFunction.h:408-418: a start vector of squared norm <= 1e-10
+        // has no run; otherwise the run starts from it normalised
+        const double n2 = nrm * nrm;
+        *o.nrm_out = n2;
+        if (fabs(n2) > 1e-10) {
+          o.sc[S_LZV] = 1.0 / nrm;
+        } else {
+          o.sc[S_LZV] = 0.0;
+          o.flags[0] = 1;
+        }
+      } else if (o.nrm_out) {
+        *o.nrm_out = nrm;
+      }
       if (o.mode & OP_LZ) {  // Lanczos.h:61-65
         if (o.hist_b) *o.hist_b = nrm;
         o.flags[1] = o.kstep;
@@ -727,7 +740,7 @@ Batch::~Batch() {
   for (auto& s : secs)
     if (s) s->dev.release();
   if (stream) (void)hipStreamSynchronize(stream);
-  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_norms, &d_gather, &d_obs}) b->release();
+  for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);
     if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
@@ -952,12 +965,15 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
     return id;
   }
   for (int q = 0; q < 3; ++q) j.lz[q] = dalloc(j, (size_t)j.n);
-  j.sc = dalloc(j, S_T + 2 * (size_t)nl + 2);
-  fill_dev(j.sc, S_T + 2 * (size_t)nl + 2);
-  dev_fill(j.sc + S_LZV, 1, 1.0, stream);  // the start vector is normalised
+  // scalars, alpha history (nl), beta history (nl + 1), stop flags, start
+  // vector's squared norm
+  j.sc = dalloc(j, S_T + 2 * (size_t)nl + 3);
+  fill_dev(j.sc, S_T + 2 * (size_t)nl + 3);
+  dev_fill(j.sc + S_LZV, 1, 1.0, stream);  // a given start vector is normalised
   j.d_alpha = j.sc + S_T;
   j.d_beta = j.d_alpha + nl;
   j.d_flags = (int*)(j.d_beta + nl + 1);
+  dev_fill(j.d_alpha + 2 * (size_t)nl + 2, 1, 1.0, stream);  // squared start norm (set again for a gathered start)
   j.ntdot = hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
   j.tdot = dalloc(j, (size_t)j.ntdot);
   if (v0_dev)
@@ -1006,17 +1022,16 @@ void Batch::gather_maps(int src, int orb, bool annihilate, std::vector<void*>& o
 }
 
 // Green's-function start vectors c(+)_orb |psi> (GreensFunction.h:395-416,
-// 437-458, one orbital each) gathered on the device, their squared norms
-// from one grouped reduction, and a Lanczos run from each normalised vector
-// whose squared norm exceeds 1e-10.
-void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids, std::vector<double>& norms) {
+// 437-458, one orbital each) gathered on the device, and a Lanczos run from
+// each.  Nothing waits for the device: one grouped reduction gives every
+// start vector's norm, which becomes its run's first scale (the stored
+// vector stays unnormalised) or, at a squared norm <= 1e-10, stops the run
+// before its first step.  The squared norms reach the host with the runs'
+// coefficients (norm2 of a finished run).
+void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int>& ids) {
   EDH_CHECK(nl >= 1, EDH_ERR_ARG, "batch: nl must be >= 1");
   ids.assign(req.size(), -1);
-  norms.assign(req.size(), 0.0);
-  std::vector<int> made(req.size(), -1);
   OpGroup g{G_NORM, {}};
-  d_norms.ensure(std::max<size_t>(req.size(), 1) * sizeof(double));
-  double* dn = d_norms.as<double>();
   for (size_t r = 0; r < req.size(); ++r) {
     const GfRequest& q = req[r];
     BJob& e = job(q.eig_job);
@@ -1036,44 +1051,20 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
     gather_maps(e.sec, q.orb, q.annihilate != 0, mp);
     const int id = submit_lanczos(q.dst_sec, nullptr, nl);
     BJob& j = job(id);
-    j.state = JOB_FAILED;  // not active until its norm is known
     dev_separable_gather(e.evecs + (size_t)q.index * e.n, a.tabs.dn.D, (const int32_t*)mp[0], (const int32_t*)mp[1],
                          (const int8_t*)mp[2], (const int8_t*)mp[3], j.lz[0], b.tabs.up.D, b.tabs.dn.D, stream);
     BOpHost o;
     o.w = j.lz[0];
     o.n = j.n;
     o.sc = j.sc;
-    o.nrm_out = dn + r;
+    o.flags = j.d_flags;
+    o.mode = OP_START;
+    o.nrm_out = j.d_alpha + 2 * (size_t)nl + 2;
     g.ops.push_back(o);
-    made[r] = id;
+    ids[r] = id;
   }
   std::vector<OpGroup> pre{g}, post;
   issue(pre, nullptr, post);
-  std::vector<double> nr(req.size(), 0.0);
-  if (!req.empty())
-    EDH_HIP(hipMemcpyAsync(nr.data(), dn, req.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-  EDH_HIP(hipStreamSynchronize(stream));
-  OpGroup sg{G_SCALE, {}};
-  for (size_t r = 0; r < req.size(); ++r) {
-    if (made[r] < 0) continue;
-    BJob& j = job(made[r]);
-    norms[r] = nr[r] * nr[r];
-    if (!(std::fabs(norms[r]) > 1e-10)) {
-      release_job(j);
-      jobs[made[r]].reset();
-      continue;
-    }
-    BOpHost o;
-    o.w = j.lz[0];
-    o.n = j.n;
-    o.sc = j.sc;
-    o.mode = OP_SCALE_NRM;
-    sg.ops.push_back(o);
-    j.state = JOB_ACTIVE;
-    ids[r] = made[r];
-  }
-  std::vector<OpGroup> pre2{sg};
-  issue(pre2, nullptr, post);
 }
 
 void Batch::free_job(int id) {
@@ -1421,7 +1412,8 @@ int Batch::step() {
     a.wprev = j->lz[1];
     a.vscale = 1.0;
     a.mode = j->step > 1 ? 1 : 0;
-    a.dscale = j->step > 1 ? j->sc + S_LZV : nullptr;  // {1 / beta_{k-1}, -beta_{k-1} / beta_{k-2}}
+    // {1 / beta_{k-1}, -beta_{k-1} / beta_{k-2}}; at the first step {1 / |v_1|, 0}
+    a.dscale = j->sc + S_LZV;
     a.done = j->d_flags;
     a.tdot = j->tdot;  // alpha = <v_k, w> from the same pass
     hv.push_back({j->sec, a});
@@ -1482,7 +1474,7 @@ int Batch::step() {
   std::vector<ReadReq> reads;
   for (size_t q = 0; q < eig.size(); ++q) reads.push_back({eig_id[q], eig[q]->ctl, (size_t)C_HDR + eig[q]->nev});
   for (size_t q = 0; q < lz.size(); ++q)
-    if (lz[q]->step == lz[q]->nl) reads.push_back({lz_id[q], lz[q]->d_alpha, 2 * (size_t)lz[q]->nl + 2});
+    if (lz[q]->step == lz[q]->nl) reads.push_back({lz_id[q], lz[q]->d_alpha, 2 * (size_t)lz[q]->nl + 3});
   issue(pre, &hv, post, &eig, &reads);
   ++sweeps;
   for (BJob* j : lz) {
@@ -1503,6 +1495,7 @@ void Batch::finish_lanczos(BJob& j) {
   j.nlanc = fl[1];
   for (int q = 0; q < nl; ++q) j.alpha[q] = q < j.nlanc ? hist[q] : 0.0;
   for (int q = 0; q <= nl; ++q) j.beta[q] = (q >= 1 && q <= j.nlanc && q != nl) ? hist[nl + q] : 0.0;
+  j.norm2 = hist[2 * nl + 2];
   j.state = JOB_DONE;
   release_job(j);  // the coefficients are on the host now
   j.lz[0] = j.lz[1] = j.lz[2] = nullptr;

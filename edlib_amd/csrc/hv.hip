@@ -129,6 +129,8 @@ __device__ __forceinline__ double s4_fma(uint32_t e, const char* sv, double a) {
 // Quads held in registers per column group (slices with more quads finish
 // in a plain loop).
 constexpr int kQuadRegs = 6;
+// Row staging: loads per thread issued before the LDS writes.
+constexpr int kStage = 8;
 
 // EF: down-spin entry format: 0 = 32-bit sliced ELL (column | value index),
 // 1/2 = 16-bit entries in slot quads (four / two distinct values),
@@ -164,7 +166,20 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
     const int u = __builtin_amdgcn_readfirstlane(p.row_order[slot]);
     const size_t rowoff = (size_t)u * Ddn;
     if (LDS) {
-      for (int d = tid; d < Ddn; d += T) s_v[d] = V[rowoff + d] * vscale;
+      // kStage loads per thread in flight together, then the LDS writes
+      for (int d0 = 0; d0 < Ddn; d0 += kStage * T) {
+        double x[kStage];
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          const int d = d0 + tid + k * T;
+          x[k] = d < Ddn ? V[rowoff + d] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+          const int d = d0 + tid + k * T;
+          if (d < Ddn) s_v[d] = x[k] * vscale;
+        }
+      }
       if (EF && tid == 0) s_v[Ddn] = 0.0;  // target of the padding entries
       __syncthreads();
     }
@@ -176,12 +191,18 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
       // loaded like the others but never stored.
 #define DCOL(j) min(c0 + tid + (j) * T, Ddn - 1)
       double acc[J];
-      // accumulated input and diagonal (generated from the bit patterns)
+      // accumulated input (the J loads issued together) and diagonal
+      // (generated from the bit patterns)
+      double win[J];
+      if (p.accumulate) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) win[j] = __builtin_nontemporal_load(p.wprev + rowoff + DCOL(j));
+      }
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int d = DCOL(j);
         const int bd = p.dn_bits[d];
-        const double wp = p.accumulate ? wscale * __builtin_nontemporal_load(p.wprev + rowoff + d) : 0.0;
+        const double wp = p.accumulate ? wscale * win[j] : 0.0;
         const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
         acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd, na), vd, wp);
       }

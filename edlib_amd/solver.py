@@ -201,7 +201,7 @@ class BatchedSolve:
         self.eig_jobs = []
         self.known = []      # (E, sector index, sector, eig job, k) of finished sectors
         self.seen = set()    # finished eig jobs
-        self.gf = {}         # (eig job, k, orbital, spin, annihilate) -> (gf job or -1, norm)
+        self.gf = {}         # (eig job, k, orbital, spin, annihilate) -> gf job or -1
         self.phase = "new"
         self.result = None
         c = p.cutoff
@@ -250,9 +250,8 @@ class BatchedSolve:
                     reqs.append((j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]))
                     keys.append(key)
         if reqs:
-            ids, norms = self.b.submit_gf(reqs, self.p.nlanc)
-            for key, jid, norm in zip(keys, ids, norms):
-                self.gf[key] = (jid, norm)
+            for key, jid in zip(keys, self.b.submit_gf(reqs, self.p.nlanc)):
+                self.gf[key] = jid
 
     def advance(self):
         """Moves the solve on; returns True when it is complete."""
@@ -274,24 +273,29 @@ class BatchedSolve:
                     for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
                         self.used.append(((j, k, o, s, ann), e))
                 needed = {key for key, _ in self.used}
-                for key, (jid, _) in self.gf.items():
+                for key, jid in self.gf.items():
                     if key not in needed and jid >= 0:
                         self.b.cancel(jid)
                         self.b.free_job(jid)
                 self.phase = "gf"
         if self.phase == "gf":
-            live = [self.gf[key][0] for key, _ in self.used if self.gf[key][0] >= 0]
+            live = [self.gf[key] for key, _ in self.used if self.gf[key] >= 0]
             if not live or not np.any(self.b.job_states(live) == 0):
                 wn = matsubara(p.beta, p.nomega)
                 norb = len(p.orbitals)
                 Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
                 Gl = np.zeros_like(Gg)
-                rows = [(self.gf[key], key, e) for key, e in self.used if self.gf[key][0] >= 0]
-                g = self.b.cf_many([r[0][0] for r in rows], [r[0][1] for r in rows],
+                n2 = dict(zip(live, self.b.lanczos_norm2(live))) if live else {}
+                # a vanishing start vector (<out|out> <= 1e-10) has no run
+                # (GreensFunction.h:408-418)
+                rows = [(self.gf[key], key, e) for key, e in self.used
+                        if self.gf[key] >= 0 and abs(n2[self.gf[key]]) > 1e-10]
+                g = self.b.cf_many([r[0] for r in rows], [n2[r[0]] for r in rows],
                                    [-1 if r[1][4] else 1 for r in rows], [_expb(p.beta, r[2], self.e0) for r in rows],
                                    [r[2] for r in rows], wn)
-                for q, ((jid, _), (_, _, o, s, ann), _) in enumerate(rows):
+                for q, (jid, (_, _, o, s, ann), _) in enumerate(rows):
                     (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
+                for jid in live:
                     self.b.free_job(jid)
                 obs = _Observables()
                 for e, _, _, j, k in self.pairs:

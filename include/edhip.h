@@ -213,12 +213,19 @@ int edh_batch_submit_eig(edh_batch* b, int sec_id, int nev, int ncv, double tol,
 int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl, int mem, int* job_id);
 /* Green's-function start vectors, `count` of them: vector r is
  * c(+)_{orbs[r]} |psi> of eigenpair indices[r] of the finished eigensolver
- * job eig_jobs[r], in sector dst_secs[r] (annihilate[r] selects c).
- * norms[r] = <out|out>; when it exceeds 1e-10 a Lanczos run of nl steps
- * starts from out/sqrt(norms[r]) and job_ids[r] is its id, else -1
- * (GreensFunction.h:395-416,437-458 for one orbital each). */
+ * job eig_jobs[r], in sector dst_secs[r] (annihilate[r] selects c), and a
+ * Lanczos run of nl steps from out/|out| (GreensFunction.h:395-416,437-458
+ * for one orbital each); job_ids[r] is its id, or -1 when the target sector
+ * is empty.  Nothing waits for the device: a run whose start vector has
+ * <out|out> <= 1e-10 stops before its first step (its continued fraction
+ * is not used), and <out|out> is known once the run has finished
+ * (edh_batch_lanczos_norm2). */
 int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
-                        const int* annihilate, const int* dst_secs, int nl, double* norms, int* job_ids);
+                        const int* annihilate, const int* dst_secs, int nl, int* job_ids);
+/* Squared start-vector norms <out|out> of finished Green's-function runs
+ * (NaN while a run has not finished; 1 for runs from
+ * edh_batch_submit_lanczos, whose start vector is normalised). */
+int edh_batch_lanczos_norm2(edh_batch* b, int count, const int* job_ids, double* norm2);
 /* Runs up to nsweeps sweeps (stops early when nothing is active). */
 int edh_batch_step(edh_batch* b, int nsweeps, int* n_active);
 /* state EDH_JOB_*, kind EDH_JOB_EIG/LANCZOS, dimension, count = nev or
