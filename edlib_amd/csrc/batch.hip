@@ -152,9 +152,10 @@ __device__ __forceinline__ void block_store_many(const double (&v)[K], int cnt, 
 
 // partial[c * kMaxBasis + i] = chunk c's sum of X_i[e] * w[e], i < cnt, and
 // (OP_NB) slot cnt = chunk c's sum of w[e]^2
+template <int KG>
 __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
                                                          double* __restrict__ partial) {
-  __shared__ double s_part[kOpBlock / 64][kHGroup];
+  __shared__ double s_part[kOpBlock / 64][KG];
   const int c = blockIdx.x;
   const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o)) return;
@@ -162,61 +163,66 @@ __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__
   const int64_t e0 = (int64_t)lc * o.chunk;
   const int64_t e1 = min(o.n, e0 + o.chunk);
   const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
-  for (int i0 = 0; i0 < tot; i0 += kHGroup) {
-    double acc[kHGroup];
+  for (int i0 = 0; i0 < tot; i0 += KG) {
+    double acc[KG];
 #pragma unroll
-    for (int g = 0; g < kHGroup; ++g) acc[g] = 0.0;
+    for (int g = 0; g < KG; ++g) acc[g] = 0.0;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
       const double we = o.w[e];
 #pragma unroll
-      for (int g = 0; g < kHGroup; ++g) {
+      for (int g = 0; g < KG; ++g) {
         const int i = i0 + g;
         if (i < o.cnt) acc[g] = fma(o.X[(int64_t)i * o.ldx + e], we, acc[g]);
         else if (i < tot) acc[g] = fma(we, we, acc[g]);
       }
     }
-    block_store_many<kHGroup>(acc, min(kHGroup, tot - i0), s_part, partial + (size_t)c * kMaxBasis + i0);
+    block_store_many<KG>(acc, min(KG, tot - i0), s_part, partial + (size_t)c * kMaxBasis + i0);
   }
 }
 
 // First pass's update fused with the second pass's projections (cnt <=
-// kFuse): w -= sum_i h[i] X_i; then, from the same registers, chunk c's sums
-// of X_i[e] * w_new[e] in slots i < cnt and of w_new[e]^2 in slot cnt.
+// K): w -= sum_i h[i] X_i; then, from the same registers, chunk c's sums of
+// X_i[e] * w_new[e] in slots i < cnt and of w_new[e]^2 in slot cnt.  K is
+// the group's largest basis size rounded up to 12 or kFuse: the registers
+// it takes set the occupancy (K = 24 runs the NCV = 11 basis ~10 % slower
+// in the whole solve).
 constexpr int kFuse = 24;
+constexpr int kFuseSmall = 12;
+template <int K>
 __global__ __launch_bounds__(kOpBlock) void bupdate_dots_kernel(const BOp* __restrict__ ops,
                                                                 const int* __restrict__ cjob,
                                                                 double* __restrict__ partial) {
-  __shared__ double s_part[kOpBlock / 64][kFuse + 1];
-  __shared__ double s_h[kFuse];
+  __shared__ double s_part[kOpBlock / 64][K + 1];
+  __shared__ double s_h[K];
   const int c = blockIdx.x;
   const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o)) return;
-  if ((int)threadIdx.x < kFuse) s_h[threadIdx.x] = (int)threadIdx.x < o.cnt ? o.sc[threadIdx.x] : 0.0;
+  if ((int)threadIdx.x < K) s_h[threadIdx.x] = (int)threadIdx.x < o.cnt ? o.sc[threadIdx.x] : 0.0;
   __syncthreads();
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * o.chunk;
   const int64_t e1 = min(o.n, e0 + o.chunk);
   const int cnt = o.cnt;
-  double acc[kFuse + 1];
+  double acc[K + 1];
 #pragma unroll
-  for (int g = 0; g <= kFuse; ++g) acc[g] = 0.0;
+  for (int g = 0; g <= K; ++g) acc[g] = 0.0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
-    double xv[kFuse];
+    double xv[K];
 #pragma unroll
-    for (int g = 0; g < kFuse; ++g) xv[g] = g < cnt ? o.X[(int64_t)g * o.ldx + e] : 0.0;
+    for (int g = 0; g < K; ++g) xv[g] = g < cnt ? o.X[(int64_t)g * o.ldx + e] : 0.0;
     double x = o.w[e];
 #pragma unroll
-    for (int g = 0; g < kFuse; ++g) x = fma(-s_h[g], xv[g], x);
+    for (int g = 0; g < K; ++g) x = fma(-s_h[g], xv[g], x);
     o.w[e] = x;
 #pragma unroll
-    for (int g = 0; g < kFuse; ++g) acc[g] = fma(xv[g], x, acc[g]);
-    acc[kFuse] = fma(x, x, acc[kFuse]);
+    for (int g = 0; g < K; ++g) acc[g] = fma(xv[g], x, acc[g]);
+    acc[K] = fma(x, x, acc[K]);
   }
   // slot layout: projections 0..cnt-1, squared norm at cnt
-  double v[kFuse + 1];
+  double v[K + 1];
 #pragma unroll
-  for (int g = 0; g <= kFuse; ++g) v[g] = g < cnt ? acc[g] : (g == cnt ? acc[kFuse] : 0.0);
-  block_store_many<kFuse + 1>(v, cnt + 1, s_part, partial + (size_t)c * kMaxBasis);
+  for (int g = 0; g <= K; ++g) v[g] = g < cnt ? acc[g] : (g == cnt ? acc[K] : 0.0);
+  block_store_many<K + 1>(v, cnt + 1, s_part, partial + (size_t)c * kMaxBasis);
 }
 
 // One block (four wavefronts) per process.  mode 0: h = its chunks'
@@ -1089,6 +1095,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   struct Placed {
     int kind, nops, nch;
     size_t ops_off, cj_off;
+    int maxcnt;  // largest projection count of the group's operations
   };
   std::vector<Placed> pl_pre, pl_post;
   size_t off = 0;
@@ -1109,9 +1116,12 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
     for (OpGroup& g : gs) {
       if (g.ops.empty()) continue;
-      int nch = 0;
-      for (const BOpHost& h : g.ops) nch += nchunks(h.n);
-      Placed p{g.kind, (int)g.ops.size(), nch, 0, 0};
+      int nch = 0, maxcnt = 0;
+      for (const BOpHost& h : g.ops) {
+        nch += nchunks(h.n);
+        maxcnt = std::max(maxcnt, h.ctl ? h.m : h.cnt);
+      }
+      Placed p{g.kind, (int)g.ops.size(), nch, 0, 0, maxcnt};
       p.ops_off = off;
       off = align16(off + g.ops.size() * sizeof(BOp));
       p.cj_off = off;
@@ -1260,17 +1270,19 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     for (const Placed& p : pls) {
       const BOp* dops = (const BOp*)(db + p.ops_off);
       const int* dcj = (const int*)(db + p.cj_off);
+      auto dots = bdots_kernel<kHGroup>;
       if (p.kind == G_PROJECT) {
-        hipLaunchKernelGGL(bdots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(dots, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
         hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_REORTH) {
         // pass 1 projections (with the norm before) -> fused pass 1 update and
         // pass 2 projections (with the DGKS gate) -> pass 2 update where gated
-        hipLaunchKernelGGL(bdots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        hipLaunchKernelGGL(dots, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
-        hipLaunchKernelGGL(bupdate_dots_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
+        auto fused = p.maxcnt <= kFuseSmall ? bupdate_dots_kernel<kFuseSmall> : bupdate_dots_kernel<kFuse>;
+        hipLaunchKernelGGL(fused, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
         hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 1);
