@@ -674,6 +674,7 @@ struct BComb {
   int m, chunk0, nch, chunk;
 };
 
+template <int K>  // K >= rows of every combination in the launch
 __global__ __launch_bounds__(kOpBlock) void bcombine_kernel(const BComb* __restrict__ ops,
                                                             const int* __restrict__ cjob) {
   __shared__ double s_c[kMaxBasis * kMaxBasis];
@@ -692,13 +693,13 @@ __global__ __launch_bounds__(kOpBlock) void bcombine_kernel(const BComb* __restr
   const int64_t e0 = (int64_t)lc * o.chunk;
   const int64_t e1 = min(o.n, e0 + o.chunk);
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
-    double xv[kMaxBasis];
+    double xv[K];
 #pragma unroll
-    for (int i = 0; i < kMaxBasis; ++i) xv[i] = i < rows ? X[(int64_t)i * o.n + e] : 0.0;
+    for (int i = 0; i < K; ++i) xv[i] = i < rows ? X[(int64_t)i * o.n + e] : 0.0;
     for (int jj = 0; jj < cols; ++jj) {
       double y = 0.0;
 #pragma unroll
-      for (int i = 0; i < kMaxBasis; ++i)
+      for (int i = 0; i < K; ++i)
         if (i < rows) y = fma(s_c[jj * rows + i], xv[i], y);
       Y[(int64_t)jj * o.n + e] = y;
     }
@@ -1306,7 +1307,10 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   if (ncyc) {
     const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);
     hipLaunchKernelGGL(bcycle_kernel, dim3(ncyc), dim3(64), smem, stream, (const BCyc*)(db + cyc_off));
-    hipLaunchKernelGGL(bcombine_kernel, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
+    // the combination reads at most max_m + 1 basis vectors per element:
+    // registers for twelve of them when that suffices (occupancy)
+    auto comb = max_m + 1 <= kFuseSmall ? bcombine_kernel<kFuseSmall> : bcombine_kernel<kMaxBasis>;
+    hipLaunchKernelGGL(comb, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
                        (const int*)(db + comb_cj_off));
   }
   if (nread) {
