@@ -168,13 +168,16 @@ __global__ __launch_bounds__(kOpBlock) void bdots_kernel(const BOp* __restrict__
 #pragma unroll
     for (int g = 0; g < KG; ++g) acc[g] = 0.0;
     for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
+      // all loads of the element first, then the FMAs
       const double we = o.w[e];
+      double xv[KG];
 #pragma unroll
       for (int g = 0; g < KG; ++g) {
         const int i = i0 + g;
-        if (i < o.cnt) acc[g] = fma(o.X[(int64_t)i * o.ldx + e], we, acc[g]);
-        else if (i < tot) acc[g] = fma(we, we, acc[g]);
+        xv[g] = i < o.cnt ? o.X[(int64_t)i * o.ldx + e] : (i < tot ? we : 0.0);
       }
+#pragma unroll
+      for (int g = 0; g < KG; ++g) acc[g] = fma(xv[g], we, acc[g]);
     }
     block_store_many<KG>(acc, min(KG, tot - i0), s_part, partial + (size_t)c * kMaxBasis + i0);
   }
@@ -321,16 +324,17 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
 }
 
 // w[e] -= sum_i h[i] X_i[e]; partial[c * kMaxBasis] = sum of the new w[e]^2.
-// The projections are applied eight at a time, so eight loads per element
-// are in flight together.
+// The projections are applied KG at a time, so KG loads per element are in
+// flight together.
+template <int KG>
 __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict__ ops, const int* __restrict__ cjob,
                                                            double* __restrict__ partial, int gated) {
   __shared__ double s_red[8];
-  __shared__ double s_h[kMaxBasis + kHGroup];
+  __shared__ double s_h[kMaxBasis + KG];
   const int c = blockIdx.x;
   const BOp o = resolve(ops[cjob[c]]);
   if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
-  for (int i = threadIdx.x; i < kMaxBasis + kHGroup; i += blockDim.x) s_h[i] = i < o.cnt ? o.sc[i] : 0.0;
+  for (int i = threadIdx.x; i < kMaxBasis + KG; i += blockDim.x) s_h[i] = i < o.cnt ? o.sc[i] : 0.0;
   __syncthreads();
   const int lc = c - o.chunk0;
   const int64_t e0 = (int64_t)lc * o.chunk;
@@ -338,12 +342,12 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
   double nsq = 0.0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
     double x = o.w[e];
-    for (int i0 = 0; i0 < o.cnt; i0 += kHGroup) {
-      double xv[kHGroup];
+    for (int i0 = 0; i0 < o.cnt; i0 += KG) {
+      double xv[KG];
 #pragma unroll
-      for (int g = 0; g < kHGroup; ++g) xv[g] = i0 + g < o.cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
+      for (int g = 0; g < KG; ++g) xv[g] = i0 + g < o.cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
 #pragma unroll
-      for (int g = 0; g < kHGroup; ++g) x = fma(-s_h[i0 + g], xv[g], x);
+      for (int g = 0; g < KG; ++g) x = fma(-s_h[i0 + g], xv[g], x);
     }
     if (o.cnt) o.w[e] = x;
     nsq = fma(x, x, nsq);
@@ -1271,11 +1275,14 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     for (const Placed& p : pls) {
       const BOp* dops = (const BOp*)(db + p.ops_off);
       const int* dcj = (const int*)(db + p.cj_off);
-      auto dots = bdots_kernel<kHGroup>;
+      // projections (and the norm slot) in one pass over a chunk when they fit twelve registers
+      const bool small = p.maxcnt + 1 <= kFuseSmall;
+      auto dots = small ? bdots_kernel<kFuseSmall> : bdots_kernel<kHGroup>;
+      auto update = small ? bupdate_kernel<kFuseSmall> : bupdate_kernel<kHGroup>;
       if (p.kind == G_PROJECT) {
         hipLaunchKernelGGL(dots, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_REORTH) {
         // pass 1 projections (with the norm before) -> fused pass 1 update and
@@ -1285,15 +1292,15 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         auto fused = p.maxcnt <= kFuseSmall ? bupdate_dots_kernel<kFuseSmall> : bupdate_dots_kernel<kFuse>;
         hipLaunchKernelGGL(fused, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
+        hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 1);
       } else if (p.kind == G_LZ) {
         // alpha from the H*v's shares, w -= alpha v_k with |w|^2, beta_k and scales
         hipLaunchKernelGGL(blz_alpha_kernel, dim3(p.nops), dim3(256), 0, stream, dops);
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_NORM) {
-        hipLaunchKernelGGL(bupdate_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
+        hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
       } else {
         hipLaunchKernelGGL(bscale_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
