@@ -270,6 +270,20 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
         if (o.hsum) o.hsum[i] += t;
       }
     }
+    if (o.mode & OP_SCALE_NRM) {
+      // the second pass and the normalisation run as one update: its norm
+      // is |w'|^2 - |s|^2 (w' - X s with s = X^T w' and orthonormal X),
+      // accurate to rounding since |s| << |w'|
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double ss = 0.0;
+        for (int i = 0; i < o.cnt; ++i) ss = fma(o.sc[i], o.sc[i], ss);
+        const double n2 = o.sc[S_NRM] * o.sc[S_NRM] - ss;
+        const double nrm = sqrt(fmax(n2, 0.0));
+        o.sc[S_NRM] = nrm;
+        o.sc[S_NEGNRM] = -nrm;
+      }
+    }
     return;
   }
   if (mode == 0) {
@@ -354,6 +368,36 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
   }
   const double t = block_sum256(nsq, s_red);
   if (threadIdx.x == 0) partial[(size_t)c * kMaxBasis] = t;
+}
+
+// Second projection pass and normalisation in one update: w = (w - sum_i
+// s_i X_i) / nrm where the DGKS gate is set, else w = w / nrm (s and nrm
+// from breduce_kernel mode 2).
+template <int KG>
+__global__ __launch_bounds__(kOpBlock) void bupdate_scale_kernel(const BOp* __restrict__ ops,
+                                                                 const int* __restrict__ cjob) {
+  __shared__ double s_h[kMaxBasis + KG];
+  const int c = blockIdx.x;
+  const BOp o = resolve(ops[cjob[c]]);
+  if (op_skip(o)) return;
+  const int cnt = o.sc[S_GATE] != 0.0 ? o.cnt : 0;
+  for (int i = threadIdx.x; i < kMaxBasis + KG; i += blockDim.x) s_h[i] = i < cnt ? o.sc[i] : 0.0;
+  __syncthreads();
+  const double f = 1.0 / o.sc[S_NRM];
+  const int lc = c - o.chunk0;
+  const int64_t e0 = (int64_t)lc * o.chunk;
+  const int64_t e1 = min(o.n, e0 + o.chunk);
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
+    double x = o.w[e];
+    for (int i0 = 0; i0 < cnt; i0 += KG) {
+      double xv[KG];
+#pragma unroll
+      for (int g = 0; g < KG; ++g) xv[g] = i0 + g < cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
+#pragma unroll
+      for (int g = 0; g < KG; ++g) x = fma(-s_h[i0 + g], xv[g], x);
+    }
+    o.w[e] = x * f;
+  }
 }
 
 // w[e] *= 1 / nrm (OP_SCALE_NRM) or *= scale
@@ -1292,8 +1336,8 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         auto fused = p.maxcnt <= kFuseSmall ? bupdate_dots_kernel<kFuseSmall> : bupdate_dots_kernel<kFuse>;
         hipLaunchKernelGGL(fused, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
         hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
-        hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 1);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 1);
+        auto finish = small ? bupdate_scale_kernel<kFuseSmall> : bupdate_scale_kernel<kHGroup>;
+        hipLaunchKernelGGL(finish, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
       } else if (p.kind == G_LZ) {
         // alpha from the H*v's shares, w -= alpha v_k with |w|^2, beta_k and scales
         hipLaunchKernelGGL(blz_alpha_kernel, dim3(p.nops), dim3(256), 0, stream, dops);
@@ -1458,16 +1502,18 @@ int Batch::step() {
     o.hsum = j->sc + S_T;
     o.mode = OP_NB;
     if (j->m <= kFuse) {
-      rf.ops.push_back(o);  // both passes in one group (fused)
+      // both passes and the normalisation in one group (fused)
+      o.mode = OP_NB | OP_SCALE_NRM;
+      rf.ops.push_back(o);
     } else {
       p1.ops.push_back(o);
       o.mode = OP_GATED;  // (4) second pass only where the DGKS test asks for it
       p2.ops.push_back(o);
+      BOpHost t = o;
+      t.hsum = nullptr;
+      t.mode = OP_SCALE_NRM;
+      sc.ops.push_back(t);
     }
-    BOpHost t = o;
-    t.hsum = nullptr;
-    t.mode = OP_SCALE_NRM;
-    sc.ops.push_back(t);
   }
   for (BJob* j : lz) {
     BOpHost o;
