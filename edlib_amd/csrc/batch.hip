@@ -530,11 +530,12 @@ __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq
   for (int l = 0; l < n; ++l) {
     int iter = 0;
     for (;;) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        const double dd = fabs(lane_value(dr, m)) + fabs(lane_value(dr, m + 1));
-        if (fabs(lane_value(er, m)) <= eps * dd) break;
-      }
+      // m = the first index >= l whose off-diagonal is negligible (else
+      // n - 1): every lane tests its own, one ballot
+      const double dnext = __shfl_down(dr, 1, 64);
+      const bool split = lane < n - 1 && fabs(er) <= eps * (fabs(dr) + fabs(dnext));
+      const unsigned long long hits = __ballot(split) & (~0ull << l);
+      const int m = hits ? __ffsll((long long)hits) - 1 : n - 1;
       if (m == l) break;
       if (++iter > 200) return false;
       const double dl = lane_value(dr, l), el = lane_value(er, l);
@@ -543,6 +544,10 @@ __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq
       g = lane_value(dr, m) - dl + el / (g + copysign(r, g));
       double sn = 1.0, c = 1.0, pp = 0.0;
       bool early = false;
+      // the eigenvector rows: rotation i turns rows i and i + 1, and row
+      // i + 1 is the row that rotation i + 1 produced, so it is carried in a
+      // register instead of being read back
+      double carry = lane < n ? qt[m * n + lane] : 0.0;
       for (int i = m - 1; i >= l; --i) {
         const double ei = lane_value(er, i);
         const double f = sn * ei;
@@ -554,6 +559,7 @@ __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq
         if (r == 0.0) {
           if (lane == i + 1) dr -= pp;
           if (lane == m) er = 0.0;
+          if (lane < n) qt[(i + 1) * n + lane] = carry;
           early = true;
           break;
         }
@@ -566,13 +572,13 @@ __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq
         if (lane == i + 1) dr = g + pp;
         g = c * r - b;
         if (lane < n) {
-          const double qf = qt[(i + 1) * n + lane];
           const double q0 = qt[i * n + lane];
-          qt[(i + 1) * n + lane] = sn * q0 + c * qf;
-          qt[i * n + lane] = c * q0 - sn * qf;
+          qt[(i + 1) * n + lane] = sn * q0 + c * carry;
+          carry = c * q0 - sn * carry;
         }
       }
       if (early) continue;
+      if (lane < n) qt[l * n + lane] = carry;
       if (lane == l) {
         dr -= pp;
         er = g;
