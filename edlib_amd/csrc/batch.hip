@@ -1025,18 +1025,23 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
     j.state = JOB_DONE;
     return id;
   }
-  for (int q = 0; q < 3; ++q) j.lz[q] = dalloc(j, (size_t)j.n);
-  // scalars, alpha history (nl), beta history (nl + 1), stop flags, start
-  // vector's squared norm
-  j.sc = dalloc(j, S_T + 2 * (size_t)nl + 3);
-  fill_dev(j.sc, S_T + 2 * (size_t)nl + 3);
+  // one allocation per run (one stream-ordered free when it ends): the
+  // three vectors, the scalars (alpha history (nl), beta history (nl + 1),
+  // stop flags, start vector's squared norm) and the per-task shares of
+  // <v, w>, each 256-byte aligned
+  auto al = [](size_t c) { return (c + 31) & ~size_t(31); };
+  const size_t nv = al((size_t)j.n), nsc = S_T + 2 * (size_t)nl + 3;
+  j.ntdot = hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
+  double* base = dalloc(j, 3 * nv + al(nsc) + (size_t)j.ntdot);
+  for (int q = 0; q < 3; ++q) j.lz[q] = base + q * nv;
+  j.sc = base + 3 * nv;
+  j.tdot = j.sc + al(nsc);
+  fill_dev(j.sc, nsc);
   dev_fill(j.sc + S_LZV, 1, 1.0, stream);  // a given start vector is normalised
   j.d_alpha = j.sc + S_T;
   j.d_beta = j.d_alpha + nl;
   j.d_flags = (int*)(j.d_beta + nl + 1);
   dev_fill(j.d_alpha + 2 * (size_t)nl + 2, 1, 1.0, stream);  // squared start norm (set again for a gathered start)
-  j.ntdot = hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
-  j.tdot = dalloc(j, (size_t)j.ntdot);
   if (v0_dev)
     EDH_HIP(hipMemcpyAsync(j.lz[0], v0_dev, (size_t)j.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   j.step = 1;
