@@ -133,6 +133,15 @@ struct Batch {
   // their one allocation) and the host blob they were copied from
   std::map<std::tuple<int, int, int>, std::vector<void*>> maps;
   std::map<std::tuple<int, int, int>, std::vector<char>> map_blobs;
+  // stream-ordered frees not yet issued: a hipFreeAsync costs tens of
+  // microseconds of host time, so they are issued while the host would
+  // otherwise wait for the device (the memory is no longer referenced by
+  // anything issued after the logical free, so issuing later is safe)
+  std::vector<void*> pending_free;
+  void defer_free(void* p) {
+    if (p) pending_free.push_back(p);
+  }
+  void issue_frees_until(hipEvent_t ev);  // until ev completes (all when ev is null)
   DevBuf d_gather;
   DevBuf d_obs;  // static-observable scratch
   std::vector<ReadSlab> slabs;

@@ -800,6 +800,7 @@ Batch::~Batch() {
   for (auto& kv : maps) (void)hipFreeAsync(kv.second[4], stream);
   for (auto& s : secs)
     if (s) s->dev.release();
+  issue_frees_until(nullptr);
   if (stream) (void)hipStreamSynchronize(stream);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
   for (int i = 0; i < 2; ++i) {
@@ -822,8 +823,16 @@ double* Batch::dalloc(BJob& j, size_t count) {
   return (double*)p;
 }
 
+void Batch::issue_frees_until(hipEvent_t ev) {
+  while (!pending_free.empty()) {
+    if (ev && hipEventQuery(ev) == hipSuccess) return;
+    (void)hipFreeAsync(pending_free.back(), stream);
+    pending_free.pop_back();
+  }
+}
+
 void Batch::release_job(BJob& j) {
-  for (void* p : j.allocs) (void)hipFreeAsync(p, stream);
+  for (void* p : j.allocs) defer_free(p);
   j.allocs.clear();
   bytes_in_use -= j.bytes;
   j.bytes = 0;
@@ -857,7 +866,7 @@ void Batch::eig_status(BJob& j, const double* h) {
       j.allocs.erase(it);
       j.bytes -= bb;
       bytes_in_use -= bb;
-      (void)hipFreeAsync(b, stream);
+      defer_free(b);
     }
   }
   j.V = j.Valt = nullptr;
@@ -868,8 +877,12 @@ void Batch::eig_status(BJob& j, const double* h) {
 bool Batch::drain_slab(bool wait) {
   if (slab_fifo.empty()) return false;
   ReadSlab& r = slabs[slab_fifo.front()];
-  if (wait) EDH_HIP(hipEventSynchronize(r.ev));
-  else if (hipEventQuery(r.ev) != hipSuccess) return false;
+  if (wait) {
+    issue_frees_until(r.ev);
+    EDH_HIP(hipEventSynchronize(r.ev));
+  } else if (hipEventQuery(r.ev) != hipSuccess) {
+    return false;
+  }
   for (size_t q = 0; q < r.entries.size(); ++q) {
     const int id = r.entries[q].job;
     if (id < 0 || id >= (int)jobs.size() || !jobs[id]) continue;
@@ -918,7 +931,7 @@ void Batch::remove_sector(int id) {
   // earlier work (every reader of them) has run
   for (auto it = maps.begin(); it != maps.end();) {
     if (std::get<0>(it->first) == id) {
-      (void)hipFreeAsync(it->second[4], stream);
+      defer_free(it->second[4]);
       map_blobs.erase(it->first);  // its copy preceded every reader the host has seen finish
       it = maps.erase(it);
     } else {
@@ -926,6 +939,8 @@ void Batch::remove_sector(int id) {
     }
   }
   bytes_in_use -= s.bytes;
+  for (void* p : s.dev.allocs) defer_free(p);
+  s.dev.allocs.clear();
   s.dev.release();
   secs[id].reset();
 }
@@ -1229,7 +1244,10 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   // copy that last read it has completed
   const int si = stage_i;
   stage_i ^= 1;
-  if (ev_used[si]) EDH_HIP(hipEventSynchronize(ev_stage[si]));
+  if (ev_used[si]) {
+    issue_frees_until(ev_stage[si]);
+    EDH_HIP(hipEventSynchronize(ev_stage[si]));
+  }
   if (off > h_stage_bytes[si]) {
     if (h_stage[si]) EDH_HIP(hipHostFree(h_stage[si]));
     h_stage[si] = nullptr;
