@@ -124,6 +124,7 @@ struct Batch {
   std::vector<std::unique_ptr<BSector>> secs;
   std::vector<std::unique_ptr<BJob>> jobs;
   DevBuf d_arena, d_partial, d_scal;
+  UploadStage uploads;  // pinned staging of sector tables (no wait per sector)
   void* h_stage[2] = {nullptr, nullptr};
   size_t h_stage_bytes[2] = {0, 0};
   hipEvent_t ev_stage[2] = {nullptr, nullptr};

@@ -802,6 +802,7 @@ Batch::~Batch() {
     if (s) s->dev.release();
   issue_frees_until(nullptr);
   if (stream) (void)hipStreamSynchronize(stream);
+  uploads.release();
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);
@@ -906,7 +907,7 @@ int Batch::add_sector(const Model& m, int nup, int ndn) {
   auto s = std::make_unique<BSector>();
   s->model.reset(new Model(m));
   build_sector(*s->model, nup, ndn, s->tabs, false);
-  upload_sector(s->tabs, m.Ns, s->dev, stream);
+  upload_sector(s->tabs, m.Ns, s->dev, stream, &uploads);
   s->bytes = s->tabs.table_bytes;
   bytes_in_use += s->bytes;
   secs.push_back(std::move(s));

@@ -200,7 +200,27 @@ struct DevSector {
   void release();
 };
 
-void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s);
+// Pinned staging for table uploads that must not wait for the stream:
+// chunks of pinned host memory, each reusable once the copies out of it
+// (marked by its event) have completed.
+struct UploadStage {
+  struct Chunk {
+    char* host = nullptr;
+    size_t cap = 0, used = 0;
+    hipEvent_t ev = nullptr;
+  };
+  std::vector<Chunk> chunks;
+  // pinned space for `bytes`; the caller copies out of it on stream s and
+  // calls mark(c, s) after that copy was issued
+  char* take(size_t bytes, int& c);
+  void mark(int c, hipStream_t s);
+  void release();  // after the stream has finished every copy
+};
+
+// Uploads a sector's tables in one allocation and one copy.  Without a
+// stage the call waits for the copy (and then times the H*v variants);
+// with one it returns at once and the variants are not timed.
+void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s, UploadStage* stage = nullptr);
 
 // Keeps freed stream-ordered allocations in the device's default pool (no
 // return to the driver at synchronisations), so that allocation in the

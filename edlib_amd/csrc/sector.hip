@@ -24,15 +24,24 @@ class Packer {
     std::memcpy(blob_.data() + off, h.data(), h.size() * sizeof(T));
     set_.push_back([&field, off](char* base) { field = reinterpret_cast<T*>(base + off); });
   }
-  // The blob must stay alive until the copy has completed (the caller
-  // synchronises the stream).
-  void commit(DevSector& ds, hipStream_t s) {
+  // Without a stage the blob must stay alive until the copy has completed
+  // (the caller synchronises the stream); with one the copy reads pinned
+  // staging memory that the stage keeps until the copy is done.
+  void commit(DevSector& ds, hipStream_t s, UploadStage* stage) {
     if (blob_.empty()) return;
     void* p = nullptr;
     hipError_t e = hipMallocAsync(&p, blob_.size(), s);
     if (e != hipSuccess) throw Error(EDH_ERR_NOMEM, std::string("hipMallocAsync failed: ") + hipGetErrorString(e));
     ds.allocs.push_back(p);
-    EDH_HIP(hipMemcpyAsync(p, blob_.data(), blob_.size(), hipMemcpyHostToDevice, s));
+    if (stage) {
+      int c = -1;
+      char* h = stage->take(blob_.size(), c);
+      std::memcpy(h, blob_.data(), blob_.size());
+      EDH_HIP(hipMemcpyAsync(p, h, blob_.size(), hipMemcpyHostToDevice, s));
+      stage->mark(c, s);
+    } else {
+      EDH_HIP(hipMemcpyAsync(p, blob_.data(), blob_.size(), hipMemcpyHostToDevice, s));
+    }
     for (auto& f : set_) f(static_cast<char*>(p));
   }
 
@@ -112,7 +121,42 @@ void configure_mem_pool(int device) {
   done[device] = true;
 }
 
-void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s) {
+char* UploadStage::take(size_t bytes, int& c) {
+  // a chunk with room, else one whose copies have all completed, else a new one
+  for (size_t q = 0; q < chunks.size(); ++q)
+    if (chunks[q].cap - chunks[q].used >= bytes) {
+      c = (int)q;
+      char* h = chunks[q].host + chunks[q].used;
+      chunks[q].used = (chunks[q].used + bytes + 255) & ~size_t(255);
+      return h;
+    }
+  for (size_t q = 0; q < chunks.size(); ++q)
+    if (chunks[q].cap >= bytes && hipEventQuery(chunks[q].ev) == hipSuccess) {
+      c = (int)q;
+      chunks[q].used = (bytes + 255) & ~size_t(255);
+      return chunks[q].host;
+    }
+  Chunk k;
+  k.cap = std::max<size_t>(size_t(16) << 20, bytes);
+  EDH_HIP(hipHostMalloc((void**)&k.host, k.cap, hipHostMallocDefault));
+  EDH_HIP(hipEventCreateWithFlags(&k.ev, hipEventDisableTiming));
+  k.used = (bytes + 255) & ~size_t(255);
+  chunks.push_back(k);
+  c = (int)chunks.size() - 1;
+  return k.host;
+}
+
+void UploadStage::mark(int c, hipStream_t s) { EDH_HIP(hipEventRecord(chunks[c].ev, s)); }
+
+void UploadStage::release() {
+  for (Chunk& k : chunks) {
+    if (k.host) (void)hipHostFree(k.host);
+    if (k.ev) (void)hipEventDestroy(k.ev);
+  }
+  chunks.clear();
+}
+
+void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s, UploadStage* stage) {
   ds.release();
   ds.astream = s;
   ds.Ns = Ns;
@@ -148,7 +192,8 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s)
   }
   pk.add(ds.counters, std::vector<int32_t>(64, 0));
   pk.add(ds.row_order, rcm_order(st.up.rowp, st.up.col, st.up.D));
-  pk.commit(ds, s);
+  pk.commit(ds, s, stage);
+  if (stage) return;
   EDH_HIP(hipStreamSynchronize(s));
   ds.variant = hv_autotune(ds, s);
 }
