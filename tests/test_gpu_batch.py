@@ -154,3 +154,38 @@ def test_served_result_same_alone_and_in_mix():
     assert np.array_equal(alone["energies"], mix["energies"])
     assert np.array_equal(alone["G"], mix["G"])
     assert st["n"] == 3 and st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_vanishing_start_vector_has_no_run():
+    """A start vector c(+)|psi> of squared norm <= 1e-10 has no Lanczos run
+    and no Green's-function term (GreensFunction.h:408-418): with the
+    impurity decoupled and deep (mu = 10) its low-lying states are doubly
+    occupied, so c^+_0 vanishes on them while c_0 has norm 1.  Batched and
+    plain solves agree on which terms exist and on G."""
+    from edlib_amd.batch import Batch
+    from edlib_amd.solver import solve_batched, solve_plain
+    p = _siam(5)
+    p["Vk"][:] = 0.0
+    p["mu"] = 10.0
+    m = product_model("siam", p)
+    b = Batch(0)
+    sid, n = b.add_sector(m, 2, 2)
+    assert n == 100
+    jid = b.submit_eig(sid, 4, 11, 1e-14, 1000, SEED)
+    b.run()
+    up, _ = b.add_sector(m, 3, 2)
+    dn, _ = b.add_sector(m, 1, 2)
+    ids = b.submit_gf([(jid, 0, 0, 0, up), (jid, 0, 0, 1, dn)], 20)
+    assert min(ids) >= 0
+    b.run()
+    n2 = b.lanczos_norm2(ids)
+    assert n2[0] <= 1e-10, n2
+    assert abs(n2[1] - 1.0) < 1e-10, n2
+    assert b.lanczos_result(ids[0], 20)[0] == 0
+    assert b.lanczos_result(ids[1], 20)[0] > 0
+    b.close()
+    sp = _params(nlanc=40)
+    rb = solve_batched(m, sp)
+    rp = solve_plain(m, sp)
+    assert rb["n_gf"] == rp["n_gf"] > 0
+    assert np.abs(rb["G"] - rp["G"]).max() < 1e-8
