@@ -233,6 +233,17 @@ class GreensFunction : public Lanczos<Ham, Mesh> {
   const GFData& G_ij() const { return Gij_; }
   double statsum() const { return Z_; }
 
+  // GreensFunction::save (GreensFunction.h:167-194): G_omega (G_omega_r on
+  // the real axis), the statistical sum at path/@Statsum and G_ij_omega
+  // when orbital pairs are listed; Archive = EDLib::archive (HDF5.h).
+  template <class Archive>
+  void save(Archive& ar, const std::string& path) const {
+    const std::string sfx = this->mesh_.real ? "_r" : "";
+    if (!orbs_.empty()) ar.write_gf(path + "/G_omega" + sfx, G_, this->mesh_, FERMIONIC);
+    ar.write(path + "/@Statsum", Z_);
+    if (!pairs_.empty()) ar.write_gf(path + "/G_ij_omega" + sfx, Gij_, this->mesh_, FERMIONIC);
+  }
+
  private:
   // create_particles / annihilate_particles + lanczos + continued fraction
   // (GreensFunction.h:389-462, 241-282).
@@ -282,6 +293,7 @@ class ChiLoc : public Lanczos<Ham, Mesh> {
   }
 
   void compute(int op, const double* avg_ptr = nullptr) {
+    type_ = op == EDH_OP_SZ ? "Sz" : "N";  // SzOperator::name / NOperator::name (ChiLoc.h:65, 90)
     const int norb = model_.interacting_orbitals(), nw = this->mesh_.extent();
     const double avg = avg_ptr ? *avg_ptr : (op == EDH_OP_SZ ? 0.0 : 1.0);
     gf_.init(nw, norb, 1);
@@ -314,6 +326,15 @@ class ChiLoc : public Lanczos<Ham, Mesh> {
   }
   const GFData& G() const { return gf_; }
   const GFData& G_ij() const { return gfij_; }
+
+  // ChiLoc::save (ChiLoc.h:418-437): Chi<Sz|N>_omega of the last compute,
+  // path/@Statsum and Chi<Sz|N>_ij_omega when orbital pairs are listed.
+  template <class Archive>
+  void save(Archive& ar, const std::string& path) const {
+    if (!orbs_.empty()) ar.write_gf(path + "/Chi" + type_ + "_omega", gf_, this->mesh_, BOSONIC);
+    ar.write(path + "/@Statsum", Z_);
+    if (!pairs_.empty()) ar.write_gf(path + "/Chi" + type_ + "_ij_omega", gfij_, this->mesh_, BOSONIC);
+  }
 
  private:
   void contribute(const Pair& pr, const Pair& gs, int op, const std::vector<int>& orbs, bool need_norm, GFData& g,
@@ -351,6 +372,7 @@ class ChiLoc : public Lanczos<Ham, Mesh> {
     G(0, i) -= 2 * s + 2 * t1 - avg * avg * beta;
   }
   typename Ham::ModelType& model_;
+  std::string type_ = "Sz";
   double cutoff_, Z_ = 0.0;
   std::vector<size_t> orbs_;
   std::vector<std::array<size_t, 2>> pairs_;

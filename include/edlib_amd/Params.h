@@ -1,11 +1,17 @@
 // Parameter object with the key/default semantics the reference takes from
 // alps::params (EDParams.h:11-37), plus a reader for the model input files.
 //
-// Input files hold the datasets of the reference's input.h5 schema as JSON:
+// Input files hold the datasets of the reference's input.h5 schema
+// (HubbardModel.h:55-66, SingleImpurityAndersonModel.h:199-222,
+// SzSymmetry.h:77-86): either the HDF5 file itself (a name ending in .h5;
+// needs EDLIB_AMD_HDF5 and the HDF5 C library) or the same datasets as JSON:
 //   {"hopping/values": {"shape": [Ns, Ns], "data": [...]}, "mu": {"shape": [], "data": [x]}, ...}
-// (HubbardModel.h:55-66, SingleImpurityAndersonModel.h:199-222, SzSymmetry.h:77-86).
 #ifndef EDLIB_AMD_PARAMS_H
 #define EDLIB_AMD_PARAMS_H
+
+#ifdef EDLIB_AMD_HDF5
+#include <hdf5.h>
+#endif
 
 #include <cctype>
 #include <cstdlib>
@@ -157,9 +163,62 @@ struct dataset {
   std::vector<double> data;
 };
 
+#ifdef EDLIB_AMD_HDF5
+// Every integer or floating-point dataset of an HDF5 file, read as doubles
+// with its shape, under its path relative to the root ("hopping/values").
+inline std::map<std::string, dataset> read_h5(const std::string& path) {
+  std::map<std::string, dataset> out;
+  const hid_t f = H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+  if (f < 0) throw std::runtime_error("cannot open HDF5 file " + path);
+  struct Ctx {
+    std::map<std::string, dataset>* out;
+    bool ok;
+  } ctx{&out, true};
+  auto visit = [](hid_t obj, const char* name, const H5O_info_t* info, void* op) -> herr_t {
+    Ctx& c = *static_cast<Ctx*>(op);
+    if (info->type != H5O_TYPE_DATASET) return 0;
+    const hid_t d = H5Dopen2(obj, name, H5P_DEFAULT);
+    if (d < 0) return 0;
+    const hid_t t = H5Dget_type(d);
+    const H5T_class_t cls = H5Tget_class(t);
+    if (cls == H5T_INTEGER || cls == H5T_FLOAT) {
+      const hid_t sp = H5Dget_space(d);
+      const int nd = H5Sget_simple_extent_ndims(sp);
+      std::vector<hsize_t> dims(nd > 0 ? nd : 0);
+      if (nd > 0) H5Sget_simple_extent_dims(sp, dims.data(), nullptr);
+      dataset ds;
+      size_t n = 1;
+      for (hsize_t x : dims) {
+        ds.shape.push_back((size_t)x);
+        n *= (size_t)x;
+      }
+      ds.data.assign(n, 0.0);
+      if (n && H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, ds.data.data()) < 0) c.ok = false;
+      (*c.out)[name] = ds;
+      H5Sclose(sp);
+    }
+    H5Tclose(t);
+    H5Dclose(d);
+    return 0;
+  };
+  const herr_t r = H5Ovisit(f, H5_INDEX_NAME, H5_ITER_NATIVE, visit, &ctx);
+  H5Fclose(f);
+  if (r < 0 || !ctx.ok) throw std::runtime_error("cannot read the datasets of " + path);
+  return out;
+}
+#endif
+
 class input_file {
  public:
   explicit input_file(const std::string& path) {
+    if (path.size() > 3 && path.compare(path.size() - 3, 3, ".h5") == 0) {
+#ifdef EDLIB_AMD_HDF5
+      sets_ = read_h5(path);
+      return;
+#else
+      throw std::runtime_error("input file " + path + ": built without HDF5 (define EDLIB_AMD_HDF5, link -lhdf5)");
+#endif
+    }
     std::ifstream in(path);
     if (!in) throw std::runtime_error("cannot open input file " + path);
     std::stringstream ss;
@@ -169,6 +228,7 @@ class input_file {
     parse_top();
   }
   bool is_data(const std::string& k) const { return sets_.count(k) != 0; }
+  const std::map<std::string, dataset>& datasets() const { return sets_; }
   const dataset& get(const std::string& k) const {
     auto it = sets_.find(k);
     if (it == sets_.end()) throw std::runtime_error("input dataset " + k + " not found");

@@ -5,7 +5,9 @@ reference code is imported or run) and golden output files, and writes:
   * GF_Chi_input.json, 4ring_input.json, 4ring_nomagfield_input.json
     (test/input/*/input.h5 as JSON: hopping, interaction, chemical potential,
     magnetic field, sectors, GF/Chi orbital lists);
-  * gom1.dat, xiats.dat, xiatd.dat copied verbatim (test/input/GF_Chi/).
+  * gom1.dat, xiats.dat, xiatd.dat copied verbatim (test/input/GF_Chi/);
+  * the three input.h5 files copied verbatim as <case>_input.h5 (read by the
+    drop-in's HDF5 reader, whose tests compare them with the JSON above).
 Usage: python tests/golden/make_fixtures.py [/root/reference]
 """
 import json
@@ -68,4 +70,6 @@ for src, dst in (("GF_Chi", "GF_Chi_input.json"), ("4ring", "4ring_input.json"),
         json.dump(d, f, indent=1, sort_keys=True)
 for name in ("gom1.dat", "xiats.dat", "xiatd.dat"):
     shutil.copyfile(os.path.join(REF, "test", "input", "GF_Chi", name), os.path.join(OUT, name))
+for case in ("4ring", "4ring_nomagfield", "GF_Chi"):
+    shutil.copyfile(os.path.join(REF, "test", "input", case, "input.h5"), os.path.join(OUT, case + "_input.h5"))
 print("fixtures written to", OUT)
