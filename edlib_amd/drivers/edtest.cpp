@@ -3,8 +3,12 @@
 //              chi_N vs xiatd.dat) with HIPSpinResolvedStorage in place of the
 //              CSR storage;
 //   4ring   -- test/HubbardModel_Test.cpp:27-71 (E0, ground sector, observables);
-//   4ring_nomag -- test/StaticObservables_Test.cpp:27-72.
+//   4ring_nomag -- test/StaticObservables_Test.cpp:27-72;
+//   dyson_hubbard0, dyson_hubbard, dyson_siam0 -- the self-energy from the
+//              Dyson equation (GreensFunction.h:202-223), see dyson().
 // Usage: edtest <case> <golden dir>.  Prints one JSON line; exit code 0 on pass.
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -105,6 +109,107 @@ static int ring4(const std::string& dir, bool mag) {
   return ok ? 0 : 1;
 }
 
+// A JSON input file (the layout of tests/golden/*_input.json) under /tmp.
+static std::string write_input(const std::string& tag,
+                               const std::vector<std::pair<std::string, std::pair<std::vector<size_t>, std::vector<double>>>>& sets) {
+  const std::string path = "/tmp/edtest_" + tag + "_" + std::to_string((long long)getpid()) + ".json";
+  std::ofstream f(path);
+  f.precision(17);
+  f << "{";
+  for (size_t q = 0; q < sets.size(); ++q) {
+    f << (q ? ",\n" : "\n") << " \"" << sets[q].first << "\": {\"shape\": [";
+    for (size_t i = 0; i < sets[q].second.first.size(); ++i) f << (i ? ", " : "") << sets[q].second.first[i];
+    f << "], \"data\": [";
+    for (size_t i = 0; i < sets[q].second.second.size(); ++i) f << (i ? ", " : "") << sets[q].second.second[i];
+    f << "]}";
+  }
+  f << "\n}\n";
+  return path;
+}
+
+static std::vector<double> all_pairs(int n) {
+  std::vector<double> v;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      v.push_back(i);
+      v.push_back(j);
+    }
+  return v;
+}
+
+// Dyson equation (GreensFunction.h:202-223, FermionicModel.h:84-108):
+//   hubbard0 -- 4-site ring at U = 0, all orbital pairs: Sigma = 0;
+//   hubbard  -- 4-site ring, U = 5, mu = U/2: Sigma_ii(i w_max) -> U <n> = 2.5;
+//   siam0    -- Anderson impurity at U = 0: Sigma = -mu, because the
+//               reference's bare function 1/(z - H0 - Delta) has no mu
+//               (SingleImpurityAndersonModel.h:378-391).
+static int dyson(const std::string& which) {
+  params p;
+  define_parameters(p);
+  p["NSPINS"] = 2;
+  p["arpack.SECTOR"] = false;
+  p["storage.EIGENVALUES_ONLY"] = 0;
+  p["arpack.NEV"] = 100;
+  p["lanc.BETA"] = 10.0;
+  p["lanc.NOMEGA"] = 1000;
+  double maxdev = 0.0, tail = 0.0;
+  bool ok = true;
+  if (which == "siam0") {
+    const double mu = 1.3;
+    p["NSITES"] = 4;
+    p["siam.NORBITALS"] = 1;
+    p["INPUT_FILE"] = write_input(which, {{"Bath/Epsk/values", {{3, 2}, {-1.1, -1.1, 0.2, 0.2, 0.9, 0.9}}},
+                                          {"Bath/Vk_0/values", {{3, 2}, {0.5, 0.5, 0.3, 0.3, 0.4, 0.4}}},
+                                          {"H0_0/values", {{1, 2}, {0.1, 0.1}}},
+                                          {"interaction/values", {{2, 2, 1, 1, 1, 1}, {0, 0, 0, 0}}},
+                                          {"mu", {{}, {mu}}},
+                                          {"GreensFunction_orbitals/values", {{1, 2}, {0, 0}}}});
+    HIPSIAMHamiltonian ham(p);
+    ham.diag(false);
+    gf::MatsubaraMesh mesh(p, gf::FERMIONIC);
+    gf::GreensFunction<HIPSIAMHamiltonian> greens(p, ham, mesh);
+    greens.compute();
+    gf::GFData bare, sigma;
+    greens.selfenergy(bare, sigma);
+    for (int w = 0; w < sigma.nw; ++w)
+      for (int s = 0; s < 2; ++s) maxdev = std::max(maxdev, std::abs(sigma(w, 0, s) - std::complex<double>(-mu, 0.0)));
+    ok = maxdev < 1e-7;
+  } else {
+    const bool interacting = which == "hubbard";
+    const double U = interacting ? 5.0 : 0.0, mu = interacting ? 2.5 : 0.7;
+    std::vector<double> t(16, 0.0);
+    for (int i = 0; i < 4; ++i) t[i * 4 + (i + 1) % 4] = t[((i + 1) % 4) * 4 + i] = -1.0;
+    p["NSITES"] = 4;
+    p["INPUT_FILE"] = write_input(which, {{"hopping/values", {{4, 4}, t}},
+                                          {"interaction/values", {{4}, std::vector<double>(4, U)}},
+                                          {"chemical_potential/values", {{4}, std::vector<double>(4, mu)}},
+                                          {"GreensFunction_orbitals/values", {{16, 2}, all_pairs(4)}}});
+    HIPHubbardHamiltonian ham(p);
+    ham.diag(false);
+    gf::MatsubaraMesh mesh(p, gf::FERMIONIC);
+    gf::GreensFunction<HIPHubbardHamiltonian> greens(p, ham, mesh);
+    greens.compute();
+    gf::GFData bare, sigma;
+    greens.selfenergy(bare, sigma);
+    for (int w = 0; w < sigma.nw; ++w)
+      for (int k = 0; k < 16; ++k)
+        for (int s = 0; s < 2; ++s) maxdev = std::max(maxdev, std::abs(sigma(w, k, s)));
+    if (interacting) {
+      tail = sigma(sigma.nw - 1, 0, 0).real();
+      ok = std::abs(tail - U / 2) < 0.02;
+      for (int i = 0; i < 4; ++i)
+        for (int s = 0; s < 2; ++s)
+          ok = ok && std::abs(sigma(sigma.nw - 1, i * 4 + i, s).real() - tail) < 1e-6;
+    } else {
+      ok = maxdev < 1e-7;
+    }
+  }
+  std::remove(p["INPUT_FILE"].as<std::string>().c_str());
+  std::printf("{\"case\": \"dyson_%s\", \"max_abs_sigma_or_dev\": %.3e, \"sigma_tail\": %.6f, \"ok\": %s}\n",
+              which.c_str(), maxdev, tail, ok ? "true" : "false");
+  return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
   if (argc < 3) {
     std::fprintf(stderr, "usage: edtest <gfchi|4ring|4ring_nomag> <golden dir>\n");
@@ -115,6 +220,7 @@ int main(int argc, char** argv) {
     if (c == "gfchi") return gfchi(dir);
     if (c == "4ring") return ring4(dir, true);
     if (c == "4ring_nomag") return ring4(dir, false);
+    if (c.rfind("dyson_", 0) == 0) return dyson(c.substr(6));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 3;

@@ -233,6 +233,40 @@ class GreensFunction : public Lanczos<Ham, Mesh> {
   const GFData& G_ij() const { return Gij_; }
   double statsum() const { return Z_; }
 
+  // Bare Green's function of the model on this mesh in G_ij's layout
+  // (bare(w, I * norb + J, s)) and the self-energy from the Dyson equation,
+  // Sigma = G0^-1 - G_ij^-1 per frequency and spin (FermionicModel.h:84-108,
+  // GreensFunction.h:202-223).  G_ij holds the pairs listed in the input
+  // (all of them for a full self-energy).
+  void selfenergy(GFData& bare, GFData& sigma) const {
+    const int norb = model_.interacting_orbitals(), nw = this->mesh_.extent();
+    bare.init(nw, norb * norb, 2);
+    sigma.init(nw, norb * norb, 2);
+    std::vector<std::complex<double>> g0, g;
+    for (int w = 0; w < nw; ++w)
+      for (int s = 0; s < 2; ++s) {
+        model_.bare_gf_matrix(this->mesh_.freq_point(w), s, g0);
+        g.assign((size_t)norb * norb, 0.0);
+        for (int k = 0; k < norb * norb; ++k) {
+          bare(w, k, s) = g0[k];
+          g[k] = Gij_(w, k, s);
+        }
+        if (!complex_inverse(norb, g0) || !complex_inverse(norb, g))
+          throw std::runtime_error("Dyson equation: singular Green's function matrix");
+        for (int k = 0; k < norb * norb; ++k) sigma(w, k, s) = g0[k] - g[k];
+      }
+  }
+  // GreensFunction::compute_selfenergy (GreensFunction.h:202-223): stores
+  // G0_omega and Sigma_omega under path and returns the self-energy.
+  template <class Archive>
+  GFData compute_selfenergy(Archive& ar, const std::string& path) const {
+    GFData bare, sigma;
+    selfenergy(bare, sigma);
+    ar.write_gf(path + "/G0_omega", bare, this->mesh_, FERMIONIC);
+    ar.write_gf(path + "/Sigma_omega", sigma, this->mesh_, FERMIONIC);
+    return sigma;
+  }
+
   // GreensFunction::save (GreensFunction.h:167-194): G_omega (G_omega_r on
   // the real axis), the statistical sum at path/@Statsum and G_ij_omega
   // when orbital pairs are listed; Archive = EDLib::archive (HDF5.h).

@@ -4,6 +4,7 @@
 #ifndef EDLIB_AMD_MODELS_H
 #define EDLIB_AMD_MODELS_H
 
+#include <complex>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -18,6 +19,40 @@ namespace EDLib {
 
 inline void edh_throw(int code) {
   if (code != EDH_OK) throw std::runtime_error(std::string("edhip: ") + edh_last_error());
+}
+
+// In-place inverse of the n x n complex matrix a (row-major) by Gauss-Jordan
+// elimination with partial pivoting; false if a pivot vanishes.
+inline bool complex_inverse(int n, std::vector<std::complex<double>>& a) {
+  std::vector<std::complex<double>> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[(size_t)r * n + c]) > std::abs(a[(size_t)piv * n + c])) piv = r;
+    if (std::abs(a[(size_t)piv * n + c]) == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)piv * n + k], a[(size_t)c * n + k]);
+        std::swap(inv[(size_t)piv * n + k], inv[(size_t)c * n + k]);
+      }
+    const std::complex<double> d = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const std::complex<double> f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  a.swap(inv);
+  return true;
 }
 
 namespace Model {
@@ -75,6 +110,21 @@ class HubbardModel : public FermionicModel, public EngineModel {
     edh_model* m = nullptr;
     edh_throw(edh_model_hubbard(&m, Ns_, t.data(), U.data(), J.data(), xmu.data(), H.data()));
     adopt(m);
+    t_ = t;
+    xmu_ = xmu;
+  }
+  // Bare Green's function matrix at frequency z for spin s, g[I * Ns + J]
+  // (HubbardModel.h:181-201): [z + mu_I - eps_I + t_IJ]^-1 with the site
+  // energies eps_I = 0 (they are never read from input, HubbardModel.h:49)
+  // and mu indexed by site.
+  void bare_gf_matrix(std::complex<double> z, int /*s*/, std::vector<std::complex<double>>& g) const {
+    const int n = Ns_;
+    g.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      g[(size_t)i * n + i] = z + xmu_[i];
+      for (int j = 0; j < n; ++j) g[(size_t)i * n + j] += t_[(size_t)i * n + j];
+    }
+    if (!complex_inverse(n, g)) throw std::runtime_error("bare Green's function: singular matrix");
   }
   precision diagonal(long long state) const {
     double x = 0;
@@ -92,6 +142,7 @@ class HubbardModel : public FermionicModel, public EngineModel {
     return d.data;
   }
   Symmetry::SzSymmetry symmetry_;
+  std::vector<double> t_, xmu_;
 };
 
 template <typename prcsn>
@@ -134,6 +185,27 @@ class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
     edh_model* m = nullptr;
     edh_throw(edh_model_siam(&m, Ns_, ml_, e.data.data(), Vk.data(), H0.data(), U.data.data(), mu));
     adopt(m);
+    epsk_ = e.data;
+    vk_ = Vk;
+    h0_ = H0;
+  }
+  // Bare Green's function matrix of the impurity orbitals at frequency z for
+  // spin s, g[I * ml + J] (SingleImpurityAndersonModel.h:378-391):
+  // 1 / (z - H0_II - sum_k V_Ik^2 / (z - eps_k)) on the diagonal, as the
+  // reference computes it (without the chemical potential and without the
+  // off-diagonal H0); the reference defines it for one impurity orbital,
+  // the diagonal form is its extension to several.
+  void bare_gf_matrix(std::complex<double> z, int s, std::vector<std::complex<double>>& g) const {
+    const int nb = Ns_ - ml_;
+    g.assign((size_t)ml_ * ml_, 0.0);
+    for (int i = 0; i < ml_; ++i) {
+      std::complex<double> delta = 0.0;
+      for (int k = 0; k < nb; ++k) {
+        const double v = vk_[((size_t)i * nb + k) * 2 + s];
+        delta += v * v / (z - epsk_[(size_t)k * 2 + s]);
+      }
+      g[(size_t)i * ml_ + i] = 1.0 / (z - h0_[((size_t)i * ml_ + i) * 2 + s] - delta);
+    }
   }
   precision diagonal(long long state) const {
     double x = 0;
@@ -147,6 +219,7 @@ class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
  private:
   Symmetry::SzSymmetry symmetry_;
   int ml_;
+  std::vector<double> epsk_, vk_, h0_;
 };
 
 }  // namespace Model

@@ -1,7 +1,10 @@
 """The C++ drop-in (HIPSpinResolvedStorage in Hamiltonian, GreensFunction,
 ChiLoc, StaticObservables) run like the reference's own tests:
 test/Lanczos_Test.cpp (G to 1e-10, chi to 1e-9), test/HubbardModel_Test.cpp,
-test/StaticObservables_Test.cpp."""
+test/StaticObservables_Test.cpp; the self-energy from the Dyson equation
+against what the mathematics fixes (Sigma = 0 at U = 0 with every orbital
+pair; Sigma(i w) -> U <n> at high frequency; Sigma = -mu for the Anderson
+model at U = 0, the reference's bare function having no mu)."""
 import json
 import os
 import subprocess
@@ -15,7 +18,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("case", ["gfchi", "4ring", "4ring_nomag"])
+@pytest.mark.parametrize("case", ["gfchi", "4ring", "4ring_nomag", "dyson_hubbard0", "dyson_hubbard", "dyson_siam0"])
 def test_dropin_reference_tests(case):
     assert os.path.exists(BIN), "edtest driver not built (run __graft_entry__.build())"
     r = subprocess.run([BIN, case, GOLDEN], capture_output=True, text=True, timeout=120)
