@@ -43,6 +43,19 @@ class Batch:
         check(_lib.lib().edh_batch_add_sector(self._h, model.handle, nup, ndn, ctypes.byref(sid), ctypes.byref(dim)))
         return sid.value, dim.value
 
+    def add_sectors(self, model, secs):
+        """Many sectors of one model at once (tables built on several host
+        threads); returns (ids, dims) in the order of secs."""
+        k = len(secs)
+        nu = np.ascontiguousarray([s[0] for s in secs], dtype=np.int32)
+        nd = np.ascontiguousarray([s[1] for s in secs], dtype=np.int32)
+        ids = np.zeros(max(k, 1), dtype=np.int32)
+        dims = np.zeros(max(k, 1), dtype=np.int64)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        check(_lib.lib().edh_batch_add_sectors(self._h, model.handle, k, ip(nu), ip(nd), ip(ids),
+                                               dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return ids[:k].tolist(), dims[:k].tolist()
+
     def remove_sector(self, sid):
         check(_lib.lib().edh_batch_remove_sector(self._h, sid))
 

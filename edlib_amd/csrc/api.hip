@@ -750,6 +750,21 @@ int edh_batch_add_sector(edh_batch* b, const edh_model* m, int nup, int ndn, int
   API_END
 }
 
+int edh_batch_add_sectors(edh_batch* b, const edh_model* m, int count, const int* nups, const int* ndns, int* sec_ids,
+                          int64_t* dims) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(m && count >= 0 && (count == 0 || (nups && ndns && sec_ids)), EDH_ERR_ARG, "null argument");
+  std::vector<std::pair<int, int>> list(count);
+  for (int q = 0; q < count; ++q) list[q] = {nups[q], ndns[q]};
+  const std::vector<int> ids = B.add_sectors(*m->m, list);
+  for (int q = 0; q < count; ++q) {
+    sec_ids[q] = ids[q];
+    if (dims) dims[q] = B.sector(ids[q]).tabs.N;
+  }
+  API_END
+}
+
 int edh_batch_remove_sector(edh_batch* b, int sec_id) {
   API_BEGIN
   use_batch(b).remove_sector(sec_id);

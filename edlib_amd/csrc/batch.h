@@ -154,6 +154,9 @@ struct Batch {
   ~Batch();
 
   int add_sector(const Model& m, int nup, int ndn);
+  // many sectors of one model: tables built on up to 16 host threads, then
+  // uploaded in order; ids in the order of `list`
+  std::vector<int> add_sectors(const Model& m, const std::vector<std::pair<int, int>>& list);
   void remove_sector(int id);
   BSector& sector(int id);
   BJob& job(int id);

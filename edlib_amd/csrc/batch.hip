@@ -23,11 +23,13 @@
 // logic (projected matrix, restarts) sees only its own numbers.  A result
 // is therefore bit-identical alone and in any mix.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 
 #include "batch.h"
 
@@ -912,6 +914,41 @@ int Batch::add_sector(const Model& m, int nup, int ndn) {
   bytes_in_use += s->bytes;
   secs.push_back(std::move(s));
   return (int)secs.size() - 1;
+}
+
+std::vector<int> Batch::add_sectors(const Model& m, const std::vector<std::pair<int, int>>& list) {
+  const size_t n = list.size();
+  std::vector<std::unique_ptr<BSector>> built(n);
+  std::vector<std::exception_ptr> err(n);
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (size_t i; (i = next++) < n;) {
+      try {
+        auto s = std::make_unique<BSector>();
+        s->model.reset(new Model(m));
+        build_sector(*s->model, list[i].first, list[i].second, s->tabs, false);
+        built[i] = std::move(s);
+      } catch (...) {
+        err[i] = std::current_exception();
+      }
+    }
+  };
+  const size_t nt = std::min<size_t>({n, 16, std::max(1u, std::thread::hardware_concurrency())});
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
+  work();
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+  std::vector<int> ids;
+  for (auto& s : built) {
+    upload_sector(s->tabs, m.Ns, s->dev, stream, &uploads);
+    s->bytes = s->tabs.table_bytes;
+    bytes_in_use += s->bytes;
+    secs.push_back(std::move(s));
+    ids.push_back((int)secs.size() - 1);
+  }
+  return ids;
 }
 
 BSector& Batch::sector(int id) {

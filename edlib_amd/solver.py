@@ -208,8 +208,8 @@ class BatchedSolve:
         self.window = math.inf if c <= 0 else (-math.log(c) / p.beta) * (1 + 1e-9) + 1e-9
 
     def start(self):
-        for sec in self.secs:
-            self.sid[sec], _ = self.b.add_sector(self.model, *sec)
+        ids, _ = self.b.add_sectors(self.model, self.secs)
+        self.sid = dict(zip(self.secs, ids))
         for i, sec in enumerate(sector_list(self.Ns, self.p)):
             if sec in self.sid:
                 self.eig_jobs.append((sec, self.b.submit_eig(self.sid[sec], self.p.nev, self.p.ncv_eff, self.p.tol,

@@ -204,6 +204,11 @@ int edh_batch_destroy(edh_batch* b);
 /* Builds the tables of sector (nup, ndn) of model m on the device
  * (SpinResolvedStorage::fill for that sector); returns its id and dimension. */
 int edh_batch_add_sector(edh_batch* b, const edh_model* m, int nup, int ndn, int* sec_id, int64_t* dim);
+/* edh_batch_add_sector for `count` sectors (nups[q], ndns[q]) of one model:
+ * the tables are built on several host threads, so admitting an impurity
+ * solve's sectors costs less host time. */
+int edh_batch_add_sectors(edh_batch* b, const edh_model* m, int count, const int* nups, const int* ndns, int* sec_ids,
+                          int64_t* dims);
 int edh_batch_remove_sector(edh_batch* b, int sec_id);
 /* Eigensolver run on a sector (Storage::diag, Storage.h:43-146, as edh_eigsh);
  * trivial sectors (n <= 64 or ncv >= n) are solved at submission. */
