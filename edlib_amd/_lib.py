@@ -77,6 +77,7 @@ _SIGS = {
     "edh_batch_submit_eig": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_uint64, _ip]),
     "edh_batch_submit_lanczos": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _ip]),
+    "edh_batch_static_observables_many": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int, _dp]),
     "edh_batch_add_sectors": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_int64)]),
     "edh_batch_submit_gf": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, ctypes.c_int, _ip]),
     "edh_batch_lanczos_norm2": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp]),

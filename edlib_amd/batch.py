@@ -150,6 +150,25 @@ class Batch:
         return {"N": out[0:k], "N_up": out[k:2 * k], "N_dn": out[2 * k:3 * k], "M": out[3 * k:4 * k],
                 "D_occ": out[4 * k:5 * k], "M_i M_j": out[5 * k:5 * k + k * k], "N_eff": out[5 * k + k * k:]}
 
+    def static_observables_many(self, pairs, norb):
+        """static_observables for many (job id, index) pairs with one wait."""
+        k = len(pairs)
+        if k == 0:
+            return []
+        sz = 5 * norb + norb * norb + 1
+        out = np.zeros(k * sz)
+        ids = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+        idx = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        check(_lib.lib().edh_batch_static_observables_many(self._h, k, ip(ids), ip(idx), norb, _d(out)))
+        res = []
+        for q in range(k):
+            o = out[q * sz:(q + 1) * sz]
+            res.append({"N": o[0:norb], "N_up": o[norb:2 * norb], "N_dn": o[2 * norb:3 * norb],
+                        "M": o[3 * norb:4 * norb], "D_occ": o[4 * norb:5 * norb],
+                        "M_i M_j": o[5 * norb:5 * norb + norb * norb], "N_eff": o[5 * norb + norb * norb:]})
+        return res
+
     def lanczos_result(self, jid, nl):
         a, b = np.zeros(nl), np.zeros(nl + 1)
         nlanc = ctypes.c_int()

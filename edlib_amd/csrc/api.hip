@@ -895,6 +895,48 @@ int edh_batch_static_observables(edh_batch* b, int job_id, int index, int norb, 
   API_END
 }
 
+int edh_batch_static_observables_many(edh_batch* b, int count, const int* job_ids, const int* indices, int norb,
+                                      double* out) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(count >= 0 && (count == 0 || (job_ids && indices && out)), EDH_ERR_ARG, "null argument");
+  std::vector<size_t> soff(count + 1, 0), hoff(count + 1, 0);
+  for (int q = 0; q < count; ++q) {
+    edh::BJob& j = B.job(job_ids[q]);
+    EDH_CHECK(j.kind == edh::JOB_EIG && j.state == edh::JOB_DONE && j.evecs, EDH_ERR_STATE,
+              "batch: eigensolver job has no eigenvectors");
+    EDH_CHECK(indices[q] >= 0 && indices[q] < j.nev, EDH_ERR_ARG, "batch: eigenpair index out of range");
+    edh::BSector& s = B.sector(j.sec);
+    EDH_CHECK(norb >= 1 && norb <= s.model->Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
+    soff[q + 1] = soff[q] + ((edh::static_obs_scratch(s.tabs, s.model->Ns) + 31) & ~size_t(31));
+    hoff[q + 1] = hoff[q] + edh::static_obs_host(s.tabs, s.model->Ns);
+  }
+  if (count == 0) return EDH_OK;
+  B.d_obs.ensure(soff[count] * sizeof(double));
+  if (hoff[count] > B.h_obs_count) {
+    EDH_HIP(hipStreamSynchronize(B.stream));  // no copy into the old buffer is pending
+    if (B.h_obs) EDH_HIP(hipHostFree(B.h_obs));
+    B.h_obs = nullptr;
+    B.h_obs_count = 0;
+    EDH_HIP(hipHostMalloc((void**)&B.h_obs, hoff[count] * sizeof(double), hipHostMallocDefault));
+    B.h_obs_count = hoff[count];
+  }
+  for (int q = 0; q < count; ++q) {
+    edh::BJob& j = B.job(job_ids[q]);
+    edh::BSector& s = B.sector(j.sec);
+    edh::static_obs_launch(s.tabs, s.dev, s.model->Ns, j.evecs + (size_t)indices[q] * j.n, B.d_obs.as<double>() + soff[q],
+                           B.stream, B.h_obs + hoff[q]);
+  }
+  EDH_HIP(hipStreamSynchronize(B.stream));
+  const size_t osz = 5 * (size_t)norb + (size_t)norb * norb + 1;
+  for (int q = 0; q < count; ++q) {
+    edh::BJob& j = B.job(job_ids[q]);
+    edh::BSector& s = B.sector(j.sec);
+    edh::static_obs_reduce(s.tabs, s.model->Ns, norb, B.h_obs + hoff[q], out + q * osz);
+  }
+  API_END
+}
+
 int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc) {
   API_BEGIN
   edh::BJob& j = use_batch(b).job(job_id);

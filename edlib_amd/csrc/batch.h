@@ -145,6 +145,8 @@ struct Batch {
   void issue_frees_until(hipEvent_t ev);  // until ev completes (all when ev is null)
   DevBuf d_gather;
   DevBuf d_obs;  // static-observable scratch
+  double* h_obs = nullptr;  // pinned host copies of the static-observable sums
+  size_t h_obs_count = 0;
   std::vector<ReadSlab> slabs;
   std::vector<int> slab_fifo;  // slabs in flight, oldest first
 

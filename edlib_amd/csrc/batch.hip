@@ -805,6 +805,7 @@ Batch::~Batch() {
   issue_frees_until(nullptr);
   if (stream) (void)hipStreamSynchronize(stream);
   uploads.release();
+  if (h_obs) (void)hipHostFree(h_obs);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
   for (int i = 0; i < 2; ++i) {
     if (h_stage[i]) (void)hipHostFree(h_stage[i]);

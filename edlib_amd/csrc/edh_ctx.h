@@ -66,6 +66,15 @@ void static_obs_run(Ctx& c, const double* dv, int norb, double* out);
 // scratch and stream.
 void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
                        DevBuf& scratch, hipStream_t stream, double* out);
+// The same in two phases, for many vectors with one wait: the device sums
+// of one vector (static_obs_scratch(st, Ns) doubles of device scratch,
+// copied into static_obs_host(st, Ns) doubles of pinned host memory), then,
+// once the copies have arrived, the host reduction into out.
+size_t static_obs_scratch(const SectorTables& st, int Ns);
+size_t static_obs_host(const SectorTables& st, int Ns);
+void static_obs_launch(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, double* scratch,
+                       hipStream_t stream, double* host);
+void static_obs_reduce(const SectorTables& st, int Ns, int norb, const double* host, double* out);
 
 // Diagonal element of state (u, d) of the current sector from the host tables.
 double host_diag(const Model& m, const SectorTables& st, int64_t u, int64_t d);

@@ -105,16 +105,22 @@ void static_obs_run(Ctx& c, const double* dv, int norb, double* out) {
   static_obs_sector(c.tabs, c.dev, c.model->Ns, dv, norb, c.obs, c.stream, out);
 }
 
-void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
-                       DevBuf& scratch, hipStream_t stream, double* out) {
+size_t static_obs_scratch(const SectorTables& st, int Ns) {
+  const int64_t Dup = st.up.D, Ddn = st.dn.D;
+  const int64_t ntile = (Dup + kColTile - 1) / kColTile;
+  return (size_t)Dup * (Ns + 2) + (size_t)ntile * Ddn + (size_t)Ddn;
+}
+
+size_t static_obs_host(const SectorTables& st, int Ns) { return (size_t)st.up.D * (Ns + 2) + (size_t)st.dn.D; }
+
+void static_obs_launch(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, double* scratch,
+                       hipStream_t stream, double* host) {
   EDH_CHECK(Ns <= kObsMaxSites, EDH_ERR_LIMIT, "static observables: more than 32 sites per spin");
-  EDH_CHECK(norb >= 1 && norb <= Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
   const int64_t Dup = st.up.D, Ddn = st.dn.D;
   const int nout = Ns + 2;
   const int64_t ntile = (Dup + kColTile - 1) / kColTile;
   const size_t rows_n = (size_t)Dup * nout, part_n = (size_t)ntile * Ddn;
-  scratch.ensure((rows_n + part_n + (size_t)Ddn) * sizeof(double));
-  double* rows = scratch.as<double>();
+  double* rows = scratch;
   double* part = rows + rows_n;
   double* col = part + part_n;
   const int grid_rows = (int)std::min<int64_t>(Dup, 4096);
@@ -124,17 +130,33 @@ void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, cons
   hipLaunchKernelGGL(obs_cols_sum_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0,
                      stream, part, ntile, Ddn, col);
   EDH_HIP(hipGetLastError());
-  std::vector<double> hr(rows_n), hc(Ddn);
-  EDH_HIP(hipMemcpyAsync(hr.data(), rows, rows_n * sizeof(double), hipMemcpyDeviceToHost, stream));
-  EDH_HIP(hipMemcpyAsync(hc.data(), col, (size_t)Ddn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipMemcpyAsync(host, rows, rows_n * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipMemcpyAsync(host + rows_n, col, (size_t)Ddn * sizeof(double), hipMemcpyDeviceToHost, stream));
+}
+
+void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
+                       DevBuf& scratch, hipStream_t stream, double* out) {
+  EDH_CHECK(norb >= 1 && norb <= Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
+  scratch.ensure(static_obs_scratch(st, Ns) * sizeof(double));
+  std::vector<double> host(static_obs_host(st, Ns));
+  static_obs_launch(st, ds, Ns, dv, scratch.as<double>(), stream, host.data());
   EDH_HIP(hipStreamSynchronize(stream));
+  static_obs_reduce(st, Ns, norb, host.data(), out);
+}
+
+void static_obs_reduce(const SectorTables& st, int Ns, int norb, const double* host, double* out) {
+  EDH_CHECK(norb >= 1 && norb <= Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
+  const int64_t Dup = st.up.D, Ddn = st.dn.D;
+  const int nout = Ns + 2;
+  const double* hr = host;
+  const double* hc = host + (size_t)Dup * nout;
   // second moments of the occupation bits (bit b of a per-spin pattern is
   // site Ns - 1 - b, FermionicModel.h:35-68)
   std::vector<double> UU((size_t)Ns * Ns, 0.0), UD((size_t)Ns * Ns, 0.0), DD((size_t)Ns * Ns, 0.0);
   double sumsq = 0.0;
   for (int64_t u = 0; u < Dup; ++u) {
     const unsigned bu = (unsigned)st.up.bits[u];
-    const double* r = hr.data() + u * nout;
+    const double* r = hr + u * nout;
     const double R = r[Ns];
     sumsq += r[Ns + 1];
     for (int a = 0; a < Ns; ++a) {

@@ -298,10 +298,11 @@ class BatchedSolve:
                 for jid in live:
                     self.b.free_job(jid)
                 obs = _Observables()
-                for e, _, _, j, k in self.pairs:
-                    bf = math.exp(-(e - self.e0) * p.beta)
-                    if _passes(bf, p.cutoff):
-                        obs.add(self.b.static_observables(j, k, _interacting(self.model)), e, bf)
+                inside = [(e, j, k, math.exp(-(e - self.e0) * p.beta)) for e, _, _, j, k in self.pairs]
+                inside = [q for q in inside if _passes(q[3], p.cutoff)]
+                sums = self.b.static_observables_many([(j, k) for _, j, k, _ in inside], _interacting(self.model))
+                for (e, _, _, bf), o in zip(inside, sums):
+                    obs.add(o, e, bf)
                 for _, j in self.eig_jobs:
                     self.b.free_job(j)
                 for sid in self.sid.values():

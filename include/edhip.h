@@ -251,6 +251,11 @@ int edh_batch_eig_result(edh_batch* b, int job_id, double* evals, double* evecs,
 /* edh_static_observables for eigenpair `index` of a finished eigensolver
  * job (its vector stays on the device). */
 int edh_batch_static_observables(edh_batch* b, int job_id, int index, int norb, double* out);
+/* edh_batch_static_observables for `count` eigenpairs (job_ids[q],
+ * indices[q]) with one wait for the device; out holds count blocks of
+ * 5*norb + norb*norb + 1 values in the same layout. */
+int edh_batch_static_observables_many(edh_batch* b, int count, const int* job_ids, const int* indices, int norb,
+                                      double* out);
 /* alpha[nlanc], beta[nl+1] (beta[0] unused), as edh_lanczos. */
 int edh_batch_lanczos_result(edh_batch* b, int job_id, double* alpha, double* beta, int* nlanc);
 /* Stops a process (it takes no further sweeps; its state becomes
