@@ -479,6 +479,19 @@ __device__ __forceinline__ double lane_value(double x, int l) {
 // lanes, and lane r rotates row r of q (stored transposed, qt[c n + r]).  On
 // return w holds the eigenvalues and column j of a the eigenvector of w[j].
 // Returns false when QL does not converge.
+// 1/sqrt(x) for x > 0 (normal): the hardware estimate refined by two
+// Newton-Raphson steps y <- y + y (1 - x y^2) / 2, each of which doubles the
+// number of correct bits, to within an ulp or so.
+__device__ __forceinline__ double rsqrt_newton(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+
 __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq, double* w, int* order) {
   const int lane = threadIdx.x;
   for (int i = lane; i < n * n; i += 64) qt[i] = (i / n == i % n) ? 1.0 : 0.0;
@@ -554,18 +567,23 @@ __device__ bool wave_sym_eig(int n, double* a, double* qt, double* v, double* qq
         const double ei = lane_value(er, i);
         const double f = sn * ei;
         const double b = c * ei;
-        // the entries are moderate (Ritz values and couplings of H), so the
-        // plain square root cannot overflow; one reciprocal serves s and c
-        r = sqrt(fma(f, f, g * g));
-        if (lane == i + 1) er = r;
-        if (r == 0.0) {
-          if (lane == i + 1) dr -= pp;
+        // the entries are moderate (Ritz values and couplings of H), so
+        // f^2 + g^2 cannot overflow; r and 1/r from one reciprocal square
+        // root (the rotation's latency chain is the QL sweep's cost)
+        const double x2 = fma(f, f, g * g);
+        if (x2 == 0.0) {
+          if (lane == i + 1) {
+            er = 0.0;
+            dr -= pp;
+          }
           if (lane == m) er = 0.0;
           if (lane < n) qt[(i + 1) * n + lane] = carry;
           early = true;
           break;
         }
-        const double rinv = 1.0 / r;
+        const double rinv = rsqrt_newton(x2);
+        r = x2 * rinv;
+        if (lane == i + 1) er = r;
         sn = f * rinv;
         c = g * rinv;
         g = lane_value(dr, i + 1) - pp;
