@@ -181,6 +181,16 @@ class Batch:
     def free_job(self, jid):
         check(_lib.lib().edh_batch_free_job(self._h, jid))
 
+    def set_graphs(self, on):
+        """Steady sweeps replay a captured hipGraph (default on)."""
+        check(_lib.lib().edh_batch_set_graphs(self._h, 1 if on else 0))
+
+    def sweep_counts(self):
+        """(sweeps, sweeps run as graph replays)."""
+        a, g = ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.lib().edh_batch_sweeps(self._h, ctypes.byref(a), ctypes.byref(g)))
+        return a.value, g.value
+
     def memory(self):
         b = ctypes.c_size_t()
         check(_lib.lib().edh_batch_memory(self._h, ctypes.byref(b)))

@@ -967,6 +967,22 @@ int edh_batch_memory(edh_batch* b, size_t* bytes) {
   API_END
 }
 
+int edh_batch_set_graphs(edh_batch* b, int on) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  B.use_graphs = on != 0;
+  if (!B.use_graphs) B.drop_graph();
+  API_END
+}
+
+int edh_batch_sweeps(edh_batch* b, int64_t* sweeps, int64_t* graph_sweeps) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  if (sweeps) *sweeps = (int64_t)B.sweeps;
+  if (graph_sweeps) *graph_sweeps = (int64_t)B.graph_sweeps;
+  API_END
+}
+
 int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block) {
   API_BEGIN
   edh::Batch& B = use_batch(b);

@@ -152,6 +152,19 @@ struct Batch {
 
   size_t bytes_in_use = 0;
   int hv_block = 256;
+  // Steady sweeps (the same descriptors as the sweep before: eigensolver
+  // runs only, no process started or finished) replay one captured graph
+  // of the sweep's kernels and skip the descriptor upload.
+  bool use_graphs = true;
+  std::vector<char> last_desc;     // descriptor bytes of the last sweep
+  std::vector<int64_t> last_cfg;   // and its launch configuration
+  const void* last_arena = nullptr;
+  hipGraphExec_t sweep_graph = nullptr;
+  size_t graph_sweeps = 0;  // sweeps run as graph replays
+  void drop_graph() {
+    if (sweep_graph) (void)hipGraphExecDestroy(sweep_graph);
+    sweep_graph = nullptr;
+  }
   long long sweeps = 0;
   ~Batch();
 

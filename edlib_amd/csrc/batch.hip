@@ -822,6 +822,7 @@ Batch::~Batch() {
     if (s) s->dev.release();
   issue_frees_until(nullptr);
   if (stream) (void)hipStreamSynchronize(stream);
+  drop_graph();
   uploads.release();
   if (h_obs) (void)hipHostFree(h_obs);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
@@ -1324,6 +1325,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       for (size_t i = 0; i < g.ops.size(); ++i) {
         const BOpHost& h = g.ops[i];
         BOp o;
+        std::memset(&o, 0, sizeof(o));  // padding too: steady sweeps compare the bytes
         o.X = h.X;
         o.ldx = h.ldx;
         o.cnt = h.cnt;
@@ -1365,10 +1367,19 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     int* cj = (int*)(hb + comb_cj_off);
     for (int q = 0; q < ncyc; ++q) {
       const BJob& j = *(*cycles)[q];
-      BCyc y{j.sc, j.ctl, j.m, j.nev, j.maxitr, j.tol, d_gather.as<double>() + soff};
+      BCyc y;
+      std::memset(&y, 0, sizeof(y));
+      y.sc = j.sc;
+      y.ctl = j.ctl;
+      y.m = j.m;
+      y.nev = j.nev;
+      y.maxitr = j.maxitr;
+      y.tol = j.tol;
+      y.status = d_gather.as<double>() + soff;
       soff += (*reads)[q].count;
       std::memcpy(hb + cyc_off + q * sizeof(BCyc), &y, sizeof(BCyc));
       BComb b;
+      std::memset(&b, 0, sizeof(b));
       b.X0 = j.V;
       b.X1 = j.Valt;
       b.evecs = j.evecs;
@@ -1385,7 +1396,11 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     int64_t roff = 0;
     for (int q = 0; q < nread; ++q) {
       if (q >= ncyc) {
-        BGather g{(*reads)[q].src, roff, (int)(*reads)[q].count};
+        BGather g;
+        std::memset(&g, 0, sizeof(g));
+        g.src = (*reads)[q].src;
+        g.off = roff;
+        g.count = (int)(*reads)[q].count;
         std::memcpy(hb + gat_off + (q - ncyc) * sizeof(BGather), &g, sizeof(BGather));
       }
       roff += (int64_t)(*reads)[q].count;
@@ -1396,9 +1411,26 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     for (const Placed& p : *v) max_ch = std::max(max_ch, p.nch);
   d_arena.ensure(off);
   d_partial.ensure((size_t)max_ch * kMaxBasis * sizeof(double));
-  EDH_HIP(hipMemcpyAsync(d_arena.p, hb, off, hipMemcpyHostToDevice, stream));
-  EDH_HIP(hipEventRecord(ev_stage[si], stream));
-  ev_used[si] = true;
+  // launch configuration beside the descriptor bytes: together they fix
+  // every kernel and argument of the sweep
+  std::vector<int64_t> cfg{(int64_t)off, hv_block, (int64_t)units.size(), ncyc, comb_nch, max_m, nread,
+                           (int64_t)read_total, (int64_t)(uintptr_t)d_partial.p, (int64_t)(uintptr_t)d_gather.p};
+  for (auto* v : {&pl_pre, &pl_post})
+    for (const Placed& p : *v)
+      for (int64_t x : {(int64_t)p.kind, (int64_t)p.nops, (int64_t)p.nch, (int64_t)p.ops_off, (int64_t)p.cj_off,
+                        (int64_t)p.maxcnt})
+        cfg.push_back(x);
+  const bool steady = use_graphs && d_arena.p == last_arena && cfg == last_cfg && last_desc.size() == off &&
+                      std::memcmp(last_desc.data(), hb, off) == 0;
+  if (!steady) {
+    drop_graph();
+    EDH_HIP(hipMemcpyAsync(d_arena.p, hb, off, hipMemcpyHostToDevice, stream));
+    EDH_HIP(hipEventRecord(ev_stage[si], stream));
+    ev_used[si] = true;
+    last_desc.assign(hb, hb + off);
+    last_cfg = cfg;
+    last_arena = d_arena.p;
+  }
   char* db = d_arena.as<char>();
   double* part = d_partial.as<double>();
   auto launch = [&](const std::vector<Placed>& pls) {
@@ -1437,23 +1469,47 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
     }
   };
-  launch(pl_pre);
-  if (has_hv)
-    launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, stream);
-  launch(pl_post);
-  if (ncyc) {
-    const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);
-    hipLaunchKernelGGL(bcycle_kernel, dim3(ncyc), dim3(64), smem, stream, (const BCyc*)(db + cyc_off));
-    // the combination reads at most max_m + 1 basis vectors per element:
-    // registers for twelve of them when that suffices (occupancy)
-    auto comb = max_m + 1 <= kFuseSmall ? bcombine_kernel<kFuseSmall> : bcombine_kernel<kMaxBasis>;
-    hipLaunchKernelGGL(comb, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
-                       (const int*)(db + comb_cj_off));
-  }
-  if (nread) {
+  auto kernels = [&]() {
+    launch(pl_pre);
+    if (has_hv)
+      launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, stream);
+    launch(pl_post);
+    if (ncyc) {
+      const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);
+      hipLaunchKernelGGL(bcycle_kernel, dim3(ncyc), dim3(64), smem, stream, (const BCyc*)(db + cyc_off));
+      // the combination reads at most max_m + 1 basis vectors per element:
+      // registers for twelve of them when that suffices (occupancy)
+      auto comb = max_m + 1 <= kFuseSmall ? bcombine_kernel<kFuseSmall> : bcombine_kernel<kMaxBasis>;
+      hipLaunchKernelGGL(comb, dim3(comb_nch), dim3(kOpBlock), 0, stream, (const BComb*)(db + comb_ops_off),
+                         (const int*)(db + comb_cj_off));
+    }
     if (nread > ncyc)
       hipLaunchKernelGGL(bgather_kernel, dim3(nread - ncyc), dim3(256), 0, stream, (const BGather*)(db + gat_off),
                          d_gather.as<double>());
+  };
+  if (!steady) {
+    kernels();
+  } else {
+    if (!sweep_graph) {
+      // the second identical sweep in a row: capture its kernels once
+      hipGraph_t g = nullptr;
+      EDH_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      kernels();
+      EDH_HIP(hipStreamEndCapture(stream, &g));
+      const hipError_t e = hipGraphInstantiate(&sweep_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        sweep_graph = nullptr;
+        use_graphs = false;
+        kernels();
+      }
+    }
+    if (sweep_graph) {
+      EDH_HIP(hipGraphLaunch(sweep_graph, stream));
+      ++graph_sweeps;
+    }
+  }
+  if (nread) {
     // a free slab large enough, else a new one
     int si2 = -1;
     for (size_t q = 0; q < slabs.size(); ++q)

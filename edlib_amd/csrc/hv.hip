@@ -719,6 +719,7 @@ int slab_geometry(const DevSector& ds, int& nslabs, int& rows_per_wg, int& nrb) 
 
 HvParams make_params(const DevSector& ds, const HvArgs& a) {
   HvParams p;
+  std::memset(&p, 0, sizeof(p));  // padding too: batched sweeps compare descriptor bytes
   p.Ns = ds.Ns;
   p.nb = ds.Ns - ds.imp_ml;
   p.Dup = (int)ds.up.D;
@@ -850,6 +851,7 @@ size_t hv_batch_job_bytes() { return sizeof(HvBatchJob); }
 
 void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out) {
   HvBatchJob j;
+  std::memset(&j, 0, sizeof(j));
   j.p = make_params(ds, a);
   j.row0 = row0;
   j.cross = ds.cross_mode;

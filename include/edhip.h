@@ -264,6 +264,12 @@ int edh_batch_cancel(edh_batch* b, int job_id);
 int edh_batch_free_job(edh_batch* b, int job_id);
 /* Device bytes held by the batch's sectors and processes. */
 int edh_batch_memory(edh_batch* b, size_t* bytes);
+/* Steady sweeps -- the same descriptors as the sweep before (eigensolver
+ * runs only, nothing started or finished) -- replay one hipGraph of the
+ * sweep's kernels without re-uploading descriptors (on by default; 0 turns
+ * it off).  edh_batch_sweeps counts the sweeps and those run as replays. */
+int edh_batch_set_graphs(edh_batch* b, int on);
+int edh_batch_sweeps(edh_batch* b, int64_t* sweeps, int64_t* graph_sweeps);
 /* Autotuner: times the grouped H*v over the given sectors for each block
  * size and keeps the fastest (returned in *block). */
 int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block);
