@@ -116,6 +116,55 @@ def solves_section(args):
     return out
 
 
+def large_sector_section(model, v0buf_ctx, vb, args):
+    """The 16-site sector (8,8) end to end on the device: 300 Lanczos steps
+    from a random start (ground-state energy from the tridiagonal matrix),
+    then the ground state from the batched eigensolver and 4 batched
+    Green's-function runs (c+ and c of orbital 0, both spins) with their
+    continued fractions, every vector resident in HBM."""
+    from scipy.linalg import eigh_tridiagonal
+    from edlib_amd.batch import Batch
+    from edlib_amd.solver import matsubara
+    ctx = v0buf_ctx
+    nl = 300
+    nlanc, a, b, ms = ctx.lanczos_time(vb, nl)
+    e_lz = float(eigh_tridiagonal(a[:nlanc], b[1:nlanc])[0][0])
+    out = {"workload": "16-site 4x4 Hubbard U=6 mu=3, sector (8,8), N=165636900: %d Lanczos steps; ground state "
+                       "(NEV=1, NCV=20, tol 1e-10) and 4 Green's-function runs of NLANC=%d (c+, c on orbital 0, "
+                       "both spins; sectors (9,8), (8,9), (7,8), (8,7)) through the batched engine, 256 Matsubara "
+                       "points at beta=10" % (nl, args.gf_nlanc),
+           "lanczos_steps": nlanc, "lanczos_s": round(ms * nlanc * 1e-3, 3), "lanczos_ms_per_step": round(ms, 3),
+           "E0_lanczos": e_lz}
+    bt = Batch(0)
+    t0 = time.perf_counter()
+    sid, _ = bt.add_sector(model, 8, 8)
+    jid = bt.submit_eig(sid, 1, 20, 1e-10, 1000, SEED + 8)
+    bt.run()
+    ev, _ = bt.eig_result(jid)
+    t1 = time.perf_counter()
+    tg = {}
+    for sec in ((9, 8), (8, 9), (7, 8), (8, 7)):
+        tg[sec], _ = bt.add_sector(model, *sec)
+    reqs = [(jid, 0, 0, 0, tg[(9, 8)]), (jid, 0, 16, 0, tg[(8, 9)]), (jid, 0, 0, 1, tg[(7, 8)]),
+            (jid, 0, 16, 1, tg[(8, 7)])]
+    ids = bt.submit_gf(reqs, args.gf_nlanc)
+    bt.run()
+    n2 = bt.lanczos_norm2(ids)
+    wn = matsubara(10.0, 256)
+    g = bt.cf_many(ids, list(n2), [1, 1, -1, -1], [1.0] * 4, [float(ev[0])] * 4, wn)
+    t2 = time.perf_counter()
+    info = bt.job_info(jid)
+    gup, gdn = g[0] + g[2], g[1] + g[3]
+    # sum rule: i w G(i w) -> 1 (the spectral weight of each spin)
+    tail = max(abs(1j * wn[-1] * gup[-1] - 1.0), abs(1j * wn[-1] * gdn[-1] - 1.0))
+    bt.close()
+    out.update({"E0_eigensolver": float(ev[0]), "dE0_rel": abs(float(ev[0]) - e_lz) / abs(e_lz),
+                "eigensolver_s": round(t1 - t0, 3), "eigensolver_hv": info["n_av"],
+                "gf_s": round(t2 - t1, 3), "gf_weights": [round(float(x), 12) for x in n2],
+                "sum_rule_dev_at_w_max": float(tail), "spin_symmetry_dG": float(np.abs(gup - gdn).max())})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +175,8 @@ def main():
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--max-active", type=int, default=4)
+    ap.add_argument("--no-large", action="store_true", help="skip the 16-site Lanczos/eigensolver/GF section")
+    ap.add_argument("--gf-nlanc", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding)",
@@ -213,6 +264,9 @@ def main():
                                  "gbs": round(2 * 8 * n / (obs_ms * 1e-3) / 1e9, 1),
                                  "what": "N, N_up, N_dn, M, D_occ, M_i M_j, N_eff of one sector vector, host call "
                                          "including the small host reduction (2 reads of v)"}
+    if not args.no_large:
+        vb.upload(v)
+        out["large_sector"] = large_sector_section(ea.HubbardModel(Ns, t, U, xmu), ctx, vb, args)
     ctx.close()
     if not args.no_cpu_baseline:
         wcpu = w.copy()
