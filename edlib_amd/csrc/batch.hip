@@ -28,6 +28,7 @@
 #include <cstddef>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -806,6 +807,39 @@ int nchunks(int64_t n) { return (int)((n + chunk_of(n) - 1) / chunk_of(n)); }
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// Pinned status read-back buffers outlive their batch: a released buffer
+// is kept for the next batch of the process, since allocating and freeing
+// pinned memory costs about a millisecond per call.
+std::mutex g_pinned_mu;
+std::vector<std::pair<double*, size_t>> g_pinned_free;  // (buffer, capacity in doubles)
+
+double* pinned_take(size_t need, size_t& cap) {
+  {
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    size_t best = g_pinned_free.size();
+    for (size_t q = 0; q < g_pinned_free.size(); ++q)
+      if (g_pinned_free[q].second >= need &&
+          (best == g_pinned_free.size() || g_pinned_free[q].second < g_pinned_free[best].second))
+        best = q;
+    if (best < g_pinned_free.size()) {
+      const auto r = g_pinned_free[best];
+      g_pinned_free.erase(g_pinned_free.begin() + (std::ptrdiff_t)best);
+      cap = r.second;
+      return r.first;
+    }
+  }
+  double* h = nullptr;
+  EDH_HIP(hipHostMalloc((void**)&h, need * sizeof(double)));
+  cap = need;
+  return h;
+}
+
+// Only once no copy into the buffer is pending.
+void pinned_give(double* h, size_t cap) {
+  std::lock_guard<std::mutex> lk(g_pinned_mu);
+  g_pinned_free.push_back({h, cap});
+}
+
 }  // namespace
 
 Batch::~Batch() {
@@ -814,7 +848,7 @@ Batch::~Batch() {
   for (auto& j : jobs)
     if (j) release_job(*j);
   for (ReadSlab& r : slabs) {
-    if (r.host) (void)hipHostFree(r.host);
+    if (r.host) pinned_give(r.host, r.cap);
     if (r.ev) (void)hipEventDestroy(r.ev);
   }
   for (auto& kv : maps) (void)hipFreeAsync(kv.second[4], stream);
@@ -1282,6 +1316,13 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     }
     off = align16(off + hv->size() * jb);
   }
+  int hv_lds = 0;
+  if (has_hv) {
+    std::vector<int64_t> spans(units.size());
+    for (size_t i = 0; i < units.size(); ++i)
+      spans[i] = (int64_t)units[i].nrows * sector((*hv)[units[i].job].first).dev.dn.D;
+    hv_lds = hv_batch_lds_elems(spans);
+  }
   const size_t units_off = off;
   off = align16(off + units.size() * sizeof(HvBatchUnit));
   place(post, pl_post);
@@ -1413,7 +1454,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   d_partial.ensure((size_t)max_ch * kMaxBasis * sizeof(double));
   // launch configuration beside the descriptor bytes: together they fix
   // every kernel and argument of the sweep
-  std::vector<int64_t> cfg{(int64_t)off, hv_block, (int64_t)units.size(), ncyc, comb_nch, max_m, nread,
+  std::vector<int64_t> cfg{(int64_t)off, hv_block, hv_lds, (int64_t)units.size(), ncyc, comb_nch, max_m, nread,
                            (int64_t)read_total, (int64_t)(uintptr_t)d_partial.p, (int64_t)(uintptr_t)d_gather.p};
   for (auto* v : {&pl_pre, &pl_post})
     for (const Placed& p : *v)
@@ -1472,7 +1513,8 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   auto kernels = [&]() {
     launch(pl_pre);
     if (has_hv)
-      launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, stream);
+      launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, hv_lds,
+                      stream);
     launch(pl_post);
     if (ncyc) {
       const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);
@@ -1519,8 +1561,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
     if (si2 < 0) {
       ReadSlab r;
-      r.cap = std::max<size_t>(read_total, 4096);
-      EDH_HIP(hipHostMalloc((void**)&r.host, r.cap * sizeof(double)));
+      r.host = pinned_take(std::max<size_t>(read_total, 4096), r.cap);
       EDH_HIP(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
       slabs.push_back(r);
       si2 = (int)slabs.size() - 1;

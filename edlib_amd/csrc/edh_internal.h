@@ -270,6 +270,10 @@ void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out
 int hv_batch_unit_rows(int64_t ddn);  // rows per unit for a down dimension (a multiple of the row group)
 int hv_batch_row_group(int64_t ddn);  // rows per wavefront task
 int64_t hv_batch_tasks(int64_t dup, int64_t ddn);  // wavefront tasks of a whole operand
-void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s);
+// LDS elements for a launch whose units span `spans` elements each (units
+// above the result run unstaged)
+int hv_batch_lds_elems(const std::vector<int64_t>& spans);
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, int lds_elems,
+                     hipStream_t s);
 
 }  // namespace edh

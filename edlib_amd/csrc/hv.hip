@@ -499,15 +499,20 @@ struct HvBatchJob {
 };
 
 // One element of the grouped H*v: w[u][d] (written to W) from the operand
-// V scaled by vscale; returns v[u][d] * w[u][d] (scaled v).
+// V scaled by vscale; returns v[u][d] * w[u][d] (scaled v).  With STAGED,
+// R is row u of V copied into LDS, and the element's own value and its
+// down-spin gathers are read from there; the arithmetic (operands and
+// order) is the same either way, so the result does not depend on it.
+template <bool STAGED>
 __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const HvParams& p, const double* __restrict__ V,
                                                    double* __restrict__ W, int u, int d, double vscale, double wscale,
-                                                   int na, unsigned lowmask) {
+                                                   int na, unsigned lowmask, const double* R) {
   const int Ddn = p.Ddn;
   const double* __restrict__ vals = p.values;
   const int cross = J.cross;
   const int sl = d >> 6;
   const size_t rowoff = (size_t)u * Ddn;
+  if (!STAGED) R = V + rowoff;
 
   const int bu = p.up_bits[u];
   const int bd = p.dn_bits[d];
@@ -517,7 +522,8 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
   else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
   else x = 0.0;
   double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
-  acc = fma(p.up_e[u] + p.dn_e[d] + x, V[rowoff + d] * vscale, acc);
+  const double vself = R[d];
+  acc = fma(p.up_e[u] + p.dn_e[d] + x, vself * vscale, acc);
   // down-spin hopping (sliced ELL, slice = d / 64)
   const int len = p.dn_sllen[sl];
   const uint32_t* __restrict__ el = p.dn_ell + p.dn_slptr[sl] + (d & 63);
@@ -525,8 +531,8 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
   for (; k + 4 <= len; k += 4) {
     const uint32_t e0 = el[k * kSlice], e1 = el[(k + 1) * kSlice];
     const uint32_t e2 = el[(k + 2) * kSlice], e3 = el[(k + 3) * kSlice];
-    const double x0 = V[rowoff + (e0 & kColMask)], x1 = V[rowoff + (e1 & kColMask)];
-    const double x2 = V[rowoff + (e2 & kColMask)], x3 = V[rowoff + (e3 & kColMask)];
+    const double x0 = R[e0 & kColMask], x1 = R[e1 & kColMask];
+    const double x2 = R[e2 & kColMask], x3 = R[e3 & kColMask];
     acc = fma(vals[e0 >> kColBits], x0 * vscale, acc);
     acc = fma(vals[e1 >> kColBits], x1 * vscale, acc);
     acc = fma(vals[e2 >> kColBits], x2 * vscale, acc);
@@ -534,7 +540,7 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
   }
   for (; k < len; ++k) {
     const uint32_t e = el[k * kSlice];
-    acc = fma(vals[e >> kColBits], V[rowoff + (e & kColMask)] * vscale, acc);
+    acc = fma(vals[e >> kColBits], R[e & kColMask] * vscale, acc);
   }
   // up-spin hopping: rows of the same columns (coalesced)
   const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
@@ -563,7 +569,7 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
     }
   }
   W[rowoff + d] = acc;
-  return V[rowoff + d] * vscale * acc;
+  return vself * vscale * acc;
 }
 
 __device__ __forceinline__ double wave_sum64(double x) {
@@ -576,12 +582,16 @@ __device__ __forceinline__ double wave_sum64(double x) {
 // the down dimension is 64 or more (a slice is exactly one ELL slice, so the
 // slot count is wavefront-uniform), else R = 64 / Dp whole rows per
 // wavefront (Dp = the down dimension rounded up to a power of two).  The
-// down-spin gathers read the row through the L1/L2 caches, and the loads of
-// four slots are issued before their FMAs.  Units start at multiples of R
+// loads of four slots are issued before their FMAs.  Units start at multiples of R
 // rows, so a task's (row group, slice) does not depend on the grouping; a
 // Lanczos run's share of <v, w> is stored there (tdot), one value per task.
+// A unit of at most lds_elems elements (its rows are contiguous in V) is
+// first copied into LDS by the whole work-group, so the down-spin gathers
+// and the element's own value come from LDS rather than through L1.
 __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __restrict__ jobs,
-                                                            const HvBatchUnit* __restrict__ units, int nunits) {
+                                                            const HvBatchUnit* __restrict__ units, int nunits,
+                                                            int lds_elems) {
+  extern __shared__ double s_unit[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;
@@ -609,12 +619,35 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
     const double wscale = p.dscale ? p.dscale[1] : p.wscale;
     const int na = 1 << (p.Ns - p.nb);
     const unsigned lowmask = (1u << p.nb) - 1u;
+    // the unit's rows are V[row0 * Ddn, (row0 + nrows) * Ddn): staged when
+    // they fit (uniform over the work-group)
+    const int span = U.nrows * Ddn;
+    const bool staged = span <= lds_elems;
+    if (staged) {
+      const double* __restrict__ src = V + (size_t)U.row0 * Ddn;
+      const int T = blockDim.x;
+      int i = threadIdx.x;
+      for (; i + 3 * T < span; i += 4 * T) {
+        const double x0 = src[i], x1 = src[i + T], x2 = src[i + 2 * T], x3 = src[i + 3 * T];
+        s_unit[i] = x0;
+        s_unit[i + T] = x1;
+        s_unit[i + 2 * T] = x2;
+        s_unit[i + 3 * T] = x3;
+      }
+      for (; i < span; i += T) s_unit[i] = src[i];
+      __syncthreads();
+    }
     for (int t = wave; t < ntask; t += nw) {
       const int g = t / nsl, sl = t - g * nsl;
       const int u = U.row0 + (g << (6 - lg)) + (lane >> lg);
       const int d = (sl << 6) + (lane & ((1 << lg) - 1));
       const bool live = u < U.row0 + U.nrows && d < Ddn;
-      const double vw = live ? hv_batch_element(J, p, V, W, u, d, vscale, wscale, na, lowmask) : 0.0;
+      double vw = 0.0;
+      if (live) {
+        vw = staged ? hv_batch_element<true>(J, p, V, W, u, d, vscale, wscale, na, lowmask,
+                                             s_unit + (size_t)(u - U.row0) * Ddn)
+                    : hv_batch_element<false>(J, p, V, W, u, d, vscale, wscale, na, lowmask, nullptr);
+      }
       if (J.tdot) {
         // Lanczos runs: this task's share of <v, w>, one wavefront tree,
         // at (row group, slice) -- the same place whatever the grouping
@@ -622,6 +655,7 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
         if (lane == 0) J.tdot[(size_t)((U.row0 >> (6 - lg)) + g) * nsl + sl] = t2;
       }
     }
+    if (staged) __syncthreads();  // the next unit overwrites the staged rows
   }
 }
 
@@ -880,11 +914,46 @@ int64_t hv_batch_tasks(int64_t dup, int64_t ddn) {
   return (dup + g - 1) / g * ((ddn + 63) / 64);
 }
 
-void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, hipStream_t s) {
+int hv_batch_lds_elems(const std::vector<int64_t>& spans) {
+  // small units (the usual case) are all staged with an allocation that
+  // leaves room for four 512-thread work-groups per CU; when most elements
+  // sit in larger units, a larger allocation stages those too
+  constexpr int64_t kSmall = 4608;
+  const int64_t budget = (int64_t)(lds_budget() / sizeof(double));
+  int64_t mx = 0, small_max = 0, big_elems = 0, all = 0;
+  for (int64_t s : spans) {
+    mx = std::max(mx, s);
+    all += s;
+    if (s <= kSmall) small_max = std::max(small_max, s);
+    else big_elems += s;
+  }
+  if (mx <= kSmall) return (int)mx;
+  if (2 * big_elems >= all) return (int)std::min(mx, budget);
+  return (int)small_max;
+}
+
+void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, int lds_elems,
+                     hipStream_t s) {
   if (nunits <= 0) return;
   EDH_CHECK(block >= 64 && block <= 512 && block % 64 == 0, EDH_ERR_ARG, "grouped H*v: bad block size");
+  EDH_CHECK(lds_elems >= 0 && (size_t)lds_elems * sizeof(double) <= lds_budget(), EDH_ERR_ARG,
+            "grouped H*v: staged unit exceeds LDS");
   const int grid = std::min(nunits, 1 << 20);
-  hipLaunchKernelGGL(hv_batch_wave_kernel, dim3(grid), dim3(block), 0, s, (const HvBatchJob*)djobs, dunits, nunits);
+  const size_t smem = (size_t)lds_elems * sizeof(double);
+  if (smem > 64 * 1024) {
+    // raised to the whole budget once per device (the first such launch is
+    // an eager one: a captured sweep repeats one that ran before)
+    static bool raised[64] = {false};
+    int dev = 0;
+    EDH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64 || !raised[dev]) {
+      EDH_HIP(hipFuncSetAttribute((const void*)hv_batch_wave_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_budget()));
+      if (dev >= 0 && dev < 64) raised[dev] = true;
+    }
+  }
+  hipLaunchKernelGGL(hv_batch_wave_kernel, dim3(grid), dim3(block), smem, s, (const HvBatchJob*)djobs, dunits,
+                     nunits, lds_elems);
   EDH_HIP(hipGetLastError());
 }
 

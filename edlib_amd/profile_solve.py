@@ -6,7 +6,9 @@ function (4 start vectors per thermally relevant pair, beta=10, 256
 Matsubara points, 60 Lanczos steps), everything through the batched engine.
 A first solve warms up the code paths; the second is the one reported, with
 the wall time of its eigensolver and Green's-function phases.
-Usage: python -m edlib_amd.profile_solve [--plain]
+Usage: python -m edlib_amd.profile_solve [--plain] [--no-graphs]
+(--no-graphs launches every sweep's kernels one by one, for counter
+collection, whose per-dispatch sampling does not follow replayed graphs)
 """
 import argparse
 import os
@@ -22,6 +24,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--plain", action="store_true", help="also time the plain per-sector solve")
     ap.add_argument("--sweeps", type=int, default=16, help="sweeps per Batch.step call")
+    ap.add_argument("--no-graphs", action="store_true", help="no hipGraph replay of steady sweeps")
     a = ap.parse_args()
     from edlib_amd.batch import Batch
     from edlib_amd.solver import BatchedSolve, SolveParams, solve_plain
@@ -29,6 +32,8 @@ def main():
     sp = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0,), seed=SEED)
     m = siam_model(12, 1, SEED + 4)
     b = Batch(0)
+    if a.no_graphs:
+        b.set_graphs(False)
     for rep in range(2):
         t0 = time.perf_counter()
         s = BatchedSolve(b, m, sp)
