@@ -585,6 +585,8 @@ __device__ __forceinline__ double wave_sum64(double x) {
 // loads of four slots are issued before their FMAs.  Units start at multiples of R
 // rows, so a task's (row group, slice) does not depend on the grouping; a
 // Lanczos run's share of <v, w> is stored there (tdot), one value per task.
+constexpr int kStageMax = 4608;  // elements of the largest staged unit (36 KB)
+
 // A unit of at most lds_elems elements (its rows are contiguous in V) is
 // first copied into LDS by the whole work-group, so the down-spin gathers
 // and the element's own value come from LDS rather than through L1.
@@ -915,43 +917,23 @@ int64_t hv_batch_tasks(int64_t dup, int64_t ddn) {
 }
 
 int hv_batch_lds_elems(const std::vector<int64_t>& spans) {
-  // small units (the usual case) are all staged with an allocation that
-  // leaves room for four 512-thread work-groups per CU; when most elements
-  // sit in larger units, a larger allocation stages those too
-  constexpr int64_t kSmall = 4608;
-  const int64_t budget = (int64_t)(lds_budget() / sizeof(double));
-  int64_t mx = 0, small_max = 0, big_elems = 0, all = 0;
-  for (int64_t s : spans) {
-    mx = std::max(mx, s);
-    all += s;
-    if (s <= kSmall) small_max = std::max(small_max, s);
-    else big_elems += s;
-  }
-  if (mx <= kSmall) return (int)mx;
-  if (2 * big_elems >= all) return (int)std::min(mx, budget);
-  return (int)small_max;
+  // units of up to kStageMax elements (the usual case) are staged, with an
+  // allocation that leaves room for four 512-thread work-groups per CU;
+  // longer rows are read through the caches (staging one 100-KB row per
+  // work-group measured 1.4-2x slower on the 16-site sectors)
+  int64_t mx = 0;
+  for (int64_t s : spans)
+    if (s <= kStageMax) mx = std::max(mx, s);
+  return (int)mx;
 }
 
 void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, int lds_elems,
                      hipStream_t s) {
   if (nunits <= 0) return;
   EDH_CHECK(block >= 64 && block <= 512 && block % 64 == 0, EDH_ERR_ARG, "grouped H*v: bad block size");
-  EDH_CHECK(lds_elems >= 0 && (size_t)lds_elems * sizeof(double) <= lds_budget(), EDH_ERR_ARG,
-            "grouped H*v: staged unit exceeds LDS");
+  EDH_CHECK(lds_elems >= 0 && lds_elems <= kStageMax, EDH_ERR_ARG, "grouped H*v: staged unit too large");
   const int grid = std::min(nunits, 1 << 20);
   const size_t smem = (size_t)lds_elems * sizeof(double);
-  if (smem > 64 * 1024) {
-    // raised to the whole budget once per device (the first such launch is
-    // an eager one: a captured sweep repeats one that ran before)
-    static bool raised[64] = {false};
-    int dev = 0;
-    EDH_HIP(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64 || !raised[dev]) {
-      EDH_HIP(hipFuncSetAttribute((const void*)hv_batch_wave_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds_budget()));
-      if (dev >= 0 && dev < 64) raised[dev] = true;
-    }
-  }
   hipLaunchKernelGGL(hv_batch_wave_kernel, dim3(grid), dim3(block), smem, s, (const HvBatchJob*)djobs, dunits,
                      nunits, lds_elems);
   EDH_HIP(hipGetLastError());
