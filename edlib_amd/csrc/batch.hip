@@ -1138,7 +1138,8 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
   // <v, w>, each 256-byte aligned
   auto al = [](size_t c) { return (c + 31) & ~size_t(31); };
   const size_t nv = al((size_t)j.n), nsc = S_T + 2 * (size_t)nl + 3;
-  j.ntdot = hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
+  j.ntdot = hv_batch_routed(s.tabs.up.D, s.tabs.dn.D) ? hv_batch_rows_partials(s.tabs.up.D)
+                                                       : hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
   double* base = dalloc(j, 3 * nv + al(nsc) + (size_t)j.ntdot);
   for (int q = 0; q < 3; ++q) j.lz[q] = base + q * nv;
   j.sc = base + 3 * nv;
@@ -1301,14 +1302,22 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   const size_t jb = hv_batch_job_bytes();
   const bool has_hv = hv && !hv->empty();
   std::vector<HvBatchUnit> units;
+  std::vector<int> routed;  // operands for the rows kernel (launched after the grouped one)
   if (has_hv) {
     // rows per unit: one while the rows of the sweep are few (parallelism),
     // up to ~4K elements per unit when there are many (less scheduling)
     int64_t total_rows = 0;
-    for (size_t q = 0; q < hv->size(); ++q) total_rows += sector((*hv)[q].first).dev.up.D;
+    for (size_t q = 0; q < hv->size(); ++q) {
+      const DevSector& ds = sector((*hv)[q].first).dev;
+      if (!hv_batch_routed(ds.up.D, ds.dn.D)) total_rows += ds.up.D;
+    }
     const int64_t spread = std::max<int64_t>(1, total_rows / 8192);
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
+      if (hv_batch_routed(ds.up.D, ds.dn.D)) {
+        routed.push_back((int)q);
+        continue;
+      }
       const int rg = hv_batch_row_group(ds.dn.D);
       const int ru = (int)((std::min<int64_t>(hv_batch_unit_rows(ds.dn.D), spread) + rg - 1) / rg * rg);
       for (int64_t r = 0; r < ds.up.D; r += ru)
@@ -1461,7 +1470,10 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       for (int64_t x : {(int64_t)p.kind, (int64_t)p.nops, (int64_t)p.nch, (int64_t)p.ops_off, (int64_t)p.cj_off,
                         (int64_t)p.maxcnt})
         cfg.push_back(x);
-  const bool steady = use_graphs && d_arena.p == last_arena && cfg == last_cfg && last_desc.size() == off &&
+  // (sweeps with rows-kernel operands launch eagerly: their arguments are
+  // not in the descriptor bytes)
+  const bool steady = use_graphs && routed.empty() && d_arena.p == last_arena && cfg == last_cfg &&
+                      last_desc.size() == off &&
                       std::memcmp(last_desc.data(), hb, off) == 0;
   if (!steady) {
     drop_graph();
@@ -1515,6 +1527,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     if (has_hv)
       launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, hv_lds,
                       stream);
+    for (int q : routed) launch_hv_batch_rows(sector((*hv)[q].first).dev, (*hv)[q].second, stream);
     launch(pl_post);
     if (ncyc) {
       const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);

@@ -78,6 +78,11 @@ struct HvParams {
   // slab kernel geometry; off32: every element offset fits 32 bits
   int nslabs, rows_per_wg, nrb, off32;
   int xcd_rows;  // rows kernel: XCD-grouped row assignment (measured 2 % faster on the 16-site sector)
+  // rows kernel in a batch: an eigensolver run's control block (v and w are
+  // then columns col and col + 1 of the basis in v or v_alt; a run whose
+  // state is not 0 is skipped)
+  const double* ctl;
+  const double* v_alt;
 };
 
 template <int CROSS>
@@ -153,6 +158,15 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   const double vscale = p.dscale ? p.dscale[0] : p.vscale;
   const double wscale = p.dscale ? p.dscale[1] : p.wscale;
   const double* __restrict__ V = p.v;
+  double* __restrict__ W = p.w;
+  if (p.ctl) {
+    if (p.ctl[kCtlState] != 0.0) return;  // uniform over the grid
+    const int64_t N = (int64_t)p.Dup * p.Ddn;
+    const double* base = p.ctl[kCtlPhase] != 0.0 ? p.v_alt : p.v;
+    const int64_t col = (int64_t)p.ctl[kCtlCol];
+    V = base + col * N;
+    W = const_cast<double*>(base) + (col + 1) * N;
+  }
   const double* __restrict__ vals = p.values;
   double dsum = 0.0;
 
@@ -375,7 +389,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
       for (int j = 0; j < J; ++j) {
         const int d = c0 + tid + j * T;
         if (d < Ddn) {
-          __builtin_nontemporal_store(acc[j], p.w + rowoff + d);
+          __builtin_nontemporal_store(acc[j], W + rowoff + d);
           if (UP && p.dot_partial) dsum = fma(LDS ? s_v[d] : V[rowoff + d] * vscale, acc[j], dsum);
         }
       }
@@ -798,6 +812,8 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   slab_geometry(ds, p.nslabs, p.rows_per_wg, p.nrb);
   p.off32 = (uint64_t)ds.N < (uint64_t(1) << 32);
   p.xcd_rows = 1;
+  p.ctl = a.ctl;
+  p.v_alt = a.v_alt;
   return p;
 }
 
@@ -937,6 +953,18 @@ void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, i
   hipLaunchKernelGGL(hv_batch_wave_kernel, dim3(grid), dim3(block), smem, s, (const HvBatchJob*)djobs, dunits,
                      nunits, lds_elems);
   EDH_HIP(hipGetLastError());
+}
+
+bool hv_batch_routed(int64_t dup, int64_t ddn) { return ddn > kStageMax && dup * ddn >= (int64_t(1) << 22); }
+
+int64_t hv_batch_rows_partials(int64_t dup) { return std::min<int64_t>(dup, 1 << 20); }
+
+void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s) {
+  EDH_CHECK(hv_batch_routed(ds.up.D, ds.dn.D), EDH_ERR_INTERNAL, "grouped H*v: operand not routed to the rows kernel");
+  EDH_CHECK(plan(ds).grid == hv_batch_rows_partials(ds.up.D), EDH_ERR_INTERNAL, "grouped H*v: rows grid mismatch");
+  HvArgs b = a;
+  b.dot_partial = a.tdot;  // one share of <v, w> per work-group (row slot)
+  launch_hv(ds, b, s, HV_SINGLE);
 }
 
 }  // namespace edh

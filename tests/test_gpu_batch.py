@@ -89,6 +89,48 @@ def test_batch_lanczos_matches_single_sector():
     b.close()
 
 
+def test_large_operands_on_rows_kernel_match_single_sector():
+    """Operands with long rows (D_dn > 4608, N >= 2^22) run on the
+    single-sector rows kernel inside a batch sweep: eigenpairs and Lanczos
+    coefficients equal the single-sector path's, and a run's coefficients
+    are bit-identical alone and next to a grouped (small) operand."""
+    import edlib_amd as ea
+    from edlib_amd.batch import Batch
+    from models import ring_hopping
+    Ns = 16
+    m = ea.HubbardModel(Ns, ring_hopping(Ns), np.full(Ns, 4.0), np.full(Ns, 2.0))
+    big = (3, 6)  # 560 x 8008 = 4,484,480 states
+    ctx = ea.Context(0)
+    ctx.set_model(m)
+    n = ctx.set_sector(*big)
+    assert n == 560 * 8008
+    ref_e, _, _ = ctx.eigsh(2, 8, 1e-12, 1000, SEED + 5, vectors=False)
+    v0 = np.random.default_rng(SEED + 6).uniform(-1, 1, n)
+    v0 /= np.linalg.norm(v0)
+    nl_ref, a_ref, b_ref = ctx.lanczos(v0, 30)
+    coef = []
+    for mixed in (False, True):
+        b = Batch(0)
+        sid, _ = b.add_sector(m, *big)
+        jid = b.submit_eig(sid, 2, 8, 1e-12, 1000, SEED + 5)
+        lid = b.submit_lanczos(sid, v0, 30)
+        if mixed:
+            small, ns = b.add_sector(m, 2, 2)
+            b.submit_eig(small, 2, 8, 1e-12, 1000, SEED + 7)
+            b.submit_lanczos(small, np.full(ns, 1.0 / math.sqrt(ns)), 30)
+        b.run()
+        ev, _ = b.eig_result(jid)
+        np.testing.assert_allclose(ev, ref_e, rtol=1e-10, atol=1e-12)
+        nl, a, bb = b.lanczos_result(lid, 30)
+        assert nl == nl_ref
+        np.testing.assert_allclose(a[:nl], a_ref[:nl], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(bb[1:nl], b_ref[1:nl], rtol=1e-10, atol=1e-12)
+        coef.append((a.copy(), bb.copy()))
+        b.close()
+    assert np.array_equal(coef[0][0], coef[1][0]) and np.array_equal(coef[0][1], coef[1][1])
+    ctx.close()
+
+
 def _params(**kw):
     from edlib_amd.solver import SolveParams
     base = dict(nev=4, beta=10.0, nomega=32, nlanc=100, orbitals=(0,), seed=SEED)
