@@ -145,7 +145,10 @@ constexpr int kStage = 8;
 // of each chunk c0; a wavefront therefore covers exactly one ELL slice per j,
 // so slot counts are wavefront-uniform and the slot loops need no padding
 // beyond their own slice.
-template <int J, int CROSS, bool KAN, bool LDS, int EF, bool UP>
+// CTL: the operands are resolved from an eigensolver run's control block
+// (batch sweeps only; a separate instantiation keeps the other launches'
+// register allocation as it is).
+template <int J, int CROSS, bool KAN, bool LDS, int EF, bool UP, bool CTL>
 __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   extern __shared__ double s_v[];
   __shared__ double s_red[16];
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   const double wscale = p.dscale ? p.dscale[1] : p.wscale;
   const double* __restrict__ V = p.v;
   double* __restrict__ W = p.w;
-  if (p.ctl) {
+  if (CTL) {
     if (p.ctl[kCtlState] != 0.0) return;  // uniform over the grid
     const int64_t N = (int64_t)p.Dup * p.Ddn;
     const double* base = p.ctl[kCtlPhase] != 0.0 ? p.v_alt : p.v;
@@ -724,7 +727,9 @@ Launch plan(const DevSector& ds) {
 
 template <int J, int CROSS, bool KAN, bool LDS, int EF>
 void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
-  auto k = up ? hv_rows_kernel<J, CROSS, KAN, LDS, EF, true> : hv_rows_kernel<J, CROSS, KAN, LDS, EF, false>;
+  auto k = !up     ? hv_rows_kernel<J, CROSS, KAN, LDS, EF, false, false>
+           : p.ctl ? hv_rows_kernel<J, CROSS, KAN, LDS, EF, true, true>
+                   : hv_rows_kernel<J, CROSS, KAN, LDS, EF, true, false>;
   if (L.smem > 64 * 1024)
     EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.smem));
   hipLaunchKernelGGL(k, dim3(L.grid), dim3(L.block), L.smem, s, p);
