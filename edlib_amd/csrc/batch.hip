@@ -1311,7 +1311,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const DevSector& ds = sector((*hv)[q].first).dev;
       if (!hv_batch_routed(ds.up.D, ds.dn.D)) total_rows += ds.up.D;
     }
-    const int64_t spread = std::max<int64_t>(1, total_rows / 2048);
+    const int64_t spread = std::max<int64_t>(1, total_rows / 8192);
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
       if (hv_batch_routed(ds.up.D, ds.dn.D)) {
