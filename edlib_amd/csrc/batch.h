@@ -160,11 +160,15 @@ struct Batch {
   std::vector<int64_t> last_cfg;   // and its launch configuration
   const void* last_arena = nullptr;
   hipGraphExec_t sweep_graph = nullptr;
+  hipEvent_t graph_ev = nullptr;  // recorded after every launch of sweep_graph
+  bool graph_launched = false;
   size_t graph_sweeps = 0;  // sweeps run as graph replays
-  void drop_graph() {
-    if (sweep_graph) (void)hipGraphExecDestroy(sweep_graph);
-    sweep_graph = nullptr;
-  }
+  // Executable graphs that were launched are destroyed only once their last
+  // launch has completed (HIP does not defer the destruction of an exec that
+  // is still running): retired ones wait here with the event of that launch.
+  std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired_graphs;
+  void drop_graph();
+  void reap_graphs(bool wait);
   long long sweeps = 0;
   ~Batch();
 

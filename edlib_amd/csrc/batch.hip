@@ -857,6 +857,8 @@ Batch::~Batch() {
   issue_frees_until(nullptr);
   if (stream) (void)hipStreamSynchronize(stream);
   drop_graph();
+  reap_graphs(true);
+  if (graph_ev) (void)hipEventDestroy(graph_ev);
   uploads.release();
   if (h_obs) (void)hipHostFree(h_obs);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
@@ -880,7 +882,35 @@ double* Batch::dalloc(BJob& j, size_t count) {
   return (double*)p;
 }
 
+void Batch::drop_graph() {
+  if (!sweep_graph) return;
+  if (graph_launched) {
+    retired_graphs.push_back({sweep_graph, graph_ev});
+    graph_ev = nullptr;
+  } else {
+    (void)hipGraphExecDestroy(sweep_graph);
+  }
+  sweep_graph = nullptr;
+  graph_launched = false;
+}
+
+void Batch::reap_graphs(bool wait) {
+  size_t keep = 0;
+  for (size_t q = 0; q < retired_graphs.size(); ++q) {
+    auto& r = retired_graphs[q];
+    const bool done = wait ? hipEventSynchronize(r.second) == hipSuccess : hipEventQuery(r.second) == hipSuccess;
+    if (done) {
+      (void)hipGraphExecDestroy(r.first);
+      (void)hipEventDestroy(r.second);
+    } else {
+      retired_graphs[keep++] = r;
+    }
+  }
+  retired_graphs.resize(keep);
+}
+
 void Batch::issue_frees_until(hipEvent_t ev) {
+  reap_graphs(false);
   while (!pending_free.empty()) {
     if (ev && hipEventQuery(ev) == hipSuccess) return;
     (void)hipFreeAsync(pending_free.back(), stream);
@@ -1560,7 +1590,10 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
     }
     if (sweep_graph) {
+      if (!graph_ev) EDH_HIP(hipEventCreateWithFlags(&graph_ev, hipEventDisableTiming));
       EDH_HIP(hipGraphLaunch(sweep_graph, stream));
+      EDH_HIP(hipEventRecord(graph_ev, stream));
+      graph_launched = true;
       ++graph_sweeps;
     }
   }
