@@ -160,6 +160,37 @@ int edh_model_siam(edh_model** out, int Ns, int ml, const double* Epsk, const do
   API_END
 }
 
+int edh_model_from_terms(edh_model** out, const edh_model_desc* d) {
+  API_BEGIN
+  EDH_CHECK(out && d, EDH_ERR_ARG, "null argument");
+  edh_model* m = new edh_model();
+  m->m.reset(build_terms(*d));
+  *out = m;
+  API_END
+}
+
+int edh_model_apply(const edh_model* m, long long state, int cap, long long* targets, double* coef, int* count) {
+  API_BEGIN
+  EDH_CHECK(m && count && cap >= 0 && (cap == 0 || (targets && coef)), EDH_ERR_ARG, "null argument");
+  const Model& md = *m->m;
+  EDH_CHECK(state >= 0 && (md.Ns >= 31 || state < (1ll << (2 * md.Ns))), EDH_ERR_ARG, "state out of range");
+  int c = 0;
+  for (const std::vector<Transition>* L : {&md.T, &md.V})
+    for (const Transition& tr : *L) {
+      if (!md.valid(tr, state)) continue;
+      state_t k;
+      int sg;
+      const double v = md.apply(tr, state, k, sg);
+      if (c < cap) {
+        targets[c] = k;
+        coef[c] = sg * v;
+      }
+      ++c;
+    }
+  *count = c;
+  API_END
+}
+
 int edh_model_free(edh_model* m) {
   delete m;
   return EDH_OK;
@@ -497,8 +528,10 @@ int lanczos_device(Ctx& c, int nl, double* alpha, double* beta, double* step_ms)
   const char* genv = getenv("EDH_LANCZOS_GRAPH");
   const bool use_graph = nl >= 4 && !(genv && genv[0] == '0');
   bool graphed = false;
+  // executable graphs are destroyed only after the stream has drained: HIP
+  // does not defer the destruction of an exec whose launches still run
+  hipGraphExec_t ex[3] = {nullptr, nullptr, nullptr};
   if (use_graph) {
-    hipGraphExec_t ex[3] = {nullptr, nullptr, nullptr};
     double* rot[3] = {cur, prev, out};
     bool ok = true;
     for (int ph = 0; ph < 3 && ok; ++ph) {
@@ -519,9 +552,12 @@ int lanczos_device(Ctx& c, int nl, double* alpha, double* beta, double* step_ms)
     if (ok) {
       for (int k = 1; k <= nl; ++k) EDH_HIP(hipGraphLaunch(ex[(k - 1) % 3], s));
       graphed = true;
+    } else {
+      for (auto& e : ex) {
+        if (e) (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
     }
-    for (auto& e : ex)
-      if (e) (void)hipGraphExecDestroy(e);
     (void)hipGetLastError();
   }
   if (!graphed) {
@@ -546,7 +582,10 @@ int lanczos_device(Ctx& c, int nl, double* alpha, double* beta, double* step_ms)
   EDH_HIP(hipMemcpyAsync(alpha, dalpha, nl * sizeof(double), hipMemcpyDeviceToHost, s));
   EDH_HIP(hipMemcpyAsync(beta, dbeta, (nl + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
   EDH_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, s));
-  EDH_HIP(hipStreamSynchronize(s));
+  const hipError_t se = hipStreamSynchronize(s);
+  for (auto& e : ex)
+    if (e) (void)hipGraphExecDestroy(e);
+  EDH_HIP(se);
   return host_st[0];
 }
 

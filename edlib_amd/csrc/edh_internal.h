@@ -65,10 +65,12 @@ struct Model {
   int Ns = 0;
   int ml = 0;         // interacting (impurity) orbitals: Ns for Hubbard
   int hop_sign = -1;  // sign rule of kind-0 transitions: -s1 s2 (Hubbard) or +s1 s2 (SIAM)
+  double v_scale = 0.5;  // kind-1 coefficient = v_scale * value (the SIAM's 1/2 U, :166)
   std::vector<Transition> T, V;
   // Diagonal decomposition: sum_s e[s][i] n_is + sum_s sum_{i!=j} S[s][i][j] n_is n_js
   //                        + sum_{i,j} C[i][j] n_i(up) n_j(dn); site indices.
   std::vector<double> e[2], S[2], C;
+  double e0 = 0.0;  // constant part (terms-built models: diagonal of the empty state)
   // Original parameters, kept for reference-order diagonal evaluation.
   std::vector<double> t, U, J, xmu, Hmag;
   std::vector<double> Epsk, Vk, H0, Ut;
@@ -87,6 +89,7 @@ Model* build_hubbard(int Ns, const double* t, const double* U, const double* J, 
                      const double* Hmag);
 Model* build_siam(int Ns, int ml, const double* Epsk, const double* Vk, const double* H0, const double* Ut,
                   double mu);
+Model* build_terms(const edh_model_desc& d);
 
 // ---------------------------------------------------------------------------
 // Per-sector tables (host copies) and their device mirror.

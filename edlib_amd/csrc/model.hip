@@ -89,12 +89,27 @@ double Model::apply(const Transition& tr, state_t nst, state_t& k, int& sign) co
   a += (1ll << (Ip - 1 - oi));
   k = a;
   sign = g;
-  return 0.5 * tr.value;
+  return v_scale * tr.value;
 }
 
 double Model::diagonal_reference(state_t st) const {
   const int Ip = 2 * Ns;
   double x = 0.0;
+  if (kind == EDH_MODEL_TERMS) {
+    // the decomposed form the model was given in
+    x = e0;
+    for (int s = 0; s < 2; ++s)
+      for (int i = 0; i < Ns; ++i) {
+        if (!occ(st, i + s * Ns, Ip)) continue;
+        x += e[s][i];
+        for (int j = i + 1; j < Ns; ++j)
+          if (occ(st, j + s * Ns, Ip)) x += S[s][i * Ns + j];
+      }
+    for (int i = 0; i < Ns; ++i)
+      for (int j = 0; j < Ns; ++j)
+        if (occ(st, i, Ip) && occ(st, j + Ns, Ip)) x += C[i * Ns + j];
+    return x;
+  }
   if (kind == EDH_MODEL_HUBBARD) {
     for (int im = 0; im < Ns; ++im) {
       for (int is = 0; is < 2; ++is) x += (0.0 - xmu[is]) * occ(st, im + is * Ns, Ip);
@@ -234,6 +249,47 @@ Model* build_siam(int Ns, int ml, const double* Epsk, const double* Vk, const do
   return m;
 }
 
+Model* build_terms(const edh_model_desc& d) {
+  const int Ns = d.Ns;
+  EDH_CHECK(Ns >= 1 && Ns <= 24, EDH_ERR_ARG, "model terms: NSITES must be in [1, 24]");
+  EDH_CHECK(d.n_hop >= 0 && d.n_int >= 0, EDH_ERR_ARG, "model terms: negative term count");
+  EDH_CHECK((d.n_hop == 0 || (d.hop && d.hop_value)) && (d.n_int == 0 || (d.inter && d.inter_value)) && d.eps &&
+                d.same && d.cross,
+            EDH_ERR_ARG, "model terms: null array");
+  const int ml = d.interacting_orbitals;
+  EDH_CHECK(ml >= 0 && ml <= Ns, EDH_ERR_ARG, "model terms: interacting_orbitals out of range");
+  std::unique_ptr<Model> m(new Model());
+  m->kind = EDH_MODEL_TERMS;
+  m->Ns = Ns;
+  m->ml = ml;
+  m->hop_sign = +1;  // the sign rule is folded into the values
+  m->v_scale = 1.0;  // two-body values are the coefficients themselves
+  for (int q = 0; q < d.n_hop; ++q) {
+    const int* h = d.hop + 3 * q;
+    EDH_CHECK(h[0] >= 0 && h[0] < Ns && h[1] >= 0 && h[1] < Ns && (h[2] == 0 || h[2] == 1), EDH_ERR_ARG,
+              "model terms: one-body term index out of range");
+    EDH_CHECK(h[0] != h[1], EDH_ERR_ARG, "model terms: a one-body term must move the electron (i != j)");
+    m->T.push_back(Transition{0, h[0], h[1], 0, 0, h[2], 0, d.hop_value[q]});
+  }
+  for (int q = 0; q < d.n_int; ++q) {
+    const int* v = d.inter + 6 * q;
+    for (int a = 0; a < 4; ++a)
+      EDH_CHECK(v[a] >= 0 && v[a] < ml, EDH_ERR_ARG, "model terms: two-body term outside the interacting orbitals");
+    EDH_CHECK((v[4] == 0 || v[4] == 1) && (v[5] == 0 || v[5] == 1), EDH_ERR_ARG, "model terms: spin index");
+    m->V.push_back(Transition{1, v[0], v[1], v[2], v[3], v[4], v[5], d.inter_value[q]});
+  }
+  EDH_CHECK(m->V.empty() || ml <= 6, EDH_ERR_LIMIT, "model terms: at most 6 interacting orbitals with two-body terms");
+  m->e0 = d.e0;
+  for (int s = 0; s < 2; ++s) {
+    m->e[s].assign(d.eps + s * Ns, d.eps + (s + 1) * Ns);
+    m->S[s].assign(Ns * Ns, 0.0);
+    for (int i = 0; i < Ns; ++i)
+      for (int j = i + 1; j < Ns; ++j) m->S[s][i * Ns + j] = d.same[(s * Ns + i) * Ns + j];
+  }
+  m->C.assign(d.cross, d.cross + Ns * Ns);
+  return m.release();
+}
+
 // ---------------------------------------------------------------------------
 // Per-sector tables.
 // ---------------------------------------------------------------------------
@@ -305,7 +361,7 @@ void build_spin(const Model& m, int spin, int nel, SpinTables& sp, std::vector<d
       for (int j = 0; j < Ns; ++j)
         if (j != i && ((b >> (Ns - 1 - j)) & 1)) x += m.S[spin][i * Ns + j];
     }
-    sp.ediag[r] = x;
+    sp.ediag[r] = spin == 0 ? x + m.e0 : x;
   }
   // hopping CSR: rows are source states, columns the targets (fill_spin)
   const int shift = spin == 0 ? Ns : 0;

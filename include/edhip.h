@@ -64,8 +64,47 @@ int edh_model_hubbard(edh_model** out, int Ns, const double* t, const double* U,
  * (interaction/values), mu (mu). */
 int edh_model_siam(edh_model** out, int Ns, int ml, const double* Epsk, const double* Vk, const double* H0,
                    const double* Ut, double mu);
+/* Any SzSymmetry model given by its terms: the storage-facing interface of
+ * the reference models (T_states(), V_states(), valid/set, diagonal() and
+ * interacting_orbitals(), as SpinResolvedStorage::fill and fill_spin use
+ * them, SpinResolvedStorage.h:120-155,622-638) reduced to plain arrays.
+ *   one-body term q: hop_value[q] * c^+_{j s} c_{i s} with (i, j, s) =
+ *     hop[3q..3q+2]; the model's sign rule is folded into the value (the
+ *     reference's Hubbard terms carry -t, HubbardModel.h:104-105, its SIAM
+ *     terms +V, SingleImpurityAndersonModel.h:77-84);
+ *   two-body term q: inter_value[q] * c^+_{i s} c^+_{j sp} c_{l sp} c_{k s}
+ *     with (i, j, k, l, s, sp) = inter[6q..6q+5], signs as
+ *     InnerInteractionState::set (SingleImpurityAndersonModel.h:156-163);
+ *     every site index below interacting_orbitals (at most 6 of them);
+ *   diagonal: e0 + sum_{o} eps[o] n_o + sum_s sum_{i<j} same[s][i][j] n_is n_js
+ *     + sum_{i,j} cross[i][j] n_i(up) n_j(dn), o = site + spin*Ns.
+ * Terms are applied in the order given (the order of the CSR merge). */
+typedef struct edh_model_desc {
+  int Ns;
+  int interacting_orbitals;
+  int n_hop;
+  const int* hop;
+  const double* hop_value;
+  int n_int;
+  const int* inter;
+  const double* inter_value;
+  double e0;
+  const double* eps;   /* 2*Ns */
+  const double* same;  /* 2*Ns*Ns, only i < j read */
+  const double* cross; /* Ns*Ns */
+} edh_model_desc;
+#define EDH_MODEL_TERMS 2
+int edh_model_from_terms(edh_model** out, const edh_model_desc* d);
+
 int edh_model_free(edh_model* m);
 int edh_model_sites(const edh_model* m, int* Ns, int* interacting_orbitals);
+
+/* Off-diagonal action of the model on one basis state (host, no GPU):
+ * every valid term in term order (T_states, then V_states) as
+ * (target state, sign * value); *count is the number of terms applied
+ * (at most cap are written).  Model::valid + Model::set,
+ * HubbardModel.h:86-108 / SingleImpurityAndersonModel.h:346-357. */
+int edh_model_apply(const edh_model* m, long long state, int cap, long long* targets, double* coef, int* count);
 
 /* T_states() (which=0) / V_states() (which=1) as (kind,i,j,k,l,s,sp) + value;
  * pass NULL arrays to query *count.  HubbardModel.h:141-147. */

@@ -1,8 +1,16 @@
-// HIPSpinResolvedStorage<Model>: the Storage concept of the reference
-// (Storage.h:16-248 + SpinResolvedStorage.h:20-639) served by the MI355X
+// HIPSpinResolvedStorage<Model>: the reference's storage concept
+// (Storage.h:15-248 + SpinResolvedStorage.h:20-639) served by the MI355X
 // engine through the C ABI in edhip.h.  Drop-in for
 //   Hamiltonian<SpinResolvedStorage<HubbardModel<double>>>  ->
 //   Hamiltonian<HIPSpinResolvedStorage<HubbardModel<double>>>.
+//
+// It derives from Storage<prec> (the "Storage.h" beside this file: the
+// reference's own inside the reference tree) and reads the model only
+// through the interface SpinResolvedStorage::fill uses (T_states(),
+// V_states(), valid/set, diagonal(), interacting_orbitals(), symmetry()),
+// so the reference's HubbardModel and SingleImpurityAndersonModel plug in
+// unchanged.  The constructor takes the caller's parameter object
+// (alps::params in the reference) as it is.
 //
 // av() keeps the reference's calling convention (host pointers owned by the
 // caller, w = H v + (clear ? 0 : w)); diag() replaces the ARPACK reverse-
@@ -18,16 +26,16 @@
 #include <type_traits>
 #include <vector>
 
-#include "../edhip.h"
-#include "Models.h"
-#include "Params.h"
-#include "Symmetry.h"
+#include "ModelTerms.h"
+#include "Storage.h"
+#include "SzSymmetry.h"
+#include "edhip.h"
 
 namespace EDLib {
 namespace Storage {
 
 template <class ModelType>
-class HIPSpinResolvedStorage {
+class HIPSpinResolvedStorage : public Storage<typename ModelType::precision> {
   static_assert(std::is_base_of<Symmetry::SzSymmetry, typename ModelType::SYMMETRY>::value,
                 "Model have wrong symmetry.");
 
@@ -35,36 +43,52 @@ class HIPSpinResolvedStorage {
   typedef ModelType Model;
   typedef typename Model::precision prec;
   static_assert(std::is_same<prec, double>::value, "the engine computes in FP64");
+  typedef Storage<prec> Base;
+  using Base::n;
+  using Base::ntot;
+  using Base::eigenvalues;
+  using Base::eigenvectors;
 
-  HIPSpinResolvedStorage(params& p, Model& m)
-      : model_(m), nev_(p["arpack.NEV"].as<int>()), eval_only_(p["storage.EIGENVALUES_ONLY"].as<int>()),
-        Ns_(p["NSITES"].as<int>()) {
-    ncv_ = p.exists("arpack.NCV") ? p["arpack.NCV"].as<int>() : 2 * nev_ + 3;  // Storage.h:26-30
-    int dev = p.exists("device.ID") ? p["device.ID"].as<int>() : 0;
+  // SpinResolvedStorage(alps::params&, Model&), SpinResolvedStorage.h:47-48.
+  // Keys: arpack.NEV, arpack.NCV, storage.EIGENVALUES_ONLY, NSITES; the
+  // optional device.ID picks the GPU (default 0).
+  template <class Params>
+  HIPSpinResolvedStorage(Params& p, Model& m) : Base(p), model_(m) {
+    nev_ = p["arpack.NEV"];
+    eval_only_ = p["storage.EIGENVALUES_ONLY"];
+    Ns_ = p["NSITES"];
+    if (p.exists("arpack.NCV")) {
+      ncv_ = p["arpack.NCV"];
+    } else {
+      ncv_ = 2 * nev_ + 3;  // Storage.h:26-30
+    }
+    int dev = 0;
+    if (p.exists("device.ID")) dev = p["device.ID"];
+    EngineModelPtr em = engine_model_from_interface(m, Ns_);
     edh_ctx* c = nullptr;
-    edh_throw(edh_create(&c, dev));
+    edh_check(edh_create(&c, dev));
     ctx_.reset(c);
-    edh_throw(edh_set_model(ctx_.get(), m.engine_model()));
+    edh_check(edh_set_model(ctx_.get(), em.get()));
   }
 
   // ---- Storage concept ---------------------------------------------------
-  void av(prec* v, prec* w, int n, bool clear = true) {
-    edh_throw(edh_av(ctx_.get(), v, w, n, clear ? 1 : 0, EDH_MEM_HOST));
+  virtual void av(prec* v, prec* w, int n, bool clear = true) {
+    edh_check(edh_av(ctx_.get(), v, w, n, clear ? 1 : 0, EDH_MEM_HOST));
   }
 
   // SpinResolvedStorage::fill: per-spin tables for the model's current sector.
   void fill() {
     reset();
-    if (n_ == 0) return;
+    if (n() == 0) return;
     int64_t dim = 0;
     const auto& s = model_.symmetry().sector();
-    edh_throw(edh_set_sector(ctx_.get(), s.nup(), s.ndown(), &dim));
+    edh_check(edh_set_sector(ctx_.get(), s.nup(), s.ndown(), &dim));
     filled_ = true;
   }
   void reset(int = 0) {
     model_.symmetry().init();
     const auto& s = model_.symmetry().sector();
-    n_ = ntot_ = (int)s.size();
+    n() = ntot() = (int)s.size();
     filled_ = false;
   }
   void init() { model_.symmetry().init(); }
@@ -79,46 +103,63 @@ class HIPSpinResolvedStorage {
     const auto& s = model_.symmetry().sector();
     int64_t outn = 0;
     outvec.assign(next_sec.size(), prec(0));
-    edh_throw(edh_adag(ctx_.get(), s.nup(), s.ndown(), i, a ? 1 : 0, invec.data(), outvec.data(), &outn,
+    edh_check(edh_adag(ctx_.get(), s.nup(), s.ndown(), i, a ? 1 : 0, invec.data(), outvec.data(), &outn,
                        EDH_MEM_HOST));
   }
 
   prec vv(const std::vector<prec>& v, const std::vector<prec>& w) {
     double out = 0;
-    edh_throw(edh_vv(ctx_.get(), v.data(), w.data(), (int64_t)v.size(), &out, EDH_MEM_HOST));
+    edh_check(edh_vv(ctx_.get(), v.data(), w.data(), (int64_t)v.size(), &out, EDH_MEM_HOST));
     return out;
   }
 
-  void constant_shift(prec shift) { edh_throw(edh_constant_shift(ctx_.get(), shift)); }
-  void zero_eigenapair() {}
-  void prepare_work_arrays(prec*, size_t = 0) {}
-  int finalize(int info, bool = true, bool = false) { return info; }
+  // Adds shift to every diagonal element of the filled sector, until the
+  // next fill() (SpinResolvedStorage.h:464-466; its refill discards it too).
+  void constant_shift(prec shift) { edh_check(edh_constant_shift(ctx_.get(), shift)); }
 
-  // Storage::diag (Storage.h:43-146): lowest eigenpairs of the current sector.
-  int diag() {
-    evals_.clear();
-    evecs_.clear();
-    if (n_ == 0) return 0;
+  // ntot() == 1: eigenvalue = the sector's one diagonal element, eigenvector
+  // [1] (SpinResolvedStorage.h:52-56).  The element is read through av on
+  // the unit vector, so a constant_shift is included as in the reference.
+  virtual void zero_eigenapair() {
     if (!filled_) fill();
-    const int ncv = std::min(ncv_, ntot_);
-    const int nev = ntot_ == 1 ? 1 : std::min(nev_, ncv - 1);
+    prec one = 1.0, d = 0.0;
+    av(&one, &d, 1, true);
+    eigenvalues().assign(1, d);
+    eigenvectors().assign(1, std::vector<prec>(1, prec(1.0)));
+  }
+  virtual void prepare_work_arrays(prec*, size_t = 0) {}
+  virtual int finalize(int info, bool = true, bool = false) { return info; }
+
+  // Storage::diag (Storage.h:43-146): lowest eigenpairs of the current
+  // sector; n == 0 gives none, ntot == 1 goes through zero_eigenapair.
+  int diag() {
+    eigenvalues().clear();
+    eigenvectors().clear();
+    if (n() == 0) return finalize(0, true, true);
+    if (!filled_) fill();
+    if (ntot() == 1) {
+      zero_eigenapair();
+      return finalize(0);
+    }
+    const int nsec = n();
+    const int ncv = std::min(ncv_, ntot());
+    const int nev = std::min(nev_, ncv - 1);
     std::vector<double> ev(std::max(nev, 1));
     std::vector<double> X;
-    if (!eval_only_) X.resize((size_t)n_ * nev);
+    if (!eval_only_) X.resize((size_t)nsec * nev);
     int nconv = 0, nav = 0;
-    int rc = edh_eigsh(ctx_.get(), nev_, ncv_, 1e-14, 1000, 0x45444c6962ull, ev.data(), eval_only_ ? nullptr : X.data(),
-                       &nconv, &nav);
-    edh_throw(rc);
-    if (nconv < std::min(nev_, (int)std::min<int64_t>(ncv_, n_) - 1) && n_ > 1)
-      std::cout << "Maximum number of iterations reached." << std::endl;  // Storage.h:122-124
-    evals_.assign(ev.begin(), ev.begin() + nconv);
+    edh_check(edh_eigsh(ctx_.get(), nev_, ncv_, 1e-14, 1000, 0x45444c6962ull, ev.data(),
+                        eval_only_ ? nullptr : X.data(), &nconv, &nav));
+    if (nconv < nev) std::cout << "Maximum number of iterations reached." << std::endl;  // Storage.h:122-124
+    eigenvalues().assign(ev.begin(), ev.begin() + nconv);
     if (!eval_only_) {
-      evecs_.resize(nconv);
-      for (int i = 0; i < nconv; ++i) evecs_[i].assign(X.begin() + (size_t)i * n_, X.begin() + (size_t)(i + 1) * n_);
+      eigenvectors().resize(nconv);
+      for (int i = 0; i < nconv; ++i)
+        eigenvectors()[i].assign(X.begin() + (size_t)i * nsec, X.begin() + (size_t)(i + 1) * nsec);
     } else {
-      evecs_.assign(nconv, std::vector<prec>(1, prec(0)));
+      eigenvectors().assign(nconv, std::vector<prec>(1, prec(0)));
     }
-    return 0;
+    return finalize(0);
   }
 
   // Device-resident Lanczos recurrence (Lanczos.h:36-77) on the filled
@@ -127,14 +168,14 @@ class HIPSpinResolvedStorage {
     alpha.assign(nl, 0.0);
     beta.assign(nl + 1, 0.0);
     int nlanc = 0;
-    edh_throw(edh_lanczos(ctx_.get(), v.data(), (int64_t)v.size(), nl, alpha.data(), beta.data(), &nlanc, EDH_MEM_HOST));
+    edh_check(edh_lanczos(ctx_.get(), v.data(), (int64_t)v.size(), nl, alpha.data(), beta.data(), &nlanc, EDH_MEM_HOST));
     return nlanc;
   }
   // Susceptibility operator on a vector of the current sector (ChiLoc.h:474-530).
   void local_op(int op, const std::vector<int>& orbs, const std::vector<prec>& in, std::vector<prec>& out) {
     const auto& s = model_.symmetry().sector();
     out.assign(in.size(), 0.0);
-    edh_throw(edh_local_op(ctx_.get(), s.nup(), s.ndown(), op, (int)orbs.size(), orbs.data(), in.data(), out.data(),
+    edh_check(edh_local_op(ctx_.get(), s.nup(), s.ndown(), op, (int)orbs.size(), orbs.data(), in.data(), out.data(),
                            EDH_MEM_HOST));
   }
   // Per-eigenvector static observables of a vector of the filled sector,
@@ -142,7 +183,7 @@ class HIPSpinResolvedStorage {
   // (norb each), M_i M_j (norb x norb), N_eff.
   void static_observables(const std::vector<prec>& v, int norb, std::vector<double>& out) {
     out.assign(5 * (size_t)norb + (size_t)norb * norb + 1, 0.0);
-    edh_throw(edh_static_observables(ctx_.get(), v.data(), (int64_t)v.size(), EDH_MEM_HOST, norb, out.data()));
+    edh_check(edh_static_observables(ctx_.get(), v.data(), (int64_t)v.size(), EDH_MEM_HOST, norb, out.data()));
   }
   // Continued fraction on the device (Lanczos.h:201-224).
   void cf_eval(const std::vector<double>& zre, const std::vector<double>& zim, double ev, int nlanc, int isign,
@@ -150,16 +191,10 @@ class HIPSpinResolvedStorage {
                std::vector<double>& im) {
     re.assign(zre.size(), 0.0);
     im.assign(zre.size(), 0.0);
-    edh_throw(edh_cf_eval(ctx_.get(), (int)zre.size(), zre.data(), zim.data(), ev, nlanc, isign, expb, alpha.data(),
+    edh_check(edh_cf_eval(ctx_.get(), (int)zre.size(), zre.data(), zim.data(), ev, nlanc, isign, expb, alpha.data(),
                           beta.data(), re.data(), im.data()));
   }
 
-  const std::vector<prec>& eigenvalues() const { return evals_; }
-  std::vector<prec>& eigenvalues() { return evals_; }
-  const std::vector<std::vector<prec>>& eigenvectors() const { return evecs_; }
-  std::vector<std::vector<prec>>& eigenvectors() { return evecs_; }
-  int& n() { return n_; }
-  int& ntot() { return ntot_; }
   edh_ctx* context() { return ctx_.get(); }
 
  private:
@@ -168,11 +203,8 @@ class HIPSpinResolvedStorage {
   };
   Model& model_;
   std::unique_ptr<edh_ctx, Destroy> ctx_;
-  int nev_, ncv_, eval_only_, Ns_;
-  int n_ = 0, ntot_ = 0;
+  int nev_ = 0, ncv_ = 0, eval_only_ = 0, Ns_ = 0;
   bool filled_ = false;
-  std::vector<prec> evals_;
-  std::vector<std::vector<prec>> evecs_;
 };
 
 }  // namespace Storage

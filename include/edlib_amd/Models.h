@@ -78,25 +78,112 @@ class FermionicModel {
   int Ns_, ms_, Ip_;
 };
 
-// Holds the engine's model object.
+// One term of T_states() / V_states(): (kind, i, j, k, l, s, sp) + value
+// as the engine lists them, in the reference's order.  kind 0 is the hop
+// c^+_{j s} c_{i s} (Hubbard::InnerState, HubbardModel.h:22-37, and
+// InnerHybridizationState, SingleImpurityAndersonModel.h:74-103); kind 1 is
+// c^+_{i s} c^+_{j sp} c_{l sp} c_{k s} with value U/2
+// (InnerInteractionState, SingleImpurityAndersonModel.h:105-184).
+template <typename prec>
+class InnerState {
+ public:
+  InnerState(const int* ints, double v) : kind_(ints[0]), i_(ints[1]), j_(ints[2]), k_(ints[3]), l_(ints[4]),
+                                          s_(ints[5]), sp_(ints[6]), value_((prec)v) {}
+  std::pair<int, int> indicies() const { return std::make_pair(i_, j_); }
+  int spin() const { return s_; }
+  int i() const { return i_; }
+  int j() const { return j_; }
+  int k() const { return k_; }
+  int l() const { return l_; }
+  prec U() const { return value_; }
+  prec value() const { return kind_ == 0 ? value_ : prec(0.5) * value_; }
+  int kind() const { return kind_; }
+  int sigma() const { return s_; }
+  int sigmaprime() const { return sp_; }
+
+ private:
+  int kind_, i_, j_, k_, l_, s_, sp_;
+  prec value_;
+};
+
+// Holds the engine's model object and the term lists read from it.
+template <typename prec>
 class EngineModel {
  public:
-  const edh_model* engine_model() const { return m_.get(); }
+  typedef InnerState<prec> St;
+  const std::vector<St>& T_states() const { return T_; }
+  const std::vector<St>& V_states() const { return V_; }
 
  protected:
   struct Free {
     void operator()(edh_model* m) const { edh_model_free(m); }
   };
   std::unique_ptr<edh_model, Free> m_;
-  void adopt(edh_model* m) { m_.reset(m); }
+  std::vector<St> T_, V_;
+  int hop_rule_ = -1;  // -s1 s2 (HubbardModel.h:104-105) or +s1 s2 (SingleImpurityAndersonModel.h:83)
+  void adopt(edh_model* m, int hop_rule) {
+    m_.reset(m);
+    hop_rule_ = hop_rule;
+    for (int which = 0; which < 2; ++which) {
+      int cnt = 0;
+      edh_throw(edh_model_transitions(m, which, &cnt, nullptr, nullptr));
+      std::vector<int> ints(7 * (size_t)cnt);
+      std::vector<double> vals(cnt);
+      if (cnt) edh_throw(edh_model_transitions(m, which, &cnt, ints.data(), vals.data()));
+      for (int q = 0; q < cnt; ++q) (which ? V_ : T_).push_back(St(ints.data() + 7 * q, vals[q]));
+    }
+  }
+  // valid / set on a full 2Ns-bit state (FermionicModel.h:35-68 signs)
+  static int occ(long long st, int o, int Ip) { return (int)((st >> (Ip - 1 - o)) & 1ll); }
+  static int sgn(long long st, int o, int Ip) {
+    return (__builtin_popcountll((unsigned long long)st >> (Ip - o)) & 1) ? -1 : 1;
+  }
+  int term_valid(const St& t, long long nst, int Ns) const {
+    const int Ip = 2 * Ns;
+    if (t.kind() == 0) return occ(nst, t.i() + t.spin() * Ns, Ip) * (1 - occ(nst, t.j() + t.spin() * Ns, Ip));
+    const int ok = t.k() + t.sigma() * Ns, ol = t.l() + t.sigmaprime() * Ns, oj = t.j() + t.sigmaprime() * Ns,
+              oi = t.i() + t.sigma() * Ns;
+    if (!occ(nst, ok, Ip)) return 0;
+    const long long a = nst - (1ll << (Ip - 1 - ok));
+    if (!occ(a, ol, Ip)) return 0;
+    const long long b = a - (1ll << (Ip - 1 - ol));
+    if (occ(b, oj, Ip)) return 0;
+    return 1 - occ(b | (1ll << (Ip - 1 - oj)), oi, Ip);
+  }
+  prec term_set(const St& t, long long nst, long long& k, int& sign, int Ns) const {
+    const int Ip = 2 * Ns;
+    if (t.kind() == 0) {
+      const int oi = t.i() + t.spin() * Ns, oj = t.j() + t.spin() * Ns;
+      const int s1 = sgn(nst, oi, Ip);
+      const long long k1 = nst - (1ll << (Ip - 1 - oi));
+      sign = hop_rule_ * s1 * sgn(k1, oj, Ip);
+      k = k1 + (1ll << (Ip - 1 - oj));
+      return t.value();
+    }
+    const int ok = t.k() + t.sigma() * Ns, ol = t.l() + t.sigmaprime() * Ns, oj = t.j() + t.sigmaprime() * Ns,
+              oi = t.i() + t.sigma() * Ns;
+    int g = sgn(nst, ok, Ip);
+    long long a = nst - (1ll << (Ip - 1 - ok));
+    g *= sgn(a, ol, Ip);
+    a -= 1ll << (Ip - 1 - ol);
+    g *= sgn(a, oj, Ip);
+    a += 1ll << (Ip - 1 - oj);
+    g *= sgn(a, oi, Ip);
+    k = a + (1ll << (Ip - 1 - oi));
+    sign = g;
+    return t.value();
+  }
 };
 
 template <typename prcsn>
-class HubbardModel : public FermionicModel, public EngineModel {
+class HubbardModel : public FermionicModel, public EngineModel<prcsn> {
  public:
   typedef prcsn precision;
   typedef Symmetry::SzSymmetry SYMMETRY;
   typedef Symmetry::SzSymmetry::Sector Sector;
+  typedef InnerState<prcsn> St;
+  using EngineModel<prcsn>::T_states;
+  using EngineModel<prcsn>::V_states;
 
   // Reads hopping/values, interaction/values, chemical_potential/values and
   // the optional magnetic_field/values, exchange/values (HubbardModel.h:55-66).
@@ -109,7 +196,7 @@ class HubbardModel : public FermionicModel, public EngineModel {
     if (in.is_data("exchange/values")) J = vec(in, "exchange/values", Ns_ * Ns_);
     edh_model* m = nullptr;
     edh_throw(edh_model_hubbard(&m, Ns_, t.data(), U.data(), J.data(), xmu.data(), H.data()));
-    adopt(m);
+    this->adopt(m, -1);
     t_ = t;
     xmu_ = xmu;
   }
@@ -128,8 +215,13 @@ class HubbardModel : public FermionicModel, public EngineModel {
   }
   precision diagonal(long long state) const {
     double x = 0;
-    edh_throw(edh_model_diagonal(m_.get(), state, &x));
+    edh_throw(edh_model_diagonal(this->m_.get(), state, &x));
     return (precision)x;
+  }
+  // HubbardModel::valid / set (HubbardModel.h:86-108)
+  int valid(const St& state, long long nst) { return this->term_valid(state, nst, Ns_); }
+  precision set(const St& state, long long nst, long long& k, int& sign) {
+    return this->term_set(state, nst, k, sign, Ns_);
   }
   int interacting_orbitals() const { return Ns_; }
   Symmetry::SzSymmetry& symmetry() { return symmetry_; }
@@ -146,11 +238,14 @@ class HubbardModel : public FermionicModel, public EngineModel {
 };
 
 template <typename prcsn>
-class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
+class SingleImpurityAndersonModel : public FermionicModel, public EngineModel<prcsn> {
  public:
   typedef prcsn precision;
   typedef Symmetry::SzSymmetry SYMMETRY;
   typedef Symmetry::SzSymmetry::Sector Sector;
+  typedef InnerState<prcsn> St;
+  using EngineModel<prcsn>::T_states;
+  using EngineModel<prcsn>::V_states;
 
   // Reads Bath/Epsk, Bath/Vk_m, H0_m, mu and interaction/values, with the
   // reference's argument checks (SingleImpurityAndersonModel.h:195-230).
@@ -184,7 +279,7 @@ class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
     const double mu = in.get("mu").data.at(0);
     edh_model* m = nullptr;
     edh_throw(edh_model_siam(&m, Ns_, ml_, e.data.data(), Vk.data(), H0.data(), U.data.data(), mu));
-    adopt(m);
+    this->adopt(m, +1);
     epsk_ = e.data;
     vk_ = Vk;
     h0_ = H0;
@@ -209,8 +304,13 @@ class SingleImpurityAndersonModel : public FermionicModel, public EngineModel {
   }
   precision diagonal(long long state) const {
     double x = 0;
-    edh_throw(edh_model_diagonal(m_.get(), state, &x));
+    edh_throw(edh_model_diagonal(this->m_.get(), state, &x));
     return (precision)x;
+  }
+  // SingleImpurityAndersonModel::valid / set (:346-357)
+  int valid(const St& state, long long nst) { return this->term_valid(state, nst, Ns_); }
+  precision set(const St& state, long long nst, long long& k, int& sign) {
+    return this->term_set(state, nst, k, sign, Ns_);
   }
   int interacting_orbitals() const { return ml_; }
   Symmetry::SzSymmetry& symmetry() { return symmetry_; }

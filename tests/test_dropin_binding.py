@@ -1,0 +1,125 @@
+"""HIPSpinResolvedStorage bound to models with only the reference models'
+public interface (tests/dropin/ref_models.h: HubbardModel.h:20-222 and
+SingleImpurityAndersonModel.h:15-417 as classes, no accessor of this
+build).  The storage builds its engine model through the transition-list
+entry (edh_model_from_terms) from T_states()/V_states()/valid/set/diagonal().
+
+CPU: the engine model so built equals the engine's own model builder on the
+same arrays -- every sector's per-spin CSR bit for bit, the diagonal, the
+off-diagonal action on every Fock state (bindtest tables).
+GPU: Hamiltonian<HIPSpinResolvedStorage<test model>> over all sectors, E0 of
+the 4-ring (HubbardModel_Test.cpp:54) and G against the oracle.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.models import kanamori_tensor, random_hubbard_params, siam_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "edlib_amd", "bin", "bindtest")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _build():
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "dropin")])
+    assert os.path.exists(BIN)
+
+
+def _ds(a):
+    a = np.asarray(a, dtype=np.float64)
+    return {"shape": list(a.shape), "data": a.reshape(-1).tolist()}
+
+
+def write_hubbard_json(path, p):
+    d = {"hopping/values": _ds(p["t"]), "interaction/values": _ds(p["U"]),
+         "chemical_potential/values": _ds(p["xmu"]), "magnetic_field/values": _ds(p["Hmag"]),
+         "exchange/values": _ds(p["J"])}
+    with open(path, "w") as f:
+        json.dump(d, f)
+
+
+def write_siam_json(path, p):
+    ml, Ns = p["ml"], p["Ns"]
+    d = {"Bath/Epsk/values": _ds(p["Epsk"]), "mu": _ds(np.array(p["mu"])), "interaction/values": _ds(p["U"])}
+    for im in range(ml):
+        d["Bath/Vk_%d/values" % im] = _ds(p["Vk"][im])
+        d["H0_%d/values" % im] = _ds(p["H0"][im])
+    with open(path, "w") as f:
+        json.dump(d, f)
+
+
+def _run(args, timeout=120):
+    r = subprocess.run([BIN] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    return r, json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _cases(tmp_path):
+    rng = np.random.default_rng(41)
+    out = [("hubbard", os.path.join(GOLDEN, "GF_Chi_input.json"), 4, None),
+           ("hubbard", os.path.join(GOLDEN, "4ring_input.json"), 4, None)]
+    p = random_hubbard_params(5, rng)  # random t, U, mu, H and exchange J
+    f = str(tmp_path / "hub5.json")
+    write_hubbard_json(f, p)
+    out.append(("hubbard", f, 5, None))
+    p = siam_params(6, 3, rng, h0_offdiag=0.2)  # Kanamori 3 orbitals + 3 bath sites
+    f = str(tmp_path / "kan6.json")
+    write_siam_json(f, p)
+    out.append(("siam", f, 6, 3))
+    p = siam_params(5, 1, rng)
+    f = str(tmp_path / "aim5.json")
+    write_siam_json(f, p)
+    out.append(("siam", f, 5, 1))
+    return out
+
+
+def test_storage_builds_engine_model_from_reference_interface(tmp_path):
+    _build()
+    for kind, f, Ns, ml in _cases(tmp_path):
+        args = ["tables", kind, f, Ns] + ([ml] if ml else [])
+        r, res = _run(args)
+        assert r.returncode == 0 and res["ok"], (kind, f, res)
+        assert res["sectors"] == (Ns + 1) ** 2
+
+
+@pytest.mark.gpu
+def test_hamiltonian_with_reference_interface_models(tmp_path):
+    from oracle import edoracle as eo
+    from tests.fixtures import hubbard_arrays, load_input
+    assert os.path.exists(BIN), "bindtest not built (run __graft_entry__.build())"
+    # 4-ring: E0 = -11.8443 in sector (2, 2) (HubbardModel_Test.cpp:53-56)
+    r, res = _run(["hamiltonian", "hubbard", os.path.join(GOLDEN, "4ring_input.json"), 4, 2])
+    assert r.returncode == 0 and res["ok"], res
+    e0, nu, nd = res["eigenpairs"][0]
+    assert abs(e0 - (-11.8443)) < 1e-4 and (nu, nd) == (2, 2)
+    # GF_Chi model: every eigenvalue and G_00 against the oracle
+    d = load_input("GF_Chi_input.json")
+    Ns, t, U, xmu, H, J = hubbard_arrays(d)
+    m = eo.hubbard(Ns, t, U, xmu, H, J)
+    params = {"NSITES": 4, "arpack.NEV": 100, "lanc.BETA": 20.0, "lanc.NOMEGA": 16, "lanc.NLANC": 100,
+              "lanc.BOLTZMANN_CUTOFF": 1e-12}
+    pairs = eo.hamiltonian_diag(m, params)
+    r, res = _run(["hamiltonian", "hubbard", os.path.join(GOLDEN, "GF_Chi_input.json"), 4, 100])
+    assert r.returncode == 0 and res["ok"], res
+    got = np.array([e[0] for e in res["eigenpairs"]])
+    want = np.array([p.eigenvalue for p in pairs[:len(got)]])
+    assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < 1e-10
+    G = eo.greens_function(m, pairs, params, gf_orbs=[0])["G"][:16, 0, 0]
+    g = np.array([complex(a, b) for a, b in res["G00"]])
+    assert np.max(np.abs(g - G)) < 1e-8
+    # Kanamori 3 orbitals + 2 bath sites: ordered eigenvalues against the oracle
+    rng = np.random.default_rng(43)
+    p = siam_params(5, 3, rng, h0_offdiag=0.2)
+    f = str(tmp_path / "kan5.json")
+    write_siam_json(f, p)
+    r, res = _run(["hamiltonian", "siam", f, 5, 3, 4])
+    assert r.returncode == 0 and res["ok"], res
+    ms = eo.siam(5, 3, p["Epsk"], p["Vk"], p["H0"], p["U"], p["mu"])
+    pairs = eo.hamiltonian_diag(ms, {"NSITES": 5, "arpack.NEV": 4})
+    got = np.array([e[0] for e in res["eigenpairs"]])
+    want = np.array([q.eigenvalue for q in pairs[:len(got)]])
+    assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < 1e-10
