@@ -43,6 +43,11 @@ _SIGS = {
     "edh_model_diagonal": (ctypes.c_int, [_vp, ctypes.c_longlong, _dp]),
     "edh_sector_csr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, _i64p, _ip, _ip, _dp]),
     "edh_sector_diagonal": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "edh_model_from_terms": (ctypes.c_int, [ctypes.POINTER(_vp), _vp]),
+    "edh_model_apply": (ctypes.c_int, [_vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _dp,
+                                       _ip]),
+    "edh_device_memory": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.POINTER(ctypes.c_size_t)]),
     "edh_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int]),
     "edh_destroy": (ctypes.c_int, [_vp]),
     "edh_set_model": (ctypes.c_int, [_vp, _vp]),

@@ -257,6 +257,17 @@ int edh_sector_diagonal(const edh_model* m, int nup, int ndn, double* out) {
   API_END
 }
 
+int edh_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+  API_BEGIN
+  EDH_CHECK(free_bytes && total_bytes, EDH_ERR_ARG, "null argument");
+  int n = 0;
+  EDH_HIP(hipGetDeviceCount(&n));
+  EDH_CHECK(device >= 0 && device < n, EDH_ERR_ARG, "invalid device index");
+  EDH_HIP(hipSetDevice(device));
+  EDH_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  API_END
+}
+
 int edh_create(edh_ctx** out, int device) {
   API_BEGIN
   EDH_CHECK(out, EDH_ERR_ARG, "null argument");

@@ -253,6 +253,24 @@ class BatchedSolve:
             for key, jid in zip(keys, self.b.submit_gf(reqs, self.p.nlanc)):
                 self.gf[key] = jid
 
+    def abort(self):
+        """Stops and frees every process and sector of this solve (after a
+        failure), leaving the batch's other solves untouched."""
+        jobs = [j for _, j in self.eig_jobs] + [j for j in self.gf.values() if j >= 0]
+        for j in jobs:
+            for fn in (self.b.cancel, self.b.free_job):
+                try:
+                    fn(j)
+                except Exception:  # noqa: BLE001 -- already finished or freed
+                    pass
+        for sid in self.sid.values():
+            try:
+                self.b.remove_sector(sid)
+            except Exception:  # noqa: BLE001 -- already removed
+                pass
+        self.eig_jobs, self.gf, self.sid = [], {}, {}
+        self.phase = "failed"
+
     def advance(self):
         """Moves the solve on; returns True when it is complete."""
         p = self.p

@@ -126,6 +126,10 @@ int edh_sector_diagonal(const edh_model* m, int nup, int ndn, double* out);
 
 /* ---- Context ------------------------------------------------------------ */
 
+/* Free and total device memory of GPU `device` (hipMemGetInfo), for
+ * admission by footprint in a server (no counterpart in the reference). */
+int edh_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+
 int edh_create(edh_ctx** out, int device);
 int edh_destroy(edh_ctx* ctx);
 
