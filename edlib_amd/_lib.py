@@ -65,6 +65,7 @@ _SIGS = {
     "edh_set_hv_variant": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edh_av_time": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "edh_vv": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _dp, ctypes.c_int]),
+    "edh_vv_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
     "edh_lanczos": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int, _dp, _dp, _ip, ctypes.c_int]),
     "edh_lanczos_time": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _dp, _dp, _ip, _dp]),
     "edh_adag": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _i64p,

@@ -279,6 +279,7 @@ int edh_create(edh_ctx** out, int device) {
   EDH_HIP(hipSetDevice(device));
   EDH_HIP(hipStreamCreate(&c->c.stream));
   edh::configure_mem_pool(device);
+  edh::noalloc_register_device(device);
   *out = c.release();
   API_END
 }
@@ -486,6 +487,16 @@ int edh_vv(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* ou
   dev_dot(da, db, n, c.partial.as<double>(), res, c.stream);
   EDH_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   EDH_HIP(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+int edh_vv_noalloc(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* out) {
+  API_BEGIN
+  require_ctx(ctx);
+  Ctx& c = ctx->c;
+  use_device(c);
+  EDH_CHECK(a && b && out, EDH_ERR_ARG, "null device pointer");
+  noalloc_dot(c.device, a, b, n, out, c.stream);
   API_END
 }
 

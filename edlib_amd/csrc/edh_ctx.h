@@ -89,4 +89,9 @@ void adag_gather_maps(int Ns, int nup, int ndn, int orb, bool annihilate, int& n
                       std::vector<int32_t>& mu, std::vector<int32_t>& md, std::vector<int8_t>& su,
                       std::vector<int8_t>& sd);
 
+// Allocation-free inner product (noalloc.hip): scratch array resolved per
+// device at context creation.
+void noalloc_register_device(int device);
+void noalloc_dot(int device, const double* a, const double* b, int64_t n, double* out, hipStream_t s);
+
 }  // namespace edh

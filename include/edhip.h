@@ -183,6 +183,14 @@ int edh_av_time(edh_ctx* ctx, const double* v, double* w, int clear, int warmup,
 /* SpinResolvedStorage::vv (SpinResolvedStorage.h:390-400): *out = <a, b>. */
 int edh_vv(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* out, int mem);
 
+/* edh_vv on device vectors without allocating: *out (a device scalar) =
+ * <a, b>, queued on the context's stream, the reduction partials in a
+ * static device array of 2048 doubles (16 KiB) per GPU instead of context
+ * buffers.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Calls that overlap on one GPU share that array and race. */
+int edh_vv_noalloc(edh_ctx* ctx, const double* a, const double* b, int64_t n, double* out);
+
 /* Lanczos::lanczos (Lanczos.h:36-77) on the current sector from the
  * normalised start vector v0 (length n): alpha[nl], beta[nl+1] (beta[0]
  * unused), *nlanc = number of steps taken (breaks when |beta| < 1e-10).
