@@ -48,6 +48,17 @@ def load_traffic(variant):
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(t, U, xmu, v, w, B):
     """SpinResolvedStorage::av restatement (oracle), OpenMP over up-rows, on
     this host: 3 accumulate-mode calls over the full sector."""
@@ -63,7 +74,75 @@ def cpu_baseline(t, U, xmu, v, w, B):
     return {"value": B / dt / 1e9, "unit": "GB/s", "cores": int(eo.lib().orc_num_threads()), "kind": "port",
             "sample": "3 accumulate-mode av calls over the full 165,636,900-state sector (oracle restatement of "
                       "SpinResolvedStorage::av, OpenMP over up-rows; SOCRSStorage needs ~106 GB here)",
-            "ms_per_call": dt * 1e3}
+            "ms_per_call": dt * 1e3, "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def cpu_socrs_configs():
+    """SOCRSStorage restatement (oracle; SOCRSStorage.h:46-87 av with the
+    :114-160 fill, OpenMP over static row chunks) on the largest sector of
+    each smaller workload: accumulate-mode av GB/s at 24 N bytes per call,
+    median of 5 calls after one warm-up (fill time reported apart)."""
+    from oracle import edoracle as eo
+    from edlib_amd.workloads import siam_arrays
+    ring = np.zeros((8, 8))
+    for i in range(8):
+        ring[i, (i + 1) % 8] = ring[(i + 1) % 8, i] = 1.0
+    tt = torus(3, 4)
+    for r in range(3):
+        for c in range(4):
+            i = r * 4 + c
+            for (dr, dc) in ((1, 1), (1, -1)):
+                j = ((r + dr) % 3) * 4 + (c + dc) % 4
+                tt[i, j] = tt[j, i] = -0.3
+    cfgs = [("8-site ring, sector (4,4)", lambda: eo.hubbard(8, ring, np.full(8, 4.0), np.full(8, 2.0)), (4, 4)),
+            ("12-site 3x4 torus t,t', sector (6,6)", lambda: eo.hubbard(12, tt, np.full(12, 8.0), np.full(12, 4.0)),
+             (6, 6)),
+            ("AIM 1+11, sector (6,6)", lambda: eo.siam(*siam_arrays(12, 1, SEED + 4)), (6, 6)),
+            ("Kanamori 3+11, sector (7,7)", lambda: eo.siam(*siam_arrays(14, 3, SEED + 1000)), (7, 7))]
+    out = []
+    for name, mk, sec in cfgs:
+        m = mk()
+        t0 = time.perf_counter()
+        s = eo.SOCRS(m, *sec)
+        tf = time.perf_counter() - t0
+        v = np.random.default_rng(SEED + 1).uniform(-1, 1, s.n)
+        w = np.random.default_rng(SEED + 2).uniform(-1, 1, s.n)
+        w = s.av(v, w, clear=False)
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            w = s.av(v, w, clear=False)
+            ts.append(time.perf_counter() - t0)
+        dt = float(np.median(ts))
+        out.append({"workload": name, "dim": s.n, "av_ms": round(dt * 1e3, 3), "gbs": round(24 * s.n / dt / 1e9, 3),
+                    "fill_s": round(tf, 2)})
+        del s
+    return out
+
+
+def cpu_solve_estimate(hv_by_sector, nlanc):
+    """CPU time of the AIM 1+11 batched solve's H*v work: for every sector,
+    the H*v count the device solve needed there (eigensolver and
+    Green's-function steps) times one SOCRSStorage av of that sector on this
+    host (oracle restatement, OpenMP), summed.  A lower bound of a CPU
+    solve: ARPACK's re-orthogonalisation and the host loops are not counted."""
+    from oracle import edoracle as eo
+    from edlib_amd.workloads import siam_arrays
+    m = eo.siam(*siam_arrays(12, 1, SEED + 4))
+    tot, nav = 0.0, 0
+    for sec, cnt in sorted(hv_by_sector.items()):
+        s = eo.SOCRS(m, *sec)
+        v = np.random.default_rng(SEED + 3).uniform(-1, 1, s.n)
+        w = s.av(v)
+        t0 = time.perf_counter()
+        s.av(v, w, clear=False)
+        tot += (time.perf_counter() - t0) * cnt
+        nav += cnt
+        del s
+    return {"cpu_s_per_solve_lower_bound": round(tot, 3), "solves_per_s_upper_bound": round(1.0 / tot, 5),
+            "hv_count": nav,
+            "what": "sum over sectors of (H*v count of the device solve in that sector) x (one SOCRS av of that "
+                    "sector on this host); excludes ARPACK's orthogonalisation, so a CPU solve is slower"}
 
 
 
@@ -91,7 +170,7 @@ def solves_section(args):
         "max_abs_dG": float(np.abs(rb["G"] - rp["G"]).max()),
         "max_rel_dE": float(np.max(np.abs(rb["energies"] - rp["energies"]) /
                                    np.maximum(np.abs(rp["energies"]), 1.0))),
-        "n_gf": rb["n_gf"]}
+        "n_gf": rb["n_gf"], "hv_by_sector": rb["hv_by_sector"]}
     # served: Kanamori 3 orbitals + 6..8 bath sites, Poisson arrivals
     sp4 = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0, 1, 2), seed=SEED)
     rng = np.random.default_rng(SEED + 7)
@@ -268,11 +347,18 @@ def main():
         vb.upload(v)
         out["large_sector"] = large_sector_section(ea.HubbardModel(Ns, t, U, xmu), ctx, vb, args)
     ctx.close()
+    if not args.no_solves:
+        out["solves"] = solves_section(args)
     if not args.no_cpu_baseline:
         wcpu = w.copy()
         out["cpu_baseline"] = cpu_baseline(t, U, xmu, v, wcpu, B)
-    if not args.no_solves:
-        out["solves"] = solves_section(args)
+        out["cpu_baseline"]["per_config"] = cpu_socrs_configs()
+        hv = out.get("solves", {}).get("batched", {}).pop("hv_by_sector", None)
+        if hv:
+            est = cpu_solve_estimate(hv, 60)
+            est["vs_batched_gpu"] = round(out["solves"]["batched"]["batched_solves_per_s"] /
+                                          est["solves_per_s_upper_bound"], 1)
+            out["cpu_baseline"]["aim12_solve"] = est
     print(json.dumps(out), flush=True)
 
 

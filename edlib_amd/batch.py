@@ -204,19 +204,27 @@ class Batch:
         return blk.value
 
     def cf_many(self, job_ids, ev, isign, expb, excited, wn):
-        """Continued fractions of finished Lanczos runs at z = excited*isign + i wn;
-        returns a (len(job_ids), len(wn)) complex array."""
+        """Continued fractions of finished Lanczos runs at z = p + excited*isign
+        for each frequency point p: real `wn` are Matsubara frequencies
+        (p = i wn), complex ones are the points themselves (the real axis's
+        w + i pi/beta); returns a (len(job_ids), len(wn)) complex array."""
         k, nz = len(job_ids), len(wn)
         if k == 0:
             return np.zeros((0, nz), dtype=complex)
         ids = np.ascontiguousarray(job_ids, dtype=np.int32)
         isg = np.ascontiguousarray(isign, dtype=np.int32)
         evs, exb, exc = (np.ascontiguousarray(x, dtype=np.float64) for x in (ev, expb, excited))
-        w = np.ascontiguousarray(wn, dtype=np.float64)
         re, im = np.zeros((k, nz)), np.zeros((k, nz))
         ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-        check(_lib.lib().edh_batch_cf_many(self._h, k, ip(ids), _d(evs), ip(isg), _d(exb), _d(exc), nz, _d(w),
-                                           _d(re), _d(im)))
+        if np.iscomplexobj(wn):
+            zr = np.ascontiguousarray(np.real(wn), dtype=np.float64)
+            zi = np.ascontiguousarray(np.imag(wn), dtype=np.float64)
+            check(_lib.lib().edh_batch_cf_many_z(self._h, k, ip(ids), _d(evs), ip(isg), _d(exb), _d(exc), nz, _d(zr),
+                                                 _d(zi), _d(re), _d(im)))
+        else:
+            w = np.ascontiguousarray(wn, dtype=np.float64)
+            check(_lib.lib().edh_batch_cf_many(self._h, k, ip(ids), _d(evs), ip(isg), _d(exb), _d(exc), nz, _d(w),
+                                               _d(re), _d(im)))
         return re + 1j * im
 
     def cf_eval(self, z, ev, nlanc, isign, expb, alpha, beta):

@@ -1054,13 +1054,12 @@ int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block) {
   API_END
 }
 
-int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
-                      const double* expb, const double* excited, int nz, const double* wn, double* out_re,
-                      double* out_im) {
-  API_BEGIN
+namespace {
+void batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign, const double* expb,
+                   const double* excited, int nz, const double* wr, const double* wn, double* out_re, double* out_im) {
   edh::Batch& B = use_batch(b);
   EDH_CHECK(njobs >= 0 && nz >= 0, EDH_ERR_ARG, "bad argument");
-  if (njobs == 0 || nz == 0) return EDH_OK;
+  if (njobs == 0 || nz == 0) return;
   EDH_CHECK(job_ids && ev && isign && expb && excited && wn && out_re && out_im, EDH_ERR_ARG, "null argument");
   std::vector<size_t> off(njobs);
   size_t tot = 0;
@@ -1077,12 +1076,13 @@ int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double*
     std::copy(J.alpha.begin(), J.alpha.end(), coef.begin() + off[j]);
     std::copy(J.beta.begin(), J.beta.end(), coef.begin() + off[j] + J.alpha.size());
   }
-  const size_t need = tot * 8 + (size_t)nz * 8 + (size_t)njobs * sizeof(edh::CfJob) + 2 * (size_t)njobs * nz * 8 + 64;
+  const size_t need = tot * 8 + 2 * (size_t)nz * 8 + (size_t)njobs * sizeof(edh::CfJob) + 2 * (size_t)njobs * nz * 8 + 64;
   B.d_scal.ensure(need);
   char* base = B.d_scal.as<char>();
   double* dcoef = (double*)base;
   double* dwn = dcoef + tot;
-  edh::CfJob* djobs = (edh::CfJob*)(dwn + nz);
+  double* dwr = dwn + nz;
+  edh::CfJob* djobs = (edh::CfJob*)(dwr + nz);
   double* dre = (double*)(djobs + njobs);
   double* dim = dre + (size_t)njobs * nz;
   std::vector<edh::CfJob> cj(njobs);
@@ -1092,11 +1092,29 @@ int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double*
   }
   EDH_HIP(hipMemcpyAsync(dcoef, coef.data(), tot * 8, hipMemcpyHostToDevice, B.stream));
   EDH_HIP(hipMemcpyAsync(dwn, wn, (size_t)nz * 8, hipMemcpyHostToDevice, B.stream));
+  if (wr) EDH_HIP(hipMemcpyAsync(dwr, wr, (size_t)nz * 8, hipMemcpyHostToDevice, B.stream));
   EDH_HIP(hipMemcpyAsync(djobs, cj.data(), cj.size() * sizeof(edh::CfJob), hipMemcpyHostToDevice, B.stream));
-  edh::dev_cf_batch(njobs, nz, dwn, djobs, dre, dim, B.stream);
+  edh::dev_cf_batch(njobs, nz, wr ? dwr : nullptr, dwn, djobs, dre, dim, B.stream);
   EDH_HIP(hipMemcpyAsync(out_re, dre, (size_t)njobs * nz * 8, hipMemcpyDeviceToHost, B.stream));
   EDH_HIP(hipMemcpyAsync(out_im, dim, (size_t)njobs * nz * 8, hipMemcpyDeviceToHost, B.stream));
   EDH_HIP(hipStreamSynchronize(B.stream));
+}
+}  // namespace
+
+int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
+                      const double* expb, const double* excited, int nz, const double* wn, double* out_re,
+                      double* out_im) {
+  API_BEGIN
+  batch_cf_many(b, njobs, job_ids, ev, isign, expb, excited, nz, nullptr, wn, out_re, out_im);
+  API_END
+}
+
+int edh_batch_cf_many_z(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
+                        const double* expb, const double* excited, int nz, const double* zre, const double* zim,
+                        double* out_re, double* out_im) {
+  API_BEGIN
+  EDH_CHECK(nz == 0 || zre, EDH_ERR_ARG, "null argument");
+  batch_cf_many(b, njobs, job_ids, ev, isign, expb, excited, nz, zre, zim, out_re, out_im);
   API_END
 }
 

@@ -44,8 +44,9 @@ struct CfJob {
   const double* alpha;
   const double* beta;
 };
-void dev_cf_batch(int njobs, int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im,
-                  hipStream_t s);
+// Run j at point q: z = (wr ? wr[q] : 0) + excited_j * isign_j + i wn[q].
+void dev_cf_batch(int njobs, int nz, const double* wr, const double* wn, const CfJob* jobs, double* out_re,
+                  double* out_im, hipStream_t s);
 void dev_cf_eval(int nz, const double* zre, const double* zim, double ev, int nlanc, int isign, double expb,
                  const double* alpha, const double* beta, double* out_re, double* out_im, hipStream_t s);
 

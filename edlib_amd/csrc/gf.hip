@@ -86,11 +86,14 @@ __global__ void cf_kernel(int nz, const double* zre, const double* zim, double e
 
 // Many continued fractions at once: block row j evaluates job j at the
 // points z = excited_j * isign_j + i wn[q] (the same recurrence as cf_kernel).
-__global__ void cf_batch_kernel(int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im) {
+__global__ void cf_batch_kernel(int nz, const double* wr, const double* wn, const CfJob* jobs, double* out_re,
+                                double* out_im) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const CfJob& J = jobs[blockIdx.y];
   if (q >= nz) return;
-  const cplx z = {J.excited * J.isign, wn[q]};
+  // z = freq_point + isign * E (Lanczos.h:146-148): i w_n on the Matsubara
+  // mesh (wr null), w + i pi/beta on the real axis
+  const cplx z = {(wr ? wr[q] : 0.0) + J.excited * J.isign, wn[q]};
   const double big = (double)FLT_MAX / 2.0;
   const double shift = 1.0 / ((double)FLT_MAX / 1000.0);
   const double* alpha = J.alpha;
@@ -123,10 +126,11 @@ __global__ void cf_batch_kernel(int nz, const double* wn, const CfJob* jobs, dou
   out_im[(size_t)blockIdx.y * nz + q] = res.im;
 }
 
-void dev_cf_batch(int njobs, int nz, const double* wn, const CfJob* jobs, double* out_re, double* out_im,
-                  hipStream_t s) {
+void dev_cf_batch(int njobs, int nz, const double* wr, const double* wn, const CfJob* jobs, double* out_re,
+                  double* out_im, hipStream_t s) {
   if (njobs <= 0 || nz <= 0) return;
-  hipLaunchKernelGGL(cf_batch_kernel, dim3((nz + 127) / 128, njobs), dim3(128), 0, s, nz, wn, jobs, out_re, out_im);
+  hipLaunchKernelGGL(cf_batch_kernel, dim3((nz + 127) / 128, njobs), dim3(128), 0, s, nz, wr, wn, jobs, out_re,
+                     out_im);
   EDH_HIP(hipGetLastError());
 }
 

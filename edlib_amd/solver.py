@@ -49,6 +49,9 @@ class SolveParams:
     orbitals: tuple = (0,)
     seed: int = 20261202
     sectors: list = field(default_factory=list)  # empty: every (N_up, N_dn)
+    mesh: str = "matsubara"  # or "real": nomega points on [emin, emax], z = w + i pi/beta
+    emin: float = -4.0
+    emax: float = 4.0
 
     @property
     def ncv_eff(self):
@@ -91,6 +94,19 @@ def sector_list(Ns, p):
 
 def matsubara(beta, nomega):
     return (2 * np.arange(nomega) + 1) * math.pi / beta
+
+
+def freq_points(p):
+    """The Green's function's frequency points (MeshFactory.h:19-34,
+    CommonUtils.h:14-20): i w_n = i (2n+1) pi / beta on the Matsubara mesh;
+    w + i pi/beta on the real axis, w on the linear grid [emin, emax] of
+    nomega points."""
+    if p.mesh == "matsubara":
+        return 1j * matsubara(p.beta, p.nomega)
+    if p.mesh == "real":
+        w = np.linspace(p.emin, p.emax, p.nomega) if p.nomega > 1 else np.array([p.emin])
+        return w + 1j * math.pi / p.beta
+    raise ValueError("mesh must be 'matsubara' or 'real'")
 
 
 def _expb(beta, e, e0):
@@ -143,7 +159,7 @@ def solve_plain(model, p, device=0, ctx=None):
     pairs = _order_pairs(pairs)
     e0 = pairs[0][0]
     Z = sum(math.exp(-(e - e0) * p.beta) for e, _, _ in pairs)
-    wn = matsubara(p.beta, p.nomega)
+    zpts = freq_points(p)
     norb = len(p.orbitals)
     Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
     Gl = np.zeros_like(Gg)
@@ -164,7 +180,7 @@ def solve_plain(model, p, device=0, ctx=None):
             ctx.set_sector(*nxt)
             nlanc, a, b = ctx.lanczos(out, p.nlanc)
             isign = -1 if ann else 1
-            g = ctx.cf_eval(1j * wn + e * isign, norm, nlanc, isign, _expb(p.beta, e, e0), a, b)
+            g = ctx.cf_eval(zpts + e * isign, norm, nlanc, isign, _expb(p.beta, e, e0), a, b)
             (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g
             n_gf += 1
     G = (Gg + Gl) / Z
@@ -299,7 +315,7 @@ class BatchedSolve:
         if self.phase == "gf":
             live = [self.gf[key] for key, _ in self.used if self.gf[key] >= 0]
             if not live or not np.any(self.b.job_states(live) == 0):
-                wn = matsubara(p.beta, p.nomega)
+                wn = freq_points(p) if p.mesh != "matsubara" else matsubara(p.beta, p.nomega)
                 norb = len(p.orbitals)
                 Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
                 Gl = np.zeros_like(Gg)
@@ -313,6 +329,20 @@ class BatchedSolve:
                                    [r[2] for r in rows], wn)
                 for q, (jid, (_, _, o, s, ann), _) in enumerate(rows):
                     (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
+                # H*v applications per sector (eigensolver runs, then
+                # Green's-function steps in their target sectors)
+                work = {}
+                jsec = {}
+                for sec, j in self.eig_jobs:
+                    jsec[j] = sec
+                    work[sec] = work.get(sec, 0) + self.b.job_info(j)["n_av"]
+                for key, jid in self.gf.items():
+                    if jid < 0 or jid not in live:
+                        continue
+                    j, _, _, s_, ann = key
+                    sec, d = jsec[j], (-1 if ann else 1)
+                    tgt = (sec[0] + d, sec[1]) if s_ == 0 else (sec[0], sec[1] + d)
+                    work[tgt] = work.get(tgt, 0) + self.b.job_info(jid)["count"]
                 for jid in live:
                     self.b.free_job(jid)
                 obs = _Observables()
@@ -327,7 +357,7 @@ class BatchedSolve:
                     self.b.remove_sector(sid)
                 self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
                                "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows),
-                               "observables": obs.result()}
+                               "observables": obs.result(), "hv_by_sector": work}
                 self.phase = "done"
         return self.phase == "done"
 

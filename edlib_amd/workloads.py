@@ -22,6 +22,11 @@ def siam_model(Ns, ml, seed, U=4.0, J=0.7):
     levels U[-0.3, 0.3]; Kanamori interaction (U, J, U' = U - 2J) for ml > 1,
     mu = U/2 (ml = 1) or (5U - 10J)/2 (ml = 3, half filling)."""
     import edlib_amd as ea
+    return ea.SingleImpurityAndersonModel(*siam_arrays(Ns, ml, seed, U, J))
+
+
+def siam_arrays(Ns, ml, seed, U=4.0, J=0.7):
+    """(Ns, ml, Epsk, Vk, H0, Utensor, mu) of siam_model."""
     rng = np.random.default_rng(seed)
     nb = Ns - ml
     eps = np.repeat(rng.uniform(-2, 2, nb)[:, None], 2, axis=1)
@@ -45,4 +50,4 @@ def siam_model(Ns, ml, seed, U=4.0, J=0.7):
     else:
         Ut = np.full((2, 2, 1, 1, 1, 1), U)
         mu = U / 2
-    return ea.SingleImpurityAndersonModel(Ns, ml, eps, Vk, H0, Ut, mu)
+    return Ns, ml, eps, Vk, H0, Ut, mu

@@ -331,6 +331,12 @@ int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block);
 int edh_batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
                       const double* expb, const double* excited, int nz, const double* wn, double* out_re,
                       double* out_im);
+/* edh_batch_cf_many at arbitrary points: run j at z = zre[q] + excited[j] *
+ * isign[j] + i zim[q] -- the real-frequency mesh's w_q + i pi/beta
+ * (CommonUtils.h:18-20, MeshFactory.h:27-34) or any other contour. */
+int edh_batch_cf_many_z(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign,
+                        const double* expb, const double* excited, int nz, const double* zre, const double* zim,
+                        double* out_re, double* out_im);
 /* Lanczos::get_frac_point on the batch's stream (as edh_cf_eval). */
 int edh_batch_cf_eval(edh_batch* b, int nz, const double* zre, const double* zim, double ev, int nlanc, int isign,
                       double expb, const double* alpha, const double* beta, double* out_re, double* out_im);

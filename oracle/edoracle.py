@@ -340,6 +340,11 @@ def matsubara_points(beta, nomega, fermionic=True):
 # ---------------------------------------------------------------------------
 # Storage::diag restatement (Storage.h:43-146) with scipy's ARPACK.
 # ---------------------------------------------------------------------------
+# The av of the ARPACK loop on host threads (the restatement of
+# SpinResolvedStorage::av over up-rows; same sums, row by row).
+AV_OMP = False
+
+
 def storage_diag(srs, nev_param, ncv_param=None, dense_limit=None):
     """Lowest eigenpairs of one sector with the reference's ARPACK parameters.
 
@@ -361,7 +366,7 @@ def storage_diag(srs, nev_param, ncv_param=None, dense_limit=None):
         e, V = np.linalg.eigh(0.5 * (H + H.T))
         return e[:nev].copy(), V[:, :nev].copy()
     from scipy.sparse.linalg import LinearOperator, eigsh
-    op = LinearOperator((n, n), matvec=lambda x: srs.av(np.asarray(x).reshape(-1)), dtype=np.float64)
+    op = LinearOperator((n, n), matvec=lambda x: srs.av(np.asarray(x).reshape(-1), omp=AV_OMP), dtype=np.float64)
     e, V = eigsh(op, k=nev, ncv=ncv, which="SA", tol=1e-14, maxiter=1000)
     order = np.argsort(e, kind="stable")
     return e[order], V[:, order]
@@ -623,6 +628,12 @@ def fock_hamiltonian(model):
 
 def lehmann_gf(model, beta, wn, orb, spin, nev_cut=None):
     """Exact G(iw) = sum_nm (e^-bEn + e^-bEm)|<m|c^+|n>|^2/(iw + En - Em) / Z."""
+    return lehmann_gf_z(model, beta, 1j * np.asarray(wn, dtype=float), orb, spin)
+
+
+def lehmann_gf_z(model, beta, z, orb, spin):
+    """The same Lehmann sum at arbitrary complex points z (the real axis's
+    w + i pi/beta of CommonUtils.h:18-20 included)."""
     Ns = model.Ns
     H = fock_hamiltonian(model)
     e, V = np.linalg.eigh(0.5 * (H + H.T))
@@ -638,8 +649,8 @@ def lehmann_gf(model, beta, wn, orb, spin, nev_cut=None):
             cnt = bin(s >> (pos + 1)).count("1")
             C[s | (1 << pos), s] = -1.0 if cnt % 2 else 1.0
     M = V.T @ C @ V  # M[m, n] = <m|c^+|n>
-    G = np.zeros(len(wn), complex)
-    for i, x in enumerate(wn):
-        den = 1j * x + e[None, :] - e[:, None]
+    G = np.zeros(len(z), complex)
+    for i, x in enumerate(z):
+        den = x + e[None, :] - e[:, None]
         G[i] = np.sum((w[None, :] + w[:, None]) * M ** 2 / den) / Z
     return G

@@ -8,6 +8,8 @@
 //       (edh_model_hubbard / edh_model_siam) on the same arrays: every
 //       sector's per-spin CSR bit for bit, every state's diagonal to 1e-12,
 //       and the off-diagonal action on every Fock state to 1e-14.
+//   bindtest sectors <Ns>
+//       no GPU: the drop-in SzSymmetry's sector order.
 //   bindtest hamiltonian hubbard <input.json> <Ns> <nev>
 //   bindtest hamiltonian siam <input.json> <Ns> <ml> <nev>
 //       GPU: Hamiltonian<HIPSpinResolvedStorage<Ref model>> diagonalised
@@ -133,6 +135,20 @@ static int run_hamiltonian(params& p) {
 
 int main(int argc, char** argv) {
   try {
+    if (argc == 3 && std::string(argv[1]) == "sectors") {
+      // the drop-in SzSymmetry's sector order (std::sort with the
+      // reference comparator, SzSymmetry.h:51-54,72-98)
+      params p;
+      define_parameters(p);
+      p["NSITES"] = std::atoi(argv[2]);
+      p["arpack.SECTOR"] = false;
+      Symmetry::SzSymmetry sym(p);
+      std::ostringstream o;
+      int q = 0;
+      while (sym.next_sector()) o << (q++ ? ", " : "") << "[" << sym.sector().nup() << ", " << sym.sector().ndown() << "]";
+      std::printf("{\"ok\": true, \"sectors\": [%s]}\n", o.str().c_str());
+      return 0;
+    }
     if (argc < 5) throw std::invalid_argument("usage: bindtest tables|hamiltonian hubbard|siam <input> <Ns> [ml] [nev]");
     const std::string what = argv[1], kind = argv[2], input = argv[3];
     const int Ns = std::atoi(argv[4]);

@@ -85,3 +85,98 @@ def test_reference_order_diagonal_is_bit_exact(kind, p):
     rng = np.random.default_rng(0)
     for st in rng.integers(0, 1 << (2 * p["Ns"]), 200):
         assert om.diagonal(int(st)) == pm.diagonal(int(st))
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("Ns", ctypes.c_int), ("interacting_orbitals", ctypes.c_int), ("n_hop", ctypes.c_int),
+                ("hop", ctypes.POINTER(ctypes.c_int)), ("hop_value", ctypes.POINTER(ctypes.c_double)),
+                ("n_int", ctypes.c_int), ("inter", ctypes.POINTER(ctypes.c_int)),
+                ("inter_value", ctypes.POINTER(ctypes.c_double)), ("e0", ctypes.c_double),
+                ("eps", ctypes.POINTER(ctypes.c_double)), ("same", ctypes.POINTER(ctypes.c_double)),
+                ("cross", ctypes.POINTER(ctypes.c_double))]
+
+
+def _crs_rows(Ns, nel, spin, terms):
+    """CRSMatrix::addElement + endLine/compress (CRSMatrix.h:46-98) over the
+    per-spin basis in fill_spin order (SpinResolvedStorage.h:622-638): a
+    duplicate (i, j) is summed in place, |value| < 1e-15 is removed, and the
+    element shifted into a removed slot is not examined."""
+    Ip = 2 * Ns
+    shift = Ns if spin == 0 else 0
+    basis = list(eo.basis(Ns, nel))
+    rank = {int(b): r for r, b in enumerate(basis)}
+    rowp, col, val = [0], [], []
+    for b in basis:
+        st = int(b) << shift
+        cols, vals = [], []
+        for (i, j, s, v) in terms:
+            if s != spin:
+                continue
+            oi, oj = i + s * Ns, j + s * Ns
+            bi, bj = Ip - 1 - oi, Ip - 1 - oj
+            if not (st >> bi) & 1 or (st >> bj) & 1:
+                continue
+            s1 = -1 if bin(st >> (bi + 1)).count("1") % 2 else 1
+            k1 = st - (1 << bi)
+            s2 = -1 if bin(k1 >> (bj + 1)).count("1") % 2 else 1
+            k = k1 + (1 << bj)
+            c = rank[k >> shift]
+            if v == 0.0:
+                continue
+            if c in cols:
+                q = max(q for q in range(len(cols)) if cols[q] == c)
+                vals[q] += s1 * s2 * v
+            else:
+                cols.append(c)
+                vals.append(s1 * s2 * v)
+        q = 0
+        while q < len(cols):
+            if abs(vals[q]) < 1e-15:
+                del cols[q]
+                del vals[q]
+            q += 1
+        col += cols
+        val += vals
+        rowp.append(len(col))
+    return np.array(rowp), np.array(col), np.array(val)
+
+
+def test_crs_compress_keeps_reference_quirk_on_cancelling_terms():
+    """Exactly cancelling terms (a generic model with duplicated hops):
+    the per-spin CSR keeps the reference's compress behaviour, including the
+    explicit zero left where two cancelled entries were adjacent."""
+    Ns = 4
+    terms = [(0, 1, 0, 0.5), (0, 1, 0, -0.5), (0, 2, 0, 0.75), (0, 2, 0, -0.75), (0, 3, 0, 1.25),
+             (1, 2, 0, 0.3), (2, 1, 0, 0.3), (1, 3, 1, 0.9), (1, 3, 1, -0.9), (2, 3, 1, 0.4)]
+    hop = np.array([[i, j, s] for i, j, s, _ in terms], dtype=np.int32).reshape(-1)
+    hv = np.array([v for *_, v in terms])
+    z2, z1 = np.zeros(2 * Ns * Ns), np.zeros(Ns * Ns)
+    eps = np.zeros(2 * Ns)
+    d = _Desc(Ns, Ns, len(terms), hop.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+              hv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 0, None, None, 0.0,
+              eps.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), z2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+              z1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    _lib.check(L.edh_model_from_terms(ctypes.byref(h), ctypes.byref(d)))
+    try:
+        saw_explicit_zero = False
+        for nup, ndn in [(1, 1), (2, 2), (1, 3), (3, 2)]:
+            for which, nel in ((0, nup), (1, ndn)):
+                nr, nnz = ctypes.c_int64(), ctypes.c_int64()
+                _lib.check(L.edh_sector_csr(h, nup, ndn, which, ctypes.byref(nr), ctypes.byref(nnz), None, None, None))
+                rp = np.zeros(nr.value + 1, dtype=np.int32)
+                col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+                val = np.zeros(max(nnz.value, 1))
+                _lib.check(L.edh_sector_csr(h, nup, ndn, which, ctypes.byref(nr), ctypes.byref(nnz),
+                                            rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            col.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            val.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+                rrp, rcol, rval = _crs_rows(Ns, nel, which, terms)
+                assert np.array_equal(rp, rrp)
+                assert np.array_equal(col[:nnz.value], rcol)
+                assert np.array_equal(val[:nnz.value], rval)
+                saw_explicit_zero |= bool(np.any(rval == 0.0))
+        assert saw_explicit_zero  # the quirk was exercised
+    finally:
+        L.edh_model_free(h)

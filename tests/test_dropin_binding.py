@@ -123,3 +123,22 @@ def test_hamiltonian_with_reference_interface_models(tmp_path):
     got = np.array([e[0] for e in res["eigenpairs"]])
     want = np.array([q.eigenvalue for q in pairs[:len(got)]])
     assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < 1e-10
+
+
+@pytest.mark.parametrize("Ns", [8, 12, 16])
+def test_dropin_sector_order_matches_oracle(Ns):
+    """The drop-in's SzSymmetry visits the sectors in the oracle's order
+    (std::sort with the reference's non-strict comparator on the i-major
+    list, SzSymmetry.h:51-54,72-98; a stable sort differs at most places)."""
+    from oracle import edoracle as eo
+    _build()
+    r, res = _run(["sectors", Ns])
+    assert r.returncode == 0
+    got = [tuple(x) for x in res["sectors"]]
+    want = eo.sector_order(Ns)
+    assert got == want
+    stable = sorted([(i, j) for i in range(Ns + 1) for j in range(Ns + 1)],
+                    key=lambda s: eo.cnk(Ns, s[0]) * eo.cnk(Ns, s[1]))
+    if Ns >= 12:
+        assert got != stable
+
