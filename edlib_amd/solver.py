@@ -163,6 +163,7 @@ def solve_plain(model, p, device=0, ctx=None):
     norb = len(p.orbitals)
     Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
     Gl = np.zeros_like(Gg)
+    wsum = np.zeros((norb, 2))
     n_gf = 0
     obs = _Observables()
     for e, sec, vec in pairs:
@@ -174,6 +175,7 @@ def solve_plain(model, p, device=0, ctx=None):
         for q, (o, s, ann, nxt) in enumerate(_gf_targets(Ns, sec, p.orbitals)):
             _, out = ctx.adag(sec, o + s * Ns, ann, vec)
             norm = ctx.vv(out, out)
+            wsum[p.orbitals.index(o), s] += bf * norm
             if not abs(norm) > 1e-10:
                 continue
             out /= math.sqrt(norm)
@@ -188,7 +190,7 @@ def solve_plain(model, p, device=0, ctx=None):
     if own:
         ctx.close()
     return {"E0": e0, "Z": Z, "G": G, "energies": np.array([q[0] for q in pairs]), "n_gf": n_gf, "n_av": n_av,
-            "observables": obs.result(), "t_eig": t_eig, "t_total": t_all}
+            "observables": obs.result(), "t_eig": t_eig, "t_total": t_all, "weights": wsum / Z}
 
 
 class BatchedSolve:
@@ -205,9 +207,15 @@ class BatchedSolve:
     started that later fall out of the window (their runs are cancelled).
     Contributions are summed in the pair order of the final eigenpair list,
     so the result does not depend on when sectors finished.
+
+    Green's-function runs hold three vectors of their target sector each
+    while they run; they are started in window order as long as the runs in
+    flight stay within `gf_budget` bytes (by default the allowance the
+    server's footprint estimate makes for them), the rest wait for runs to
+    finish.  A run's arithmetic does not depend on when it starts.
     """
 
-    def __init__(self, batch, model, p):
+    def __init__(self, batch, model, p, gf_budget=None):
         self.b = batch
         self.model = model
         self.p = p
@@ -218,6 +226,11 @@ class BatchedSolve:
         self.known = []      # (E, sector index, sector, eig job, k) of finished sectors
         self.seen = set()    # finished eig jobs
         self.gf = {}         # (eig job, k, orbital, spin, annihilate) -> gf job or -1
+        self.gf_queue = []   # (key, request, bytes) waiting for memory
+        self.gf_queued = set()
+        self.gf_live = {}    # running gf job -> bytes
+        big = max(binom(self.Ns, a) * binom(self.Ns, b) for a, b in self.secs) if self.secs else 0
+        self.gf_budget = gf_budget if gf_budget is not None else 3 * 8 * big * (4 * len(p.orbitals) * p.nev * 4)
         self.phase = "new"
         self.result = None
         c = p.cutoff
@@ -256,18 +269,45 @@ class BatchedSolve:
             bounds = bounds[~np.isnan(bounds)]
             if bounds.size:
                 e0 = min(e0, float(bounds.min()))
-        reqs, keys = [], []
-        for e, _, sec, j, k in self.known:
+        for e, _, sec, j, k in sorted(self.known, key=lambda q: q[0]):
             if e - e0 > self.window:
                 continue
             for o, s, ann, nxt in _gf_targets(self.Ns, sec, self.p.orbitals):
                 key = (j, k, o, s, ann)
-                if key not in self.gf:
-                    reqs.append((j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]))
-                    keys.append(key)
+                if key not in self.gf and key not in self.gf_queued:
+                    nbytes = 3 * 8 * binom(self.Ns, nxt[0]) * binom(self.Ns, nxt[1]) + 4096
+                    self.gf_queue.append((key, (j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]), nbytes))
+                    self.gf_queued.add(key)
+        self._pump()
+
+    def _pump(self, needed=None):
+        """Starts queued Green's-function runs while the runs in flight fit
+        the budget (at least one run at a time); with `needed`, queued runs
+        outside that set are dropped."""
+        if self.gf_live:
+            ids = list(self.gf_live)
+            for jid, st in zip(ids, self.b.job_states(ids)):
+                if st != 0:
+                    del self.gf_live[jid]
+        if needed is not None:
+            self.gf_queue = [q for q in self.gf_queue if q[0] in needed]
+        used = sum(self.gf_live.values())
+        reqs, keys, sizes = [], [], []
+        while self.gf_queue:
+            key, req, nbytes = self.gf_queue[0]
+            if (self.gf_live or reqs) and used + nbytes > self.gf_budget:
+                break
+            self.gf_queue.pop(0)
+            self.gf_queued.discard(key)
+            reqs.append(req)
+            keys.append(key)
+            sizes.append(nbytes)
+            used += nbytes
         if reqs:
-            for key, jid in zip(keys, self.b.submit_gf(reqs, self.p.nlanc)):
+            for key, jid, nbytes in zip(keys, self.b.submit_gf(reqs, self.p.nlanc), sizes):
                 self.gf[key] = jid
+                if jid >= 0:
+                    self.gf_live[jid] = nbytes
 
     def abort(self):
         """Stops and frees every process and sector of this solve (after a
@@ -293,6 +333,8 @@ class BatchedSolve:
         if self.phase == "eig":
             if self._collect():
                 self._submit_window()
+            elif self.gf_queue:
+                self._pump()
             if len(self.seen) == len(self.eig_jobs):
                 # final eigenpair order: energy, ties in sector order
                 pairs = _order_pairs([(e, idx, sec, j, k) for e, idx, sec, j, k in
@@ -311,10 +353,26 @@ class BatchedSolve:
                     if key not in needed and jid >= 0:
                         self.b.cancel(jid)
                         self.b.free_job(jid)
+                        self.gf_live.pop(jid, None)
+                # every needed run not started yet (possible only when the
+                # memory budget held it back) joins the queue
+                for e, _, sec, j, k in pairs:
+                    if not _passes(math.exp(-(e - self.e0) * p.beta), p.cutoff):
+                        continue
+                    for o, s_, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
+                        key = (j, k, o, s_, ann)
+                        if key not in self.gf and key not in self.gf_queued:
+                            nbytes = 3 * 8 * binom(self.Ns, nxt[0]) * binom(self.Ns, nxt[1]) + 4096
+                            self.gf_queue.append((key, (j, k, o + s_ * self.Ns, 1 if ann else 0, self.sid[nxt]),
+                                                  nbytes))
+                            self.gf_queued.add(key)
+                self.gf_needed = needed
                 self.phase = "gf"
         if self.phase == "gf":
-            live = [self.gf[key] for key, _ in self.used if self.gf[key] >= 0]
-            if not live or not np.any(self.b.job_states(live) == 0):
+            self._pump(self.gf_needed)
+            pending = bool(self.gf_queue) or any(key not in self.gf for key, _ in self.used)
+            live = [self.gf[key] for key, _ in self.used if key in self.gf and self.gf[key] >= 0]
+            if not pending and (not live or not np.any(self.b.job_states(live) == 0)):
                 wn = freq_points(p) if p.mesh != "matsubara" else matsubara(p.beta, p.nomega)
                 norb = len(p.orbitals)
                 Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
@@ -329,6 +387,13 @@ class BatchedSolve:
                                    [r[2] for r in rows], wn)
                 for q, (jid, (_, _, o, s, ann), _) in enumerate(rows):
                     (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
+                # spectral weight <c c+> + <c+ c> per orbital and spin
+                # (1 in every eigenstate: the anticommutator)
+                wsum = np.zeros((norb, 2))
+                for key, e in self.used:
+                    jid = self.gf[key]
+                    if jid >= 0:
+                        wsum[p.orbitals.index(key[2]), key[3]] += math.exp(-(e - self.e0) * p.beta) * n2[jid]
                 # H*v applications per sector (eigensolver runs, then
                 # Green's-function steps in their target sectors)
                 work = {}
@@ -357,17 +422,19 @@ class BatchedSolve:
                     self.b.remove_sector(sid)
                 self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
                                "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows),
-                               "observables": obs.result(), "hv_by_sector": work}
+                               "observables": obs.result(), "hv_by_sector": work, "weights": wsum / self.Z}
                 self.phase = "done"
         return self.phase == "done"
 
 
-def solve_batched(model, p, device=0, batch=None, autotune=True):
-    """Every sector's eigensolver run advanced together, then every start vector."""
+def solve_batched(model, p, device=0, batch=None, autotune=True, gf_budget=None):
+    """Every sector's eigensolver run advanced together, then every start
+    vector (gf_budget: bytes of Green's-function runs in flight, see
+    BatchedSolve)."""
     own = batch is None
     b = Batch(device) if own else batch
     t0 = time.perf_counter()
-    s = BatchedSolve(b, model, p)
+    s = BatchedSolve(b, model, p, gf_budget=gf_budget)
     s.start()
     if autotune:
         b.autotune(list(s.sid.values()))

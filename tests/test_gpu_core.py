@@ -52,8 +52,10 @@ def test_av_parity(ctx, name, kind, p):
         assert n == srs.n
         v = rng.uniform(-1, 1, n)
         w0 = rng.uniform(-1, 1, n)
-        assert _rel(ctx.av(v), srs.av(v)) < 1e-12
-        assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False)) < 1e-12
+        for var in (0, 1):  # every H*v variant, whatever the autotuner would pick
+            ctx.hv_variant = var
+            assert _rel(ctx.av(v), srs.av(v)) < 1e-12, var
+            assert _rel(ctx.av(v, w0, clear=False), srs.av(v, w0, clear=False)) < 1e-12, var
 
 
 @pytest.fixture(scope="module")
@@ -332,7 +334,7 @@ def test_lanczos_graph_replay_matches_eager():
 
 def test_av_parity_16site_sector():
     """The 16-site 4x4 Hubbard sector (8,8) (N = 165,636,900) against the
-    oracle's SpinResolvedStorage::av restatement, both H*v variants."""
+    oracle's SpinResolvedStorage::av restatement, every H*v variant."""
     Ns = 16
     t = models.torus_hopping(4, 4)
     om = eo.hubbard(Ns, t, np.full(Ns, 6.0), np.full(Ns, 3.0))
@@ -356,7 +358,7 @@ def test_av_parity_16site_sector():
 @pytest.mark.parametrize("nup", [1, 2, 3])
 def test_av_parity_16site_long_rows(ctx, nup):
     """Rows of 12,870 columns (four column chunks) with few rows: sectors
-    (nup, 8) of the 4x4 torus, both H*v variants against the oracle,
+    (nup, 8) of the 4x4 torus, every H*v variant against the oracle,
     accumulate and clear modes."""
     Ns = 16
     t = models.torus_hopping(4, 4)

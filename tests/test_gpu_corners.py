@@ -85,8 +85,6 @@ def test_12site_plain_solve_against_oracle():
     res = solve_plain(pm, sp)
     # oracle vectors written by tests/golden/make_plain12.py
     fx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plain12_solve.json")
-    if not os.path.exists(fx):
-        pytest.skip("oracle fixture plain12_solve.json not generated")
     with open(fx) as f:
         ref = json.load(f)
     ev = np.array(ref["energies"])
