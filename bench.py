@@ -34,18 +34,31 @@ sys.path.insert(0, ROOT)
 from edlib_amd.workloads import siam_model, torus  # noqa: E402
 
 METRIC = "H·v GB/s (% HBM peak); impurity solves/s + p99 latency, batched vs plain"
+
+
+def progress(msg):
+    """One line on stderr per finished part (stdout carries the JSON line only)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
-def load_traffic(variant):
+def load_traffic(variant, kernel_ms):
     """Memory traffic per launch of the given H*v variant, measured by the
-    rocprofv3 FETCH_SIZE/WRITE_SIZE counters and kept under profiles/, or None."""
+    rocprofv3 FETCH_SIZE/WRITE_SIZE counters and kept under profiles/ with
+    the kernel time of the profiled build; None when absent or when this
+    run's kernel time differs from that by more than 5 % (the kernel has
+    changed since the counters were taken)."""
     p = os.path.join(ROOT, "profiles", "hv_16site_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f).get("traffic_bytes_per_launch_by_variant", {}).get(str(variant))
-    return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    t = d.get("traffic_bytes_per_launch_by_variant", {}).get(str(variant))
+    ms = d.get("kernel_ms_by_variant", {}).get(str(variant))
+    if t is None or ms is None or abs(kernel_ms - ms) > 0.05 * ms:
+        return None
+    return t
 
 
 def cpu_model():
@@ -116,6 +129,7 @@ def cpu_socrs_configs():
         dt = float(np.median(ts))
         out.append({"workload": name, "dim": s.n, "av_ms": round(dt * 1e3, 3), "gbs": round(24 * s.n / dt / 1e9, 3),
                     "fill_s": round(tf, 2)})
+        progress("CPU SOCRS av, %s: %.1f ms" % (name, dt * 1e3))
         del s
     return out
 
@@ -139,6 +153,7 @@ def cpu_solve_estimate(hv_by_sector, nlanc):
         tot += (time.perf_counter() - t0) * cnt
         nav += cnt
         del s
+    progress("CPU H*v time of the AIM 1+11 solve's H*v count: %.1f s" % tot)
     return {"cpu_s_per_solve_lower_bound": round(tot, 3), "solves_per_s_upper_bound": round(1.0 / tot, 5),
             "hv_count": nav,
             "what": "sum over sectors of (H*v count of the device solve in that sector) x (one SOCRS av of that "
@@ -158,9 +173,11 @@ def solves_section(args):
     t = time.perf_counter()
     rb = solve_batched(m, sp)
     tb = time.perf_counter() - t
+    progress("AIM 1+11 batched solve %.2f s" % tb)
     t = time.perf_counter()
     rp = solve_plain(m, sp)
     tp = time.perf_counter() - t
+    progress("AIM 1+11 plain solve %.2f s" % tp)
     out["batched"] = {
         "workload": "AIM 1 orbital + 11 bath sites, U=4, all 169 (N_up, N_dn) sectors, NEV=4, NCV=11, tol 1e-14; "
                     "G of the impurity orbital from 4 start vectors per pair (spin x particle/hole), beta=10, "
@@ -180,6 +197,7 @@ def solves_section(args):
     reqs = [(r, siam_model(3 + b, 3, SEED + 1000 + r), sp4) for r, b in enumerate(baths)]
     srv = Server(0, sweeps_per_tick=8, max_active=args.max_active)
     done, wall = srv.run_stream(reqs, rate=args.rate, seed=SEED + 7)
+    progress("served stream of %d requests (bath sizes %s): %.2f s" % (len(reqs), baths, wall))
     blocked = dict(srv.blocked_by)
     budget, peak = srv.budget, srv.peak_reserved
     srv.close()
@@ -191,6 +209,7 @@ def solves_section(args):
         t = time.perf_counter()
         rp = solve_plain(m, p)
         t_plain.append(time.perf_counter() - t)
+        progress("plain solve of request %d (%d sites): %.2f s" % (r, m.Ns, t_plain[-1]))
         if "error" in done[r]:
             continue
         dg = max(dg, float(np.abs(done[r]["G"] - rp["G"]).max()))
@@ -317,6 +336,7 @@ def main():
     ctx.hv_variant = chosen
     achieved = B / (kern_ms * 1e-3) / 1e9
     value = B / wall / 1e9
+    progress("H*v %.3f ms per step (kernel %.3f ms)" % (wall * 1e3, kern_ms))
     # host vectors: staged over PCIe both ways (the reference's av() contract)
     import ctypes
     from edlib_amd import _lib
@@ -352,7 +372,7 @@ def main():
                    "algorithmic_bytes_per_step": B, "parallelism": "single GPU"},
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(chosen),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(chosen, kern_ms),
                      "kernel": {0: "edh::hv_rows_kernel (single pass)",
                                 1: "edh::hv_rows_kernel (no up term) + edh::hv_upslab_kernel (two passes)"}[chosen],
                      "kernel_ms": round(kern_ms, 4), "variants": per_variant},
@@ -365,12 +385,14 @@ def main():
     if not args.no_large:
         vb.upload(v)
         out["large_sector"] = large_sector_section(ea.HubbardModel(Ns, t, U, xmu), ctx, vb, args)
+        progress("16-site sector: Lanczos, eigensolver and Green's functions done")
     ctx.close()
     if not args.no_solves:
         out["solves"] = solves_section(args)
     if not args.no_cpu_baseline:
         wcpu = w.copy()
         out["cpu_baseline"] = cpu_baseline(t, U, xmu, v, wcpu, B)
+        progress("CPU av of the 16-site sector: %.1f ms" % out["cpu_baseline"]["ms_per_call"])
         out["cpu_baseline"]["per_config"] = cpu_socrs_configs()
         hv = out.get("solves", {}).get("batched", {}).pop("hv_by_sector", None)
         if hv:

@@ -5,10 +5,13 @@
 //     (diag, eigenvalues, G(i w_n))
 // Both also store the static observables (HDF5Utils.h:85-89).  INPUT_FILE
 // is the reference's input.h5; OUTPUT_FILE the HDF5 result file.
-//   edsolve --dump file.h5  prints every numeric dataset of an HDF5 file and
-//     the Statsum attribute of its results group as one JSON object
-//     {"path": {"shape": [...], "data": [...]}, ...} (for tests).
+//   edsolve --dump file.h5  prints every numeric dataset of an HDF5 file, its
+//     string datasets, the __complex__ attributes of datasets and the
+//     Statsum attribute of its results group as one JSON object
+//     {"path": {"shape": [...], "data": [...]} | {"string": "..."}, ...}
+//     (for tests).
 #include <cstdio>
+#include <map>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -30,8 +33,57 @@ static int dump(const std::string& path) {
     }
     H5Fclose(f);
   }
+  // string datasets (mesh kinds, version strings) and data/@__complex__
+  std::map<std::string, std::string> strs;
+  std::map<std::string, int> cplx;
+  {
+    const hid_t f2 = H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+    struct Ctx {
+      std::map<std::string, std::string>* strs;
+      std::map<std::string, int>* cplx;
+    } ctx{&strs, &cplx};
+    auto visit = [](hid_t obj, const char* name, const H5O_info_t* info, void* op) -> herr_t {
+      Ctx& c = *static_cast<Ctx*>(op);
+      if (info->type != H5O_TYPE_DATASET) return 0;
+      const hid_t d = H5Dopen2(obj, name, H5P_DEFAULT);
+      const hid_t t = H5Dget_type(d);
+      if (H5Tget_class(t) == H5T_STRING && H5Tis_variable_str(t) > 0) {
+        char* v = nullptr;
+        const hid_t mt = H5Tcopy(H5T_C_S1);
+        H5Tset_size(mt, H5T_VARIABLE);
+        if (H5Dread(d, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, &v) >= 0 && v) {
+          (*c.strs)[name] = v;
+          H5free_memory(v);
+        }
+        H5Tclose(mt);
+      }
+      if (H5Aexists(d, "__complex__") > 0) {
+        int x = 0;
+        const hid_t a = H5Aopen(d, "__complex__", H5P_DEFAULT);
+        H5Aread(a, H5T_NATIVE_INT, &x);
+        H5Aclose(a);
+        (*c.cplx)[name] = x;
+      }
+      H5Tclose(t);
+      H5Dclose(d);
+      return 0;
+    };
+    if (f2 >= 0) {
+      H5Ovisit(f2, H5_INDEX_NAME, H5_ITER_NATIVE, visit, &ctx);
+      H5Fclose(f2);
+    }
+  }
   std::printf("{");
   bool first = true;
+  for (const auto& kv : strs) {
+    std::printf("%s\n \"%s\": {\"string\": \"%s\"}", first ? "" : ",", kv.first.c_str(), kv.second.c_str());
+    first = false;
+  }
+  for (const auto& kv : cplx) {
+    std::printf("%s\n \"%s/@__complex__\": {\"shape\": [], \"data\": [%d]}", first ? "" : ",", kv.first.c_str(),
+                kv.second);
+    first = false;
+  }
   for (const auto& kv : sets) {
     std::printf("%s\n \"%s\": {\"shape\": [", first ? "" : ",", kv.first.c_str());
     for (size_t i = 0; i < kv.second.shape.size(); ++i) std::printf("%s%zu", i ? ", " : "", kv.second.shape[i]);
@@ -67,6 +119,37 @@ static void run(params& p, bool susceptibilities) {
 }
 
 int main(int argc, char** argv) {
+  if (argc >= 3 && std::string(argv[1]) == "--write-gf-sample") {
+    // a small Green's function (4 Matsubara points, 2 orbitals, 2 spins), a
+    // susceptibility (no spin index) and a real-axis function in the layout
+    // GreensFunction::save writes, without a device (for the layout tests)
+    try {
+      params p;
+      define_parameters(p);
+      p["lanc.BETA"] = 10.0;
+      p["lanc.NOMEGA"] = 4;
+      p["lanc.EMIN"] = -2.0;
+      p["lanc.EMAX"] = 2.0;
+      gf::MatsubaraMesh fm(p, gf::FERMIONIC), bm(p, gf::BOSONIC);
+      gf::RealFrequencyMesh rm(p);
+      gf::GFData g, chi;
+      g.init(4, 2, 2);
+      chi.init(4, 2, 1);
+      for (int w = 0; w < 4; ++w)
+        for (int o = 0; o < 2; ++o) {
+          for (int sp = 0; sp < 2; ++sp) g(w, o, sp) = std::complex<double>(w + 0.5 * o, -0.25 * sp - w);
+          chi(w, o) = std::complex<double>(o - w, 0.0);
+        }
+      archive ar(argv[2], "w");
+      ar.write_gf("results/G_omega", g, fm, gf::FERMIONIC);
+      ar.write_gf("results/ChiSz_omega", chi, bm, gf::BOSONIC);
+      ar.write_gf("results/G_real", g, rm, gf::FERMIONIC);
+      return 0;
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "error: %s\n", e.what());
+      return 3;
+    }
+  }
   if (argc >= 3 && std::string(argv[1]) == "--dump") {
     try {
       return dump(argv[2]);

@@ -295,7 +295,11 @@ def test_kanamori_stream_to_14_sites_served():
     waits for memory.  Each result is bit-identical alone and in the mix
     (the 9- and 11-site requests run alone again); every request's spectral
     weight <c c+> + <c+ c> is 1 for each orbital and spin, its Green's
-    function is spin-symmetric (the model is) and i w G(i w) -> 1."""
+    function is spin-symmetric (the model is) and i w G(i w) -> 1.  The
+    14-site ground energy equals the oracle's (ARPACK on the oracle's
+    H*v restatement in sector (8,9), kanamori14_e0.json) to 1e-10."""
+    import json
+    import os
     from edlib_amd.serve import Server, footprint_bytes
     from edlib_amd.solver import solve_batched
     baths = (6, 11, 8)
@@ -318,6 +322,11 @@ def test_kanamori_stream_to_14_sites_served():
         assert np.max(np.abs(w - 1.0)) < 1e-10, (r, w)
         assert np.max(np.abs(G[:, :, 0] - G[:, :, 1])) < 1e-10
         assert np.max(np.abs(-wn[-1] * G[-1, :, :].imag - 1.0)) < 2e-2
+    fx = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kanamori14_e0.json")
+    with open(fx) as f:
+        ref = json.load(f)
+    e0 = float(np.min(done[1]["energies"]))
+    assert abs(e0 - ref["E0"]) < 1e-10 * abs(ref["E0"]), (e0, ref["E0"])
     for r in (0, 2):
         alone = solve_batched(models[r], sp)
         assert np.array_equal(alone["energies"], done[r]["energies"])

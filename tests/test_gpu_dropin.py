@@ -56,6 +56,9 @@ def test_hdf5_driver_4ring(tmp_path):
     assert res["results/@Statsum"]["data"][0] >= 1.0
     assert res["results/G_omega/data"]["shape"] == [64, 4, 2, 2]
     assert res["results/G_omega/mesh/1/N"]["data"] == [64]
+    assert res["results/G_omega/mesh/1/kind"]["string"] == "MATSUBARA"
+    assert res["results/G_omega/mesh/2/kind"]["string"] == "INDEX"
+    assert res["results/G_omega/data/@__complex__"]["data"] == [1]
     assert res["results/ChiSz_omega/data"]["shape"] == [64, 4, 2]
     assert res["results/ChiN_omega/data"]["shape"] == [64, 4, 2]
     n = res["results/static_observables/N"]["data"]
