@@ -106,6 +106,18 @@ class HubbardModel(_Model):
         t_, U_, x_, H_, J_ = self._keep
         check(_lib.lib().edh_model_hubbard(ctypes.byref(self._h), Ns, _d(t_), _d(U_), _d(J_), _d(x_), _d(H_)))
 
+    def bare_gf(self, z):
+        """HubbardModel::bare_greens_function (HubbardModel.h:181-201): per
+        frequency point and spin, G0 = (z + mu_I delta_IJ + t_IJ)^-1 (the
+        model's on-site energies are zero); returns (len(z), Ns*Ns, 2) with
+        entry I*Ns + J."""
+        z = np.atleast_1d(np.asarray(z, dtype=complex))
+        t_, x_ = self._keep[0], self._keep[2]
+        Ns = t_.shape[0]
+        m = t_[None, :, :] + np.einsum("w,ij->wij", z, np.eye(Ns)) + np.diag(x_)[None, :, :]
+        g = np.linalg.inv(m).reshape(len(z), Ns * Ns)
+        return np.repeat(g[:, :, None], 2, axis=2)
+
 
 class SingleImpurityAndersonModel(_Model):
     """SingleImpurityAndersonModel<double> (SingleImpurityAndersonModel.h:195-275)."""
@@ -119,6 +131,24 @@ class SingleImpurityAndersonModel(_Model):
         e, v, h, u = self._keep
         ep = _d(e) if nb else None
         check(_lib.lib().edh_model_siam(ctypes.byref(self._h), Ns, ml, ep, _d(v), _d(h), _d(u), float(mu)))
+
+    def bare_gf(self, z):
+        """SingleImpurityAndersonModel::bare_greens_function
+        (SingleImpurityAndersonModel.h:378-391): per impurity orbital and
+        spin, G0 = 1 / (z - H0_ii - sum_k V_ik^2 / (z - eps_k)) (the reference
+        leaves mu out, so at U = 0 the self-energy is -mu); returned in the
+        non-local layout (len(z), ml*ml, 2), diagonal entries i*ml + i set."""
+        z = np.atleast_1d(np.asarray(z, dtype=complex))
+        e, v, h = self._keep[0], self._keep[1], self._keep[2]
+        ml = h.shape[0]
+        g = np.zeros((len(z), ml * ml, 2), dtype=complex)
+        for s in range(2):
+            for i in range(ml):
+                delta = np.zeros(len(z), dtype=complex)
+                for k in range(e.shape[0]):
+                    delta += v[i, k, s] * v[i, k, s] / (z - e[k, s])
+                g[:, i * ml + i, s] = 1.0 / (z - h[i, i, s] - delta)
+        return g
 
 
 class DeviceBuffer:

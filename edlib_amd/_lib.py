@@ -86,6 +86,7 @@ _SIGS = {
     "edh_batch_static_observables_many": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int, _dp]),
     "edh_batch_add_sectors": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_int64)]),
     "edh_batch_submit_gf": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, ctypes.c_int, _ip]),
+    "edh_batch_submit_gf2": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _ip, _ip, _ip, _ip, ctypes.c_int, _ip]),
     "edh_batch_lanczos_norm2": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp]),
     "edh_batch_step": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "edh_batch_job_info": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, _i64p, _ip, _ip]),

@@ -73,17 +73,24 @@ class Batch:
 
     def submit_gf(self, requests, nl):
         """Green's-function start vectors c(+)_orb |psi_index>, one per request
-        (eig_job, index, orb, annihilate, dst_sid), and their Lanczos runs.
-        Returns the job ids (-1 where the target sector is empty); nothing
-        waits for the device.  A run whose start vector vanishes (squared
-        norm <= 1e-10) stops at once; the squared norms are known once the
-        runs have finished (lanczos_norm2)."""
+        (eig_job, index, orb, annihilate, dst_sid[, orb2]), and their Lanczos
+        runs; with orb2 >= 0 the start vector is (c(+)_orb + c(+)_orb2)|psi>
+        (a non-local pair).  Returns the job ids (-1 where the target sector
+        is empty); nothing waits for the device.  A run whose start vector
+        vanishes (squared norm <= 1e-10) stops at once; the squared norms are
+        known once the runs have finished (lanczos_norm2)."""
         k = len(requests)
         cols = [np.ascontiguousarray([int(r[c]) for r in requests], dtype=np.int32) for c in range(5)]
         ids = np.zeros(max(k, 1), dtype=np.int32)
         ip = [c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) for c in cols]
-        check(_lib.lib().edh_batch_submit_gf(self._h, k, ip[0], ip[1], ip[2], ip[3], ip[4], nl,
-                                             ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        out = ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        if any(len(r) > 5 and r[5] >= 0 for r in requests):
+            o2 = np.ascontiguousarray([int(r[5]) if len(r) > 5 else -1 for r in requests], dtype=np.int32)
+            check(_lib.lib().edh_batch_submit_gf2(self._h, k, ip[0], ip[1], ip[2],
+                                                  o2.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ip[3], ip[4], nl,
+                                                  out))
+        else:
+            check(_lib.lib().edh_batch_submit_gf(self._h, k, ip[0], ip[1], ip[2], ip[3], ip[4], nl, out))
         return ids[:k].tolist()
 
     def lanczos_norm2(self, jids):

@@ -871,6 +871,24 @@ int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int*
   API_END
 }
 
+int edh_batch_submit_gf2(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
+                         const int* orbs2, const int* annihilate, const int* dst_secs, int nl, int* job_ids) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(count >= 0, EDH_ERR_ARG, "negative count");
+  EDH_CHECK(count == 0 || (eig_jobs && indices && orbs && orbs2 && annihilate && dst_secs && job_ids), EDH_ERR_ARG,
+            "null argument");
+  std::vector<edh::GfRequest> req(count);
+  for (int r = 0; r < count; ++r) {
+    req[r] = {eig_jobs[r], indices[r], orbs[r], annihilate[r] ? 1 : 0, dst_secs[r]};
+    req[r].orb2 = orbs2[r] < 0 ? -1 : orbs2[r];
+  }
+  std::vector<int> ids;
+  B.submit_gf(req, nl, ids);
+  for (int r = 0; r < count; ++r) job_ids[r] = ids[r];
+  API_END
+}
+
 int edh_batch_lanczos_norm2(edh_batch* b, int count, const int* job_ids, double* norm2) {
   API_BEGIN
   edh::Batch& B = use_batch(b);

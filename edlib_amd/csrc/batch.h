@@ -116,6 +116,7 @@ struct ReadSlab {
 
 struct GfRequest {
   int eig_job, index, orb, annihilate, dst_sec;
+  int orb2 = -1;  // >= 0: start vector (c(+)_orb + c(+)_orb2)|psi>
 };
 
 struct Batch {

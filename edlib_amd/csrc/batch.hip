@@ -1226,7 +1226,8 @@ void Batch::gather_maps(int src, int orb, bool annihilate, std::vector<void*>& o
 }
 
 // Green's-function start vectors c(+)_orb |psi> (GreensFunction.h:395-416,
-// 437-458, one orbital each) gathered on the device, and a Lanczos run from
+// 437-458; one orbital, or the sum over the two orbitals of a non-local
+// pair) gathered on the device, and a Lanczos run from
 // each.  Nothing waits for the device: one grouped reduction gives every
 // start vector's norm, which becomes its run's first scale (the stored
 // vector stays unnormalised) or, at a squared norm <= 1e-10, stops the run
@@ -1247,6 +1248,8 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
     const int Ns = a.model->Ns;
     EDH_CHECK(q.orb >= 0 && q.orb < 2 * Ns, EDH_ERR_ARG, "batch: orbital out of range");
     const int spin = q.orb >= Ns ? 1 : 0;
+    EDH_CHECK(q.orb2 < 0 || (q.orb2 < 2 * Ns && (q.orb2 >= Ns ? 1 : 0) == spin && q.orb2 != q.orb), EDH_ERR_ARG,
+              "batch: second orbital out of range, of the other spin or equal to the first");
     const int d = q.annihilate ? -1 : 1;
     EDH_CHECK(b.tabs.nup == a.tabs.nup + (spin == 0 ? d : 0) && b.tabs.ndn == a.tabs.ndn + (spin == 1 ? d : 0),
               EDH_ERR_ARG, "batch: target sector does not match the operator");
@@ -1257,6 +1260,15 @@ void Batch::submit_gf(const std::vector<GfRequest>& req, int nl, std::vector<int
     BJob& j = job(id);
     dev_separable_gather(e.evecs + (size_t)q.index * e.n, a.tabs.dn.D, (const int32_t*)mp[0], (const int32_t*)mp[1],
                          (const int8_t*)mp[2], (const int8_t*)mp[3], j.lz[0], b.tabs.up.D, b.tabs.dn.D, stream);
+    if (q.orb2 >= 0) {
+      // the second orbital's vector added to the first (non-local start
+      // vectors, GreensFunction.h:290-327)
+      std::vector<void*> mp2;
+      gather_maps(e.sec, q.orb2, q.annihilate != 0, mp2);
+      dev_separable_gather(e.evecs + (size_t)q.index * e.n, a.tabs.dn.D, (const int32_t*)mp2[0],
+                           (const int32_t*)mp2[1], (const int8_t*)mp2[2], (const int8_t*)mp2[3], j.lz[0], b.tabs.up.D,
+                           b.tabs.dn.D, stream, true);
+    }
     BOpHost o;
     o.w = j.lz[0];
     o.n = j.n;

@@ -34,7 +34,8 @@ void dev_lanczos_passB(double* w, const double* x, const double* coef, int64_t n
 
 // Green's-function helpers (gf.hip)
 void dev_separable_gather(const double* in, int64_t ldin, const int32_t* mu, const int32_t* md, const int8_t* su,
-                          const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s);
+                          const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s,
+                          bool accumulate = false);
 void dev_separable_scale(const double* in, const double* fu, const double* fd, double* out, int64_t Dup, int64_t Ddn,
                          hipStream_t s);
 // One continued fraction of a batch (device pointers into a packed array).

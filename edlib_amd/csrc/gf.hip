@@ -13,15 +13,18 @@
 
 namespace edh {
 
+// out = c(+)|in> (accumulate: out = c(+)|in> + out, the order in which the
+// reference sums the vectors of several orbitals, GreensFunction.h:434-438)
 __global__ void separable_gather_kernel(const double* in, int64_t ldin, const int32_t* mu, const int32_t* md,
-                                        const int8_t* su, const int8_t* sd, double* out, int64_t Dup, int64_t Ddn) {
+                                        const int8_t* su, const int8_t* sd, double* out, int64_t Dup, int64_t Ddn,
+                                        int accumulate) {
   const int64_t n = Dup * Ddn;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t u = e / Ddn, d = e - u * Ddn;
     const int32_t a = mu[u], b = md[d];
     double x = 0.0;
     if (a >= 0 && b >= 0) x = (double)(su[u] * sd[d]) * in[(int64_t)a * ldin + b];
-    out[e] = x;
+    out[e] = accumulate ? x + out[e] : x;
   }
 }
 
@@ -135,10 +138,10 @@ void dev_cf_batch(int njobs, int nz, const double* wr, const double* wn, const C
 }
 
 void dev_separable_gather(const double* in, int64_t ldin, const int32_t* mu, const int32_t* md, const int8_t* su,
-                          const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s) {
+                          const int8_t* sd, double* out, int64_t Dup, int64_t Ddn, hipStream_t s, bool accumulate) {
   if (Dup * Ddn == 0) return;
   hipLaunchKernelGGL(separable_gather_kernel, dim3(reduce_grid(Dup * Ddn)), dim3(kVecBlock), 0, s, in, ldin, mu, md,
-                     su, sd, out, Dup, Ddn);
+                     su, sd, out, Dup, Ddn, accumulate ? 1 : 0);
   EDH_HIP(hipGetLastError());
 }
 

@@ -44,7 +44,7 @@ def footprint_bytes(Ns, p, nterms=None):
         n = binom(Ns, s[0]) * binom(Ns, s[1])
         tot += n * 8 * (2 * (m + 1) + p.nev + 1) + table_bytes_estimate(Ns, s[0], s[1], nterms)
         big = max(big, n)
-    ngf = 4 * len(p.orbitals) * p.nev * 4
+    ngf = 4 * (len(p.gf_orbitals) + len(p.pairs)) * p.nev * 4
     return tot + 3 * 8 * big * ngf
 
 

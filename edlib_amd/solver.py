@@ -14,7 +14,15 @@ local Green's function of the listed orbitals:
   with isign -1; each start vector is normalised, its squared norm is the
   weight, and a Lanczos run of NLANC steps feeds the continued fraction at
   z = i w_n + isign * E (Lanczos.h:82-100, 146-148, 201-224);
-* G = (G_g + G_l) / Z on the fermionic Matsubara mesh w_n = (2n+1) pi / beta.
+* G = (G_g + G_l) / Z on the fermionic Matsubara mesh w_n = (2n+1) pi / beta
+  (or at w + i pi/beta on the real axis);
+* for each listed orbital pair (i, j), i != j (GreensFunction.h:65-102,
+  290-327, 336-359): start vectors (c+_i + c+_j)|psi> and (c_i + c_j)|psi>,
+  whose fractions give G_ij = ((G_{i+j} - G_ii) - G_jj) / 2 per part, the
+  diagonal entries copied from G, both parts summed; the members of every
+  pair join the orbital list.  With `selfenergy`, the model's bare G0 and
+  Sigma = G0^-1 - G_ij^-1 per frequency and spin (GreensFunction.h:202-223,
+  FermionicModel.h:84-108) over the interacting orbitals.
 
 ``solve_plain`` runs one sector, then one start vector, at a time on a
 Context -- what the drop-in storage does inside Hamiltonian and
@@ -52,6 +60,19 @@ class SolveParams:
     mesh: str = "matsubara"  # or "real": nomega points on [emin, emax], z = w + i pi/beta
     emin: float = -4.0
     emax: float = 4.0
+    pairs: tuple = ()        # orbital pairs (i, j), i != j, of the non-local G_ij
+    selfenergy: bool = False  # G0 and Sigma from G_ij (every entry of G_ij listed)
+
+    @property
+    def gf_orbitals(self):
+        """The orbitals of the local G: the listed ones, then the members of
+        the pairs that are not listed (GreensFunction.h:84-100)."""
+        out = list(self.orbitals)
+        for pr in self.pairs:
+            for o in pr:
+                if o not in out:
+                    out.append(o)
+        return out
 
     @property
     def ncv_eff(self):
@@ -122,10 +143,12 @@ def _order_pairs(pairs):
     return sorted(pairs, key=lambda q: q[0])
 
 
-def _gf_targets(Ns, sector, orbitals):
-    """(orbital, spin, annihilate, next sector) of every start vector of one pair."""
+def _gf_targets(Ns, sector, p):
+    """(orbitals, spin, annihilate, next sector) of every start vector of one
+    eigenpair: one orbital (o,) per entry of the local G, then the two of
+    each non-local pair (i, j)."""
     out = []
-    for o in orbitals:
+    for ops in [(o,) for o in p.gf_orbitals] + [tuple(int(x) for x in pr) for pr in p.pairs]:
         for s in (0, 1):
             for ann in (False, True):
                 n = sector[s]
@@ -133,13 +156,55 @@ def _gf_targets(Ns, sector, orbitals):
                     continue
                 d = -1 if ann else 1
                 nxt = (sector[0] + d, sector[1]) if s == 0 else (sector[0], sector[1] + d)
-                out.append((o, s, ann, nxt))
+                out.append((ops, s, ann, nxt))
+    return out
+
+
+def _check_pairs(model, p):
+    ml = _interacting(model)
+    for pr in p.pairs:
+        if len(pr) != 2 or pr[0] == pr[1] or not all(0 <= o < ml for o in pr):
+            raise ValueError("orbital pairs are two different interacting orbitals: %r" % (pr,))
+    if p.selfenergy and sorted(set(p.gf_orbitals)) != list(range(ml)):
+        raise ValueError("the self-energy needs the local G of every interacting orbital (0..%d)" % (ml - 1))
+
+
+def _finish_gf(model, p, zpts, Gg, Gl, Ggij, Glij, Z):
+    """Normalisation, non_local_gf (GreensFunction.h:336-359) and, on
+    request, the self-energy; returns the result entries."""
+    Gg, Gl = Gg / Z, Gl / Z
+    out = {"G": Gg + Gl}
+    if not p.pairs and not p.selfenergy:
+        return out
+    ml = _interacting(model)
+    orbs = p.gf_orbitals
+    Ggij, Glij = Ggij / Z, Glij / Z
+    for i, j in p.pairs:
+        k = ml * i + j
+        for o in (i, j):
+            Ggij[:, k, :] -= Gg[:, orbs.index(o), :]
+            Glij[:, k, :] -= Gl[:, orbs.index(o), :]
+        Ggij[:, k, :] *= 0.5
+        Glij[:, k, :] *= 0.5
+    for q, o in enumerate(orbs):
+        Ggij[:, ml * o + o, :] = Gg[:, q, :]
+        Glij[:, ml * o + o, :] = Gl[:, q, :]
+    Gij = Ggij + Glij
+    out["G_ij"] = Gij
+    if p.selfenergy:
+        G0 = model.bare_gf(zpts)
+        a = np.moveaxis(G0.reshape(len(zpts), ml, ml, 2), 3, 1)
+        b = np.moveaxis(Gij.reshape(len(zpts), ml, ml, 2), 3, 1)
+        sig = np.linalg.inv(a) - np.linalg.inv(b)
+        out["G0"] = G0
+        out["Sigma"] = np.moveaxis(sig, 1, 3).reshape(len(zpts), ml * ml, 2)
     return out
 
 
 def solve_plain(model, p, device=0, ctx=None):
     """One sector and one start vector at a time (the drop-in path)."""
     Ns = model.Ns
+    _check_pairs(model, p)
     own = ctx is None
     ctx = Context(device) if own else ctx
     t0 = time.perf_counter()
@@ -160,10 +225,13 @@ def solve_plain(model, p, device=0, ctx=None):
     e0 = pairs[0][0]
     Z = sum(math.exp(-(e - e0) * p.beta) for e, _, _ in pairs)
     zpts = freq_points(p)
-    norb = len(p.orbitals)
-    Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
+    orbs = p.gf_orbitals
+    ml = _interacting(model)
+    Gg = np.zeros((p.nomega, len(orbs), 2), dtype=complex)
     Gl = np.zeros_like(Gg)
-    wsum = np.zeros((norb, 2))
+    Ggij = np.zeros((p.nomega, ml * ml, 2), dtype=complex)
+    Glij = np.zeros_like(Ggij)
+    wsum = np.zeros((len(orbs), 2))
     n_gf = 0
     obs = _Observables()
     for e, sec, vec in pairs:
@@ -171,11 +239,15 @@ def solve_plain(model, p, device=0, ctx=None):
         if not _passes(bf, p.cutoff):
             continue
         ctx.set_sector(*sec)
-        obs.add(ctx.static_observables(vec, _interacting(model)), e, bf)
-        for q, (o, s, ann, nxt) in enumerate(_gf_targets(Ns, sec, p.orbitals)):
-            _, out = ctx.adag(sec, o + s * Ns, ann, vec)
+        obs.add(ctx.static_observables(vec, ml), e, bf)
+        for ops, s, ann, nxt in _gf_targets(Ns, sec, p):
+            out = None
+            for o in ops:  # the pair's vectors summed in orbital order
+                _, x = ctx.adag(sec, o + s * Ns, ann, vec)
+                out = x if out is None else x + out
             norm = ctx.vv(out, out)
-            wsum[p.orbitals.index(o), s] += bf * norm
+            if len(ops) == 1:
+                wsum[orbs.index(ops[0]), s] += bf * norm
             if not abs(norm) > 1e-10:
                 continue
             out /= math.sqrt(norm)
@@ -183,14 +255,18 @@ def solve_plain(model, p, device=0, ctx=None):
             nlanc, a, b = ctx.lanczos(out, p.nlanc)
             isign = -1 if ann else 1
             g = ctx.cf_eval(zpts + e * isign, norm, nlanc, isign, _expb(p.beta, e, e0), a, b)
-            (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g
+            if len(ops) == 1:
+                (Gl if ann else Gg)[:, orbs.index(ops[0]), s] += g
+            else:
+                (Glij if ann else Ggij)[:, ml * ops[0] + ops[1], s] += g
             n_gf += 1
-    G = (Gg + Gl) / Z
+    res = _finish_gf(model, p, zpts, Gg, Gl, Ggij, Glij, Z)
     t_all = time.perf_counter() - t0
     if own:
         ctx.close()
-    return {"E0": e0, "Z": Z, "G": G, "energies": np.array([q[0] for q in pairs]), "n_gf": n_gf, "n_av": n_av,
-            "observables": obs.result(), "t_eig": t_eig, "t_total": t_all, "weights": wsum / Z}
+    res.update({"E0": e0, "Z": Z, "energies": np.array([q[0] for q in pairs]), "n_gf": n_gf, "n_av": n_av,
+                "observables": obs.result(), "t_eig": t_eig, "t_total": t_all, "weights": wsum / Z})
+    return res
 
 
 class BatchedSolve:
@@ -225,18 +301,28 @@ class BatchedSolve:
         self.eig_jobs = []
         self.known = []      # (E, sector index, sector, eig job, k) of finished sectors
         self.seen = set()    # finished eig jobs
-        self.gf = {}         # (eig job, k, orbital, spin, annihilate) -> gf job or -1
+        self.gf = {}         # (eig job, k, orbitals, spin, annihilate) -> gf job or -1
         self.gf_queue = []   # (key, request, bytes) waiting for memory
         self.gf_queued = set()
         self.gf_live = {}    # running gf job -> bytes
         big = max(binom(self.Ns, a) * binom(self.Ns, b) for a, b in self.secs) if self.secs else 0
-        self.gf_budget = gf_budget if gf_budget is not None else 3 * 8 * big * (4 * len(p.orbitals) * p.nev * 4)
+        nops = len(p.gf_orbitals) + len(p.pairs)
+        self.gf_budget = gf_budget if gf_budget is not None else 3 * 8 * big * (4 * nops * p.nev * 4)
         self.phase = "new"
         self.result = None
         c = p.cutoff
         self.window = math.inf if c <= 0 else (-math.log(c) / p.beta) * (1 + 1e-9) + 1e-9
 
+    def _request(self, key, nxt):
+        j, k, ops, s, ann = key
+        req = (j, k, ops[0] + s * self.Ns, 1 if ann else 0, self.sid[nxt])
+        if len(ops) > 1:
+            req += (ops[1] + s * self.Ns,)
+        nbytes = 3 * 8 * binom(self.Ns, nxt[0]) * binom(self.Ns, nxt[1]) + 4096
+        return (key, req, nbytes)
+
     def start(self):
+        _check_pairs(self.model, self.p)
         ids, _ = self.b.add_sectors(self.model, self.secs)
         self.sid = dict(zip(self.secs, ids))
         for i, sec in enumerate(sector_list(self.Ns, self.p)):
@@ -272,11 +358,10 @@ class BatchedSolve:
         for e, _, sec, j, k in sorted(self.known, key=lambda q: q[0]):
             if e - e0 > self.window:
                 continue
-            for o, s, ann, nxt in _gf_targets(self.Ns, sec, self.p.orbitals):
-                key = (j, k, o, s, ann)
+            for ops, s, ann, nxt in _gf_targets(self.Ns, sec, self.p):
+                key = (j, k, ops, s, ann)
                 if key not in self.gf and key not in self.gf_queued:
-                    nbytes = 3 * 8 * binom(self.Ns, nxt[0]) * binom(self.Ns, nxt[1]) + 4096
-                    self.gf_queue.append((key, (j, k, o + s * self.Ns, 1 if ann else 0, self.sid[nxt]), nbytes))
+                    self.gf_queue.append(self._request(key, nxt))
                     self.gf_queued.add(key)
         self._pump()
 
@@ -346,8 +431,8 @@ class BatchedSolve:
                 for e, _, sec, j, k in pairs:
                     if not _passes(math.exp(-(e - self.e0) * p.beta), p.cutoff):
                         continue
-                    for o, s, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
-                        self.used.append(((j, k, o, s, ann), e))
+                    for ops, s, ann, nxt in _gf_targets(self.Ns, sec, p):
+                        self.used.append(((j, k, ops, s, ann), e))
                 needed = {key for key, _ in self.used}
                 for key, jid in self.gf.items():
                     if key not in needed and jid >= 0:
@@ -359,12 +444,10 @@ class BatchedSolve:
                 for e, _, sec, j, k in pairs:
                     if not _passes(math.exp(-(e - self.e0) * p.beta), p.cutoff):
                         continue
-                    for o, s_, ann, nxt in _gf_targets(self.Ns, sec, p.orbitals):
-                        key = (j, k, o, s_, ann)
+                    for ops, s_, ann, nxt in _gf_targets(self.Ns, sec, p):
+                        key = (j, k, ops, s_, ann)
                         if key not in self.gf and key not in self.gf_queued:
-                            nbytes = 3 * 8 * binom(self.Ns, nxt[0]) * binom(self.Ns, nxt[1]) + 4096
-                            self.gf_queue.append((key, (j, k, o + s_ * self.Ns, 1 if ann else 0, self.sid[nxt]),
-                                                  nbytes))
+                            self.gf_queue.append(self._request(key, nxt))
                             self.gf_queued.add(key)
                 self.gf_needed = needed
                 self.phase = "gf"
@@ -374,9 +457,12 @@ class BatchedSolve:
             live = [self.gf[key] for key, _ in self.used if key in self.gf and self.gf[key] >= 0]
             if not pending and (not live or not np.any(self.b.job_states(live) == 0)):
                 wn = freq_points(p) if p.mesh != "matsubara" else matsubara(p.beta, p.nomega)
-                norb = len(p.orbitals)
-                Gg = np.zeros((p.nomega, norb, 2), dtype=complex)
+                orbs = p.gf_orbitals
+                ml = _interacting(self.model)
+                Gg = np.zeros((p.nomega, len(orbs), 2), dtype=complex)
                 Gl = np.zeros_like(Gg)
+                Ggij = np.zeros((p.nomega, ml * ml, 2), dtype=complex)
+                Glij = np.zeros_like(Ggij)
                 n2 = dict(zip(live, self.b.lanczos_norm2(live))) if live else {}
                 # a vanishing start vector (<out|out> <= 1e-10) has no run
                 # (GreensFunction.h:408-418)
@@ -385,15 +471,18 @@ class BatchedSolve:
                 g = self.b.cf_many([r[0] for r in rows], [n2[r[0]] for r in rows],
                                    [-1 if r[1][4] else 1 for r in rows], [_expb(p.beta, r[2], self.e0) for r in rows],
                                    [r[2] for r in rows], wn)
-                for q, (jid, (_, _, o, s, ann), _) in enumerate(rows):
-                    (Gl if ann else Gg)[:, p.orbitals.index(o), s] += g[q]
+                for q, (jid, (_, _, ops, s, ann), _) in enumerate(rows):
+                    if len(ops) == 1:
+                        (Gl if ann else Gg)[:, orbs.index(ops[0]), s] += g[q]
+                    else:
+                        (Glij if ann else Ggij)[:, ml * ops[0] + ops[1], s] += g[q]
                 # spectral weight <c c+> + <c+ c> per orbital and spin
                 # (1 in every eigenstate: the anticommutator)
-                wsum = np.zeros((norb, 2))
+                wsum = np.zeros((len(orbs), 2))
                 for key, e in self.used:
                     jid = self.gf[key]
-                    if jid >= 0:
-                        wsum[p.orbitals.index(key[2]), key[3]] += math.exp(-(e - self.e0) * p.beta) * n2[jid]
+                    if jid >= 0 and len(key[2]) == 1:
+                        wsum[orbs.index(key[2][0]), key[3]] += math.exp(-(e - self.e0) * p.beta) * n2[jid]
                 # H*v applications per sector (eigensolver runs, then
                 # Green's-function steps in their target sectors)
                 work = {}
@@ -420,9 +509,11 @@ class BatchedSolve:
                     self.b.free_job(j)
                 for sid in self.sid.values():
                     self.b.remove_sector(sid)
-                self.result = {"E0": self.e0, "Z": self.Z, "G": (Gg + Gl) / self.Z,
-                               "energies": np.array([q[0] for q in self.pairs]), "n_gf": len(rows),
-                               "observables": obs.result(), "hv_by_sector": work, "weights": wsum / self.Z}
+                zpts = freq_points(p)
+                self.result = _finish_gf(self.model, p, zpts, Gg, Gl, Ggij, Glij, self.Z)
+                self.result.update({"E0": self.e0, "Z": self.Z, "energies": np.array([q[0] for q in self.pairs]),
+                                    "n_gf": len(rows), "observables": obs.result(), "hv_by_sector": work,
+                                    "weights": wsum / self.Z})
                 self.phase = "done"
         return self.phase == "done"
 

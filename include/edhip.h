@@ -278,6 +278,14 @@ int edh_batch_submit_lanczos(edh_batch* b, int sec_id, const double* v0, int nl,
  * (edh_batch_lanczos_norm2). */
 int edh_batch_submit_gf(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
                         const int* annihilate, const int* dst_secs, int nl, int* job_ids);
+/* edh_batch_submit_gf with a second orbital per request: orbs2[q] >= 0 (same
+ * spin as orbs[q], a different orbital) gives the start vector
+ * (c(+)_orbs[q] + c(+)_orbs2[q])|psi> of the non-local Green's function
+ * (GreensFunction::nonlocal_contribution, GreensFunction.h:290-327, with
+ * create_particles / annihilate_particles over two orbitals, :389-462);
+ * orbs2[q] < 0 is the one-orbital request. */
+int edh_batch_submit_gf2(edh_batch* b, int count, const int* eig_jobs, const int* indices, const int* orbs,
+                         const int* orbs2, const int* annihilate, const int* dst_secs, int nl, int* job_ids);
 /* Squared start-vector norms <out|out> of finished Green's-function runs
  * (NaN while a run has not finished; 1 for runs from
  * edh_batch_submit_lanczos, whose start vector is normalised). */

@@ -10,7 +10,9 @@
   512 Matsubara points at beta=20 from 150 Lanczos steps per start vector
   (from the two lowest pairs, see the test),
   against the oracle (scipy's ARPACK for the eigenpairs, the oracle's
-  GreensFunction restatement for G), whose vectors are committed fixtures.
+  GreensFunction restatement for G), whose vectors are committed fixtures;
+* the 14-site Kanamori sector (7,7) (11,778,624 states): H*v and a short
+  Lanczos run against the oracle.
 """
 import ctypes
 import math
@@ -92,3 +94,37 @@ def test_12site_plain_solve_against_oracle():
     for s in range(2):
         G = np.array(ref["G_re"][s]) + 1j * np.array(ref["G_im"][s])
         assert np.max(np.abs(res["G"][:, 0, s] - G)) < 1e-8
+
+
+def test_kanamori_14site_sector_against_oracle():
+    """The largest sector of the three-orbital Kanamori stream at its stated
+    size: 3 orbitals + 11 bath sites (the served stream's 14-site request,
+    same seed), sector (7,7) of 11,778,624 states.  H*v in accumulate and
+    clear modes against the oracle's SpinResolvedStorage::av restatement
+    (1e-13 relative to the largest entry) and the first 10 Lanczos
+    coefficients against the oracle's Lanczos::lanczos (1e-10)."""
+    p = models.siam_params(14, 3, np.random.default_rng(20261202 + 2011))
+    om, pm = models.oracle_model("siam", p), models.product_model("siam", p)
+    srs = eo.SRS(om, 7, 7)
+    assert srs.n == 3432 ** 2
+    rng = np.random.default_rng(20261202 + 14)
+    v = rng.uniform(-1, 1, srs.n)
+    w0 = rng.uniform(-1, 1, srs.n)
+    ref_acc = srs.av(v, w0.copy(), clear=False, omp=True)
+    ref = srs.av(v, omp=True)
+    ctx = ea.Context(0)
+    try:
+        ctx.set_model(pm)
+        assert ctx.set_sector(7, 7) == srs.n
+        for var in (0, 1):
+            ctx.hv_variant = var
+            assert _rel(ctx.av(v, w0, clear=False), ref_acc) < 1e-13, var
+            assert _rel(ctx.av(v), ref) < 1e-13, var
+        v /= np.linalg.norm(v)
+        nl_o, a_o, b_o = srs.lanczos(v, 10)
+        nl_g, a_g, b_g = ctx.lanczos(v, 10)
+        assert nl_o == nl_g == 10
+        assert np.max(np.abs(a_o - a_g)) < 1e-10
+        assert np.max(np.abs(b_o[1:10] - b_g[1:10])) < 1e-10
+    finally:
+        ctx.close()
