@@ -176,7 +176,6 @@ struct DevSpin {
   uint64_t* ell_q4 = nullptr;
   uint32_t* ell_s4 = nullptr;
   double s4_scale = 0.0;
-  int32_t* rank = nullptr;
 };
 
 struct DevSector {
@@ -184,6 +183,7 @@ struct DevSector {
   int64_t N = 0;
   DevSpin up, dn;
   double* values = nullptr;
+  int nvalues = 0;  // entries of values
   int imp_ml = 0;
   int cross_mode = 0;
   double cross_u = 0.0;
@@ -191,8 +191,7 @@ struct DevSector {
   double* cross_rowtab = nullptr;
   int kan_ml = 0;
   int32_t* kan_ptr = nullptr;
-  int32_t* kan_a = nullptr;
-  int32_t* kan_b = nullptr;
+  int64_t* kan_off = nullptr;  // element offset of each term's partner state
   double* kan_c = nullptr;
   int32_t* row_order = nullptr;  // processing order of up rows
   int variant = HV_SINGLE;       // chosen H*v variant

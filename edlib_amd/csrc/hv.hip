@@ -42,7 +42,6 @@ struct HvParams {
   const double* up_e;
   const int32_t* up_rowp;
   const uint32_t* up_packed;
-  const int32_t* up_rank;
   // down spin
   const int32_t* dn_bits;
   const double* dn_e;
@@ -54,8 +53,8 @@ struct HvParams {
   double dn_vt[4];
   const uint32_t* dn_s4;
   double dn_s4_scale;
-  const int32_t* dn_rank;
   const double* values;
+  int nvals;  // entries of values (staged in LDS after the row with EF == 4)
   // cross term
   double cross_u;
   const double* cross_imp;
@@ -63,8 +62,7 @@ struct HvParams {
   // impurity interaction
   int kan_ml;
   const int32_t* kan_ptr;
-  const int32_t* kan_a;
-  const int32_t* kan_b;
+  const int64_t* kan_off;  // partner state's element offset per term
   const double* kan_c;
   // operands
   const double* v;
@@ -139,7 +137,8 @@ constexpr int kStage = 8;
 
 // EF: down-spin entry format: 0 = 32-bit sliced ELL (column | value index),
 // 1/2 = 16-bit entries in slot quads (four / two distinct values),
-// 3 = signed byte offsets in 16-byte slot quads (values {a} or {a, -a}).
+// 3 = signed byte offsets in 16-byte slot quads (values {a} or {a, -a}),
+// 4 = the 32-bit sliced ELL with the value table staged in LDS after the row.
 //
 // Thread tid of a block with T threads owns columns c0 + tid + j*T (j < J)
 // of each chunk c0; a wavefront therefore covers exactly one ELL slice per j,
@@ -156,7 +155,6 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   const int tid = threadIdx.x;
   const int Ddn = p.Ddn;
   const int na = 1 << (p.Ns - p.nb);
-  const unsigned lowmask = (1u << p.nb) - 1u;
   if (p.done && *p.done) return;
   const double vscale = p.dscale ? p.dscale[0] : p.vscale;
   const double wscale = p.dscale ? p.dscale[1] : p.wscale;
@@ -172,6 +170,11 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   }
   const double* __restrict__ vals = p.values;
   double dsum = 0.0;
+  // EF == 4: the value table after the row (the first row's barrier orders
+  // these writes before any read)
+  double* s_vals = s_v + Ddn + 1;
+  if (EF == 4)
+    for (int i = tid; i < p.nvals; i += T) s_vals[i] = vals[i];
 
   for (int slot0 = blockIdx.x; slot0 < p.Dup; slot0 += gridDim.x) {
     // work-groups go to the XCDs in turn (b mod 8): within each window of
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
             qpa = qpn;
           }
         }
-      } else if (EF) {
+      } else if (EF == 1 || EF == 2) {
         // 16-bit entries, four per 64-bit quad, same pipelining
         constexpr int kQ = 8;
         uint64_t qa[kQ], qn[kQ];
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           }
         }
       } else {
-        // 32-bit sliced ELL: slot k of all J columns loaded together
+        // 32-bit sliced ELL: four slots of all J columns loaded together,
+        // then consumed in slot order
         int len[J], base[J];
         int maxlen = 0;
 #pragma unroll
@@ -345,25 +349,69 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           base[j] = len[j] > 0 ? p.dn_slptr[sl] + (DCOL(j) & 63) : 0;
           maxlen = max(maxlen, len[j]);
         }
-        for (int k = 0; k < maxlen; ++k) {
-          // k < maxlen means some column has entries, so the array is not
-          // empty; indices are clamped into each column's own slots
-          uint32_t e[J];
+        const double* __restrict__ vt = EF == 4 ? s_vals : vals;
+        // k < maxlen means some column has entries, so the array is not
+        // empty; indices are clamped into each column's own slots, and a
+        // column past its length reads entry 0 (value 0.0)
+        constexpr int kSlots = 4;
+        for (int k0 = 0; k0 < maxlen; k0 += kSlots) {
+          uint32_t e[kSlots][J];
 #pragma unroll
-          for (int j = 0; j < J; ++j) {
-            const uint32_t t = p.dn_ell[base[j] + min(k, max(len[j] - 1, 0)) * kSlice];
-            e[j] = k < len[j] ? t : 0u;
-          }
+          for (int s = 0; s < kSlots; ++s)
 #pragma unroll
-          for (int j = 0; j < J; ++j) {
-            const int c = (int)(e[j] & kColMask);
-            acc[j] = fma(vals[e[j] >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
+            for (int j = 0; j < J; ++j) e[s][j] = p.dn_ell[base[j] + min(k0 + s, max(len[j] - 1, 0)) * kSlice];
+#pragma unroll
+          for (int s = 0; s < kSlots; ++s) {
+            if (k0 + s >= maxlen) break;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+              const uint32_t x = k0 + s < len[j] ? e[s][j] : 0u;
+              const int c = (int)(x & kColMask);
+              acc[j] = fma(vt[x >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
+            }
           }
         }
       }
-      // up-spin hopping: linear combination of whole rows (coalesced)
+      // up-spin hopping: linear combination of whole rows (coalesced); the
+      // loads of four (then two) neighbour rows are in flight together,
+      // each column's FMAs still in neighbour order.  The quad forms (EF 1-3)
+      // keep two: their slot quads already fill the registers.
       if (UP) {
-        for (int q = hb; q < he; ++q) {
+        int q = hb;
+        for (; (EF == 0 || EF == 4) && q + 3 < he; q += 4) {
+          uint32_t e[4];
+          double h[4];
+          const double* __restrict__ src[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            e[r] = p.up_packed[q + r];
+            h[r] = vals[e[r] >> kColBits] * vscale;
+            src[r] = V + (size_t)(e[r] & kColMask) * Ddn;
+          }
+          double x[4][J];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < J; ++j) x[r][j] = src[r][DCOL(j)];
+#pragma unroll
+          for (int j = 0; j < J; ++j)
+            acc[j] = fma(h[3], x[3][j], fma(h[2], x[2][j], fma(h[1], x[1][j], fma(h[0], x[0][j], acc[j]))));
+        }
+        for (; q + 1 < he; q += 2) {
+          const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
+          const double h0 = vals[e0 >> kColBits] * vscale, h1 = vals[e1 >> kColBits] * vscale;
+          const double* __restrict__ src0 = V + (size_t)(e0 & kColMask) * Ddn;
+          const double* __restrict__ src1 = V + (size_t)(e1 & kColMask) * Ddn;
+          double x0[J], x1[J];
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+            x0[j] = src0[DCOL(j)];
+            x1[j] = src1[DCOL(j)];
+          }
+#pragma unroll
+          for (int j = 0; j < J; ++j) acc[j] = fma(h1, x1[j], fma(h0, x0[j], acc[j]));
+        }
+        if (q < he) {
           const uint32_t e = p.up_packed[q];
           const double h = vals[e >> kColBits] * vscale;
           const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
@@ -381,11 +429,8 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           const int bd = p.dn_bits[DCOL(j)];
           const int b = bd >> p.nb;
           const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-          for (int q = kb; q < ke; ++q) {
-            const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[q] << p.nb)];
-            const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[q] << p.nb)];
-            acc[j] = fma(p.kan_c[q], V[(size_t)up2 * Ddn + dn2] * vscale, acc[j]);
-          }
+          for (int q = kb; q < ke; ++q)
+            acc[j] = fma(p.kan_c[q], V[(int64_t)rowoff + DCOL(j) + p.kan_off[q]] * vscale, acc[j]);
         }
       }
 #pragma unroll
@@ -523,7 +568,7 @@ struct HvBatchJob {
 template <bool STAGED>
 __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const HvParams& p, const double* __restrict__ V,
                                                    double* __restrict__ W, int u, int d, double vscale, double wscale,
-                                                   int na, unsigned lowmask, const double* R) {
+                                                   int na, const double* R) {
   const int Ddn = p.Ddn;
   const double* __restrict__ vals = p.values;
   const int cross = J.cross;
@@ -579,11 +624,7 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
   if (J.kan) {
     const int a = bu >> p.nb, b = bd >> p.nb;
     const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
-    for (int r = kb; r < ke; ++r) {
-      const int up2 = p.up_rank[(bu & lowmask) | (p.kan_a[r] << p.nb)];
-      const int dn2 = p.dn_rank[(bd & lowmask) | (p.kan_b[r] << p.nb)];
-      acc = fma(p.kan_c[r], V[(size_t)up2 * Ddn + dn2] * vscale, acc);
-    }
+    for (int r = kb; r < ke; ++r) acc = fma(p.kan_c[r], V[(int64_t)rowoff + d + p.kan_off[r]] * vscale, acc);
   }
   W[rowoff + d] = acc;
   return vself * vscale * acc;
@@ -637,7 +678,6 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
     const double vscale = p.dscale ? p.dscale[0] : p.vscale;
     const double wscale = p.dscale ? p.dscale[1] : p.wscale;
     const int na = 1 << (p.Ns - p.nb);
-    const unsigned lowmask = (1u << p.nb) - 1u;
     // the unit's rows are V[row0 * Ddn, (row0 + nrows) * Ddn): staged when
     // they fit (uniform over the work-group)
     const int span = U.nrows * Ddn;
@@ -663,9 +703,9 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
       const bool live = u < U.row0 + U.nrows && d < Ddn;
       double vw = 0.0;
       if (live) {
-        vw = staged ? hv_batch_element<true>(J, p, V, W, u, d, vscale, wscale, na, lowmask,
+        vw = staged ? hv_batch_element<true>(J, p, V, W, u, d, vscale, wscale, na,
                                              s_unit + (size_t)(u - U.row0) * Ddn)
-                    : hv_batch_element<false>(J, p, V, W, u, d, vscale, wscale, na, lowmask, nullptr);
+                    : hv_batch_element<false>(J, p, V, W, u, d, vscale, wscale, na, nullptr);
       }
       if (J.tdot) {
         // Lanczos runs: this task's share of <v, w>, one wavefront tree,
@@ -710,8 +750,13 @@ Launch plan(const DevSector& ds) {
   const size_t rowb = (size_t)(Ddn + 1) * sizeof(double);
   L.lds = rowb <= lds_budget();
   L.ef = 0;
+  size_t valb = 0;
   if (L.lds && ds.dn.ell_s4 != nullptr) L.ef = 3;
   else if (L.lds && ds.dn.ell_q4 != nullptr) L.ef = ds.dn.nvt16 <= 2 ? 2 : 1;
+  else if (L.lds && rowb + (size_t)ds.nvalues * sizeof(double) <= lds_budget()) {
+    L.ef = 4;  // 32-bit entries with the value table beside the row in LDS
+    valb = (size_t)ds.nvalues * sizeof(double);
+  }
   int J = 1;
   while (J < 4 && (Ddn + J - 1) / J > 256) J *= 2;
   // as few chunks as 1024 threads allow, split evenly so that few lanes idle
@@ -721,7 +766,7 @@ Launch plan(const DevSector& ds) {
   L.J = J;
   L.block = (int)T;
   L.grid = (int)std::min<int64_t>(ds.up.D, 1 << 20);
-  L.smem = L.lds ? rowb : 0;
+  L.smem = L.lds ? rowb + valb : 0;
   return L;
 }
 
@@ -739,13 +784,15 @@ void launch_t(const HvParams& p, const Launch& L, bool up, hipStream_t s) {
 template <int J, int CROSS>
 void launch_c(const HvParams& p, const Launch& L, bool kan, bool up, hipStream_t s) {
   if (kan) {
-    if (L.ef == 3) launch_t<J, CROSS, true, true, 3>(p, L, up, s);
+    if (L.ef == 4) launch_t<J, CROSS, true, true, 4>(p, L, up, s);
+    else if (L.ef == 3) launch_t<J, CROSS, true, true, 3>(p, L, up, s);
     else if (L.ef == 2) launch_t<J, CROSS, true, true, 2>(p, L, up, s);
     else if (L.ef == 1) launch_t<J, CROSS, true, true, 1>(p, L, up, s);
     else if (L.lds) launch_t<J, CROSS, true, true, 0>(p, L, up, s);
     else launch_t<J, CROSS, true, false, 0>(p, L, up, s);
   } else {
-    if (L.ef == 3) launch_t<J, CROSS, false, true, 3>(p, L, up, s);
+    if (L.ef == 4) launch_t<J, CROSS, false, true, 4>(p, L, up, s);
+    else if (L.ef == 3) launch_t<J, CROSS, false, true, 3>(p, L, up, s);
     else if (L.ef == 2) launch_t<J, CROSS, false, true, 2>(p, L, up, s);
     else if (L.ef == 1) launch_t<J, CROSS, false, true, 1>(p, L, up, s);
     else if (L.lds) launch_t<J, CROSS, false, true, 0>(p, L, up, s);
@@ -784,7 +831,6 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   p.up_e = ds.up.ediag;
   p.up_rowp = ds.up.rowp;
   p.up_packed = ds.up.packed;
-  p.up_rank = ds.up.rank;
   p.dn_bits = ds.dn.bits;
   p.dn_e = ds.dn.ediag;
   p.dn_slptr = ds.dn.sl_ptr;
@@ -795,15 +841,14 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   p.dn_s4 = ds.dn.ell_s4;
   p.dn_s4_scale = ds.dn.s4_scale;
   for (int q = 0; q < 4; ++q) p.dn_vt[q] = ds.dn.vt16[q];
-  p.dn_rank = ds.dn.rank;
   p.values = ds.values;
+  p.nvals = ds.nvalues;
   p.cross_u = ds.cross_u;
   p.cross_imp = ds.cross_imp;
   p.cross_rowtab = ds.cross_rowtab;
   p.kan_ml = ds.kan_ml;
   p.kan_ptr = ds.kan_ptr;
-  p.kan_a = ds.kan_a;
-  p.kan_b = ds.kan_b;
+  p.kan_off = ds.kan_off;
   p.kan_c = ds.kan_c;
   p.v = a.v;
   p.w = a.w;

@@ -66,7 +66,6 @@ void upload_spin(Packer& pk, const SpinTables& sp, DevSpin& d) {
   pk.add(d.ell_q4, sp.ell_q4);
   pk.add(d.ell_s4, sp.ell_s4);
   d.s4_scale = sp.s4_scale;
-  pk.add(d.rank, sp.rank);
 }
 
 // Reverse Cuthill-McKee order of the graph whose adjacency is the CSR
@@ -169,25 +168,42 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s,
   upload_spin(pk, st.up, ds.up);
   upload_spin(pk, st.dn, ds.dn);
   pk.add(ds.values, st.values);
+  ds.nvalues = (int)st.values.size();
   ds.cross_mode = st.cross_mode;
   ds.cross_u = st.cross_u;
   pk.add(ds.cross_imp, st.cross_imp);
   pk.add(ds.cross_rowtab, st.cross_rowtab);
   ds.kan_ml = st.kan_ml;
-  std::vector<int32_t> ka, kb;
-  std::vector<double> kc;
   if (st.kan_ml > 0) {
-    ka.resize(st.kan.size());
-    kb.resize(st.kan.size());
-    kc.resize(st.kan.size());
-    for (size_t q = 0; q < st.kan.size(); ++q) {
-      ka[q] = st.kan[q].new_a;
-      kb[q] = st.kan[q].new_b;
-      kc[q] = st.kan[q].coef;
-    }
+    // Each impurity term moves the impurity bits (a, b) -> (a', b') with the
+    // same occupation counts.  In the basis order (positions in ascending
+    // lexicographic order, impurity orbitals on the highest positions) the
+    // states sharing their bath bits form one contiguous run ordered by the
+    // impurity pattern alone, so the partner of (u, d) is (u + du, d + dd)
+    // with du, dd fixed by (a, a') and (b, b'): one element offset per term,
+    // du * D_dn + dd, read from the representative state of its group.
+    const int ml = st.kan_ml, na = 1 << ml, nb = ds.Ns - ml;
+    std::vector<int64_t> koff(st.kan.size());
+    std::vector<double> kc(st.kan.size());
+    for (int a = 0; a < na; ++a)
+      for (int b = 0; b < na; ++b) {
+        const int idx = a * na + b;
+        if (st.kan_ptr[idx + 1] == st.kan_ptr[idx]) continue;
+        const int bath_up = st.nup - __builtin_popcount(a), bath_dn = st.ndn - __builtin_popcount(b);
+        const int32_t lu = (int32_t)((1ll << bath_up) - 1), ld = (int32_t)((1ll << bath_dn) - 1);
+        const int32_t ru = st.up.rank[lu | (a << nb)], rd = st.dn.rank[ld | (b << nb)];
+        for (int q = st.kan_ptr[idx]; q < st.kan_ptr[idx + 1]; ++q) {
+          const KanEntry& e = st.kan[q];
+          EDH_CHECK(__builtin_popcount(e.new_a) == __builtin_popcount(a) &&
+                        __builtin_popcount(e.new_b) == __builtin_popcount(b),
+                    EDH_ERR_INTERNAL, "impurity term changes an occupation count");
+          const int64_t du = st.up.rank[lu | (e.new_a << nb)] - ru, dd = st.dn.rank[ld | (e.new_b << nb)] - rd;
+          koff[q] = du * st.dn.D + dd;
+          kc[q] = e.coef;
+        }
+      }
     pk.add(ds.kan_ptr, st.kan_ptr);
-    pk.add(ds.kan_a, ka);
-    pk.add(ds.kan_b, kb);
+    pk.add(ds.kan_off, koff);
     pk.add(ds.kan_c, kc);
   }
   pk.add(ds.counters, std::vector<int32_t>(64, 0));
