@@ -277,9 +277,10 @@ int64_t hv_batch_tasks(int64_t dup, int64_t ddn);  // wavefront tasks of a whole
 int hv_batch_lds_elems(const std::vector<int64_t>& spans);
 void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, int block, int lds_elems,
                      hipStream_t s);
-// Large operands with long rows (D_dn > 4608, N >= 2^22) of a batch go to
-// the single-sector rows kernel instead (their rows staged whole in LDS,
-// 1024-thread work-groups); the choice depends on the operand alone.  Its
+// Large operands (N >= 2^22 with D_dn > 4608, or with D_up and D_dn both
+// >= 1024) of a batch go to the single-sector rows kernel instead (their
+// rows staged whole in LDS, up to 1024-thread work-groups); the choice
+// depends on the operand alone.  Its
 // Lanczos <v, w> shares are one per work-group (hv_batch_rows_partials).
 bool hv_batch_routed(int64_t dup, int64_t ddn);
 int64_t hv_batch_rows_partials(int64_t dup);

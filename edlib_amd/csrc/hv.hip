@@ -759,10 +759,13 @@ Launch plan(const DevSector& ds) {
   }
   int J = 1;
   while (J < 4 && (Ddn + J - 1) / J > 256) J *= 2;
-  // as few chunks as 1024 threads allow, split evenly so that few lanes idle
-  const int64_t nch = (Ddn + (int64_t)J * 1024 - 1) / ((int64_t)J * 1024);
+  // (J, largest work-group) = (4, 1024) is within 5 % of the best of
+  // (1..4, 256..1024) on 14- and 16-site sectors of all three models
+  const int64_t tmax = 1024;
+  // as few chunks as tmax threads allow, split evenly so that few lanes idle
+  const int64_t nch = (Ddn + (int64_t)J * tmax - 1) / ((int64_t)J * tmax);
   int64_t T = ((Ddn + J * nch - 1) / (J * nch) + 63) / 64 * 64;
-  T = std::min<int64_t>(std::max<int64_t>(T, 64), 1024);
+  T = std::min<int64_t>(std::max<int64_t>(T, 64), tmax);
   L.J = J;
   L.block = (int)T;
   L.grid = (int)std::min<int64_t>(ds.up.D, 1 << 20);
@@ -1005,7 +1008,13 @@ void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, i
   EDH_HIP(hipGetLastError());
 }
 
-bool hv_batch_routed(int64_t dup, int64_t ddn) { return ddn > kStageMax && dup * ddn >= (int64_t(1) << 22); }
+// Large operands go to the rows kernel: rows too long to stage in the
+// grouped kernel, or at least 1024 rows of at least 1024 elements (its
+// row-per-work-group form then beats the grouped per-element form: 14-site
+// Kanamori sector (7,7), 11.8 M states, 0.40 against 0.56 ms per H*v).
+bool hv_batch_routed(int64_t dup, int64_t ddn) {
+  return dup * ddn >= (int64_t(1) << 22) && (ddn > kStageMax || (ddn >= 1024 && dup >= 1024));
+}
 
 int64_t hv_batch_rows_partials(int64_t dup) { return std::min<int64_t>(dup, 1 << 20); }
 
