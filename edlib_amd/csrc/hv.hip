@@ -375,10 +375,12 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
       // up-spin hopping: linear combination of whole rows (coalesced); the
       // loads of four (then two) neighbour rows are in flight together,
       // each column's FMAs still in neighbour order.  The quad forms (EF 1-3)
-      // keep two: their slot quads already fill the registers.
+      // take one row at a time: their slot quads already fill the registers
+      // (two at a time made the 16-site sector 4.05 -> 4.22 ms).
+      constexpr bool kWide = EF == 0 || EF == 4;
       if (UP) {
         int q = hb;
-        for (; (EF == 0 || EF == 4) && q + 3 < he; q += 4) {
+        for (; kWide && q + 3 < he; q += 4) {
           uint32_t e[4];
           double h[4];
           const double* __restrict__ src[4];
@@ -397,7 +399,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
           for (int j = 0; j < J; ++j)
             acc[j] = fma(h[3], x[3][j], fma(h[2], x[2][j], fma(h[1], x[1][j], fma(h[0], x[0][j], acc[j]))));
         }
-        for (; q + 1 < he; q += 2) {
+        for (; kWide && q + 1 < he; q += 2) {
           const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
           const double h0 = vals[e0 >> kColBits] * vscale, h1 = vals[e1 >> kColBits] * vscale;
           const double* __restrict__ src0 = V + (size_t)(e0 & kColMask) * Ddn;
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
 #pragma unroll
           for (int j = 0; j < J; ++j) acc[j] = fma(h1, x1[j], fma(h0, x0[j], acc[j]));
         }
-        if (q < he) {
+        for (; q < he; ++q) {
           const uint32_t e = p.up_packed[q];
           const double h = vals[e >> kColBits] * vscale;
           const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
