@@ -128,7 +128,7 @@ def cpu_socrs_configs():
             ts.append(time.perf_counter() - t0)
         dt = float(np.median(ts))
         out.append({"workload": name, "dim": s.n, "av_ms": round(dt * 1e3, 3), "gbs": round(24 * s.n / dt / 1e9, 3),
-                    "fill_s": round(tf, 2)})
+                    "fill_s": round(tf, 2), "threads": int(eo.lib().orc_num_threads())})
         progress("CPU SOCRS av, %s: %.1f ms" % (name, dt * 1e3))
         del s
     return out
