@@ -97,3 +97,27 @@ def test_canonical_anticommutation():
             ac = Cd[i].T @ Cd[j] + Cd[j] @ Cd[i].T
             assert np.allclose(ac, np.eye(dim) if i == j else 0.0)
             assert np.allclose(Cd[i] @ Cd[j] + Cd[j] @ Cd[i], 0.0)
+
+
+@pytest.mark.parametrize("Ns,ml", [(7, 2), (9, 3), (12, 1), (14, 3)])
+def test_impurity_moves_shift_the_rank_by_a_constant(Ns, ml):
+    """The device applies an impurity interaction term (a, b) -> (a', b')
+    (impurity orbitals 0..ml-1, the highest bits of each spin half; the
+    bath bits unchanged, the occupation count of the impurity kept) as one
+    element offset: in the reference basis order (SzSymmetry.h:191-202) the
+    states sharing their bath bits form a contiguous run ordered by the
+    impurity pattern alone, so rank(bath | a') - rank(bath | a) depends on
+    (a, a') only.  Checked for every pattern pair of every particle number."""
+    nb = Ns - ml
+    low = (1 << nb) - 1
+    for k in range(Ns + 1):
+        B = eo.basis(Ns, k)
+        rank = {int(b): i for i, b in enumerate(B)}
+        shift = {}
+        for i, b in enumerate(B):
+            a = int(b) >> nb
+            for a2 in range(1 << ml):
+                if bin(a2).count("1") != bin(a).count("1"):
+                    continue
+                j = rank[(int(b) & low) | (a2 << nb)]
+                assert shift.setdefault((a, a2), j - i) == j - i, (k, a, a2)
