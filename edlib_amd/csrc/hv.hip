@@ -226,203 +226,219 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
         const double vd = LDS ? s_v[d] : V[rowoff + d] * vscale;
         acc[j] = fma(eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd, na), vd, wp);
       }
-      // down-spin hopping, gathers from the staged row
-      if (EF == 3) {
-        // all quads of column group j are loaded at once and those of group
-        // j+1 are in flight while group j is consumed
-        const uint4* __restrict__ S4 = reinterpret_cast<const uint4*>(p.dn_s4);
-        const char* sv = reinterpret_cast<const char*>(s_v);
-        uint4 qa[kQuadRegs], qn[kQuadRegs];
-        int nqa, nqn = 0;
-        const uint4* qpa;
-        {
-          const int d0 = DCOL(0);
-          const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
-          nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
-          qpa = S4 + p.dn_q4ptr[sl] + (d0 & 63);
-#pragma unroll
-          for (int q = 0; q < kQuadRegs; ++q)
-            if (q < nqa) qa[q] = qpa[q * kSlice];
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const uint4* qpn = qpa;
-          if (j + 1 < J) {
-            const int d1 = DCOL(j + 1);
-            const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
-            nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
-            qpn = S4 + p.dn_q4ptr[sl] + (d1 & 63);
-#pragma unroll
+      auto down_term = [&]() {
+        // down-spin hopping, gathers from the staged row
+        if (EF == 3) {
+          // all quads of column group j are loaded at once and those of group
+          // j+1 are in flight while group j is consumed
+          const uint4* __restrict__ S4 = reinterpret_cast<const uint4*>(p.dn_s4);
+          const char* sv = reinterpret_cast<const char*>(s_v);
+          uint4 qa[kQuadRegs], qn[kQuadRegs];
+          int nqa, nqn = 0;
+          const uint4* qpa;
+          {
+            const int d0 = DCOL(0);
+            const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
+            nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+            qpa = S4 + p.dn_q4ptr[sl] + (d0 & 63);
+  #pragma unroll
             for (int q = 0; q < kQuadRegs; ++q)
-              if (q < nqn) qn[q] = qpn[q * kSlice];
+              if (q < nqa) qa[q] = qpa[q * kSlice];
           }
-          double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-          for (int q = 0; q < kQuadRegs; ++q) {
-            if (q < nqa) {
-              a0 = s4_fma(qa[q].x, sv, a0);
-              a1 = s4_fma(qa[q].y, sv, a1);
-              a0 = s4_fma(qa[q].z, sv, a0);
-              a1 = s4_fma(qa[q].w, sv, a1);
-            }
-          }
-          for (int q = kQuadRegs; q < nqa; ++q) {
-            const uint4 c = qpa[q * kSlice];
-            a0 = s4_fma(c.x, sv, a0);
-            a1 = s4_fma(c.y, sv, a1);
-            a0 = s4_fma(c.z, sv, a0);
-            a1 = s4_fma(c.w, sv, a1);
-          }
-          acc[j] = fma(p.dn_s4_scale, a0 + a1, acc[j]);
-          if (j + 1 < J) {
-#pragma unroll
-            for (int q = 0; q < kQuadRegs; ++q) qa[q] = qn[q];
-            nqa = nqn;
-            qpa = qpn;
-          }
-        }
-      } else if (EF == 1 || EF == 2) {
-        // 16-bit entries, four per 64-bit quad, same pipelining
-        constexpr int kQ = 8;
-        uint64_t qa[kQ], qn[kQ];
-        int nqa, nqn = 0;
-        const uint64_t* qpa;
-        {
-          const int d0 = DCOL(0);
-          const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
-          nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
-          qpa = p.dn_q4 + p.dn_q4ptr[sl] + (d0 & 63);
-#pragma unroll
-          for (int q = 0; q < kQ; ++q)
-            if (q < nqa) qa[q] = qpa[q * kSlice];
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const uint64_t* qpn = qpa;
-          if (j + 1 < J) {
-            const int d1 = DCOL(j + 1);
-            const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
-            nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
-            qpn = p.dn_q4 + p.dn_q4ptr[sl] + (d1 & 63);
-#pragma unroll
-            for (int q = 0; q < kQ; ++q)
-              if (q < nqn) qn[q] = qpn[q * kSlice];
-          }
-          double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-          for (int q = 0; q < kQ; ++q) {
-            if (q < nqa) {
-              const uint64_t c = qa[q];
-              const uint32_t e0 = (uint32_t)c & 0xFFFFu, e1 = (uint32_t)(c >> 16) & 0xFFFFu;
-              const uint32_t e2 = (uint32_t)(c >> 32) & 0xFFFFu, e3 = (uint32_t)(c >> 48);
-              a0 = fma(q4_value<EF>(p, e0), s_v[e0 & 0x3FFFu], a0);
-              a1 = fma(q4_value<EF>(p, e1), s_v[e1 & 0x3FFFu], a1);
-              a0 = fma(q4_value<EF>(p, e2), s_v[e2 & 0x3FFFu], a0);
-              a1 = fma(q4_value<EF>(p, e3), s_v[e3 & 0x3FFFu], a1);
-            }
-          }
-          for (int q = kQ; q < nqa; ++q) {
-            const uint64_t c = qpa[q * kSlice];
-#pragma unroll
-            for (int sub = 0; sub < 4; ++sub) {
-              const uint32_t e = (uint32_t)(c >> (16 * sub)) & 0xFFFFu;
-              a0 = fma(q4_value<EF>(p, e), s_v[e & 0x3FFFu], a0);
-            }
-          }
-          acc[j] += a0 + a1;
-          if (j + 1 < J) {
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) qa[q] = qn[q];
-            nqa = nqn;
-            qpa = qpn;
-          }
-        }
-      } else {
-        // 32-bit sliced ELL: four slots of all J columns loaded together,
-        // then consumed in slot order
-        int len[J], base[J];
-        int maxlen = 0;
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const int sl = DCOL(j) >> 6;
-          len[j] = p.dn_sllen[sl];
-          base[j] = len[j] > 0 ? p.dn_slptr[sl] + (DCOL(j) & 63) : 0;
-          maxlen = max(maxlen, len[j]);
-        }
-        const double* __restrict__ vt = EF == 4 ? s_vals : vals;
-        // k < maxlen means some column has entries, so the array is not
-        // empty; indices are clamped into each column's own slots, and a
-        // column past its length reads entry 0 (value 0.0)
-        constexpr int kSlots = 4;
-        for (int k0 = 0; k0 < maxlen; k0 += kSlots) {
-          uint32_t e[kSlots][J];
-#pragma unroll
-          for (int s = 0; s < kSlots; ++s)
-#pragma unroll
-            for (int j = 0; j < J; ++j) e[s][j] = p.dn_ell[base[j] + min(k0 + s, max(len[j] - 1, 0)) * kSlice];
-#pragma unroll
-          for (int s = 0; s < kSlots; ++s) {
-            if (k0 + s >= maxlen) break;
-#pragma unroll
-            for (int j = 0; j < J; ++j) {
-              const uint32_t x = k0 + s < len[j] ? e[s][j] : 0u;
-              const int c = (int)(x & kColMask);
-              acc[j] = fma(vt[x >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
-            }
-          }
-        }
-      }
-      // up-spin hopping: linear combination of whole rows (coalesced); the
-      // loads of four (then two) neighbour rows are in flight together,
-      // each column's FMAs still in neighbour order.  The quad forms (EF 1-3)
-      // take one row at a time: their slot quads already fill the registers
-      // (two at a time made the 16-site sector 4.05 -> 4.22 ms).
-      constexpr bool kWide = EF == 0 || EF == 4;
-      if (UP) {
-        int q = hb;
-        for (; kWide && q + 3 < he; q += 4) {
-          uint32_t e[4];
-          double h[4];
-          const double* __restrict__ src[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            e[r] = p.up_packed[q + r];
-            h[r] = vals[e[r] >> kColBits] * vscale;
-            src[r] = V + (size_t)(e[r] & kColMask) * Ddn;
-          }
-          double x[4][J];
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int j = 0; j < J; ++j) x[r][j] = src[r][DCOL(j)];
-#pragma unroll
-          for (int j = 0; j < J; ++j)
-            acc[j] = fma(h[3], x[3][j], fma(h[2], x[2][j], fma(h[1], x[1][j], fma(h[0], x[0][j], acc[j]))));
-        }
-        for (; kWide && q + 1 < he; q += 2) {
-          const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
-          const double h0 = vals[e0 >> kColBits] * vscale, h1 = vals[e1 >> kColBits] * vscale;
-          const double* __restrict__ src0 = V + (size_t)(e0 & kColMask) * Ddn;
-          const double* __restrict__ src1 = V + (size_t)(e1 & kColMask) * Ddn;
-          double x0[J], x1[J];
-#pragma unroll
+  #pragma unroll
           for (int j = 0; j < J; ++j) {
-            x0[j] = src0[DCOL(j)];
-            x1[j] = src1[DCOL(j)];
+            const uint4* qpn = qpa;
+            if (j + 1 < J) {
+              const int d1 = DCOL(j + 1);
+              const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
+              nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+              qpn = S4 + p.dn_q4ptr[sl] + (d1 & 63);
+  #pragma unroll
+              for (int q = 0; q < kQuadRegs; ++q)
+                if (q < nqn) qn[q] = qpn[q * kSlice];
+            }
+            double a0 = 0.0, a1 = 0.0;
+  #pragma unroll
+            for (int q = 0; q < kQuadRegs; ++q) {
+              if (q < nqa) {
+                a0 = s4_fma(qa[q].x, sv, a0);
+                a1 = s4_fma(qa[q].y, sv, a1);
+                a0 = s4_fma(qa[q].z, sv, a0);
+                a1 = s4_fma(qa[q].w, sv, a1);
+              }
+            }
+            for (int q = kQuadRegs; q < nqa; ++q) {
+              const uint4 c = qpa[q * kSlice];
+              a0 = s4_fma(c.x, sv, a0);
+              a1 = s4_fma(c.y, sv, a1);
+              a0 = s4_fma(c.z, sv, a0);
+              a1 = s4_fma(c.w, sv, a1);
+            }
+            acc[j] = fma(p.dn_s4_scale, a0 + a1, acc[j]);
+            if (j + 1 < J) {
+  #pragma unroll
+              for (int q = 0; q < kQuadRegs; ++q) qa[q] = qn[q];
+              nqa = nqn;
+              qpa = qpn;
+            }
           }
-#pragma unroll
-          for (int j = 0; j < J; ++j) acc[j] = fma(h1, x1[j], fma(h0, x0[j], acc[j]));
+        } else if (EF == 1 || EF == 2) {
+          // 16-bit entries, four per 64-bit quad, same pipelining
+          constexpr int kQ = 8;
+          uint64_t qa[kQ], qn[kQ];
+          int nqa, nqn = 0;
+          const uint64_t* qpa;
+          {
+            const int d0 = DCOL(0);
+            const int sl = __builtin_amdgcn_readfirstlane(d0 >> 6);
+            nqa = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+            qpa = p.dn_q4 + p.dn_q4ptr[sl] + (d0 & 63);
+  #pragma unroll
+            for (int q = 0; q < kQ; ++q)
+              if (q < nqa) qa[q] = qpa[q * kSlice];
+          }
+  #pragma unroll
+          for (int j = 0; j < J; ++j) {
+            const uint64_t* qpn = qpa;
+            if (j + 1 < J) {
+              const int d1 = DCOL(j + 1);
+              const int sl = __builtin_amdgcn_readfirstlane(d1 >> 6);
+              nqn = __builtin_amdgcn_readfirstlane((p.dn_sllen[sl] + 3) >> 2);
+              qpn = p.dn_q4 + p.dn_q4ptr[sl] + (d1 & 63);
+  #pragma unroll
+              for (int q = 0; q < kQ; ++q)
+                if (q < nqn) qn[q] = qpn[q * kSlice];
+            }
+            double a0 = 0.0, a1 = 0.0;
+  #pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+              if (q < nqa) {
+                const uint64_t c = qa[q];
+                const uint32_t e0 = (uint32_t)c & 0xFFFFu, e1 = (uint32_t)(c >> 16) & 0xFFFFu;
+                const uint32_t e2 = (uint32_t)(c >> 32) & 0xFFFFu, e3 = (uint32_t)(c >> 48);
+                a0 = fma(q4_value<EF>(p, e0), s_v[e0 & 0x3FFFu], a0);
+                a1 = fma(q4_value<EF>(p, e1), s_v[e1 & 0x3FFFu], a1);
+                a0 = fma(q4_value<EF>(p, e2), s_v[e2 & 0x3FFFu], a0);
+                a1 = fma(q4_value<EF>(p, e3), s_v[e3 & 0x3FFFu], a1);
+              }
+            }
+            for (int q = kQ; q < nqa; ++q) {
+              const uint64_t c = qpa[q * kSlice];
+  #pragma unroll
+              for (int sub = 0; sub < 4; ++sub) {
+                const uint32_t e = (uint32_t)(c >> (16 * sub)) & 0xFFFFu;
+                a0 = fma(q4_value<EF>(p, e), s_v[e & 0x3FFFu], a0);
+              }
+            }
+            acc[j] += a0 + a1;
+            if (j + 1 < J) {
+  #pragma unroll
+              for (int q = 0; q < kQ; ++q) qa[q] = qn[q];
+              nqa = nqn;
+              qpa = qpn;
+            }
+          }
+        } else {
+          // 32-bit sliced ELL: four slots of all J columns loaded together,
+          // then consumed in slot order
+          int len[J], base[J];
+          int maxlen = 0;
+  #pragma unroll
+          for (int j = 0; j < J; ++j) {
+            const int sl = DCOL(j) >> 6;
+            len[j] = p.dn_sllen[sl];
+            base[j] = len[j] > 0 ? p.dn_slptr[sl] + (DCOL(j) & 63) : 0;
+            maxlen = max(maxlen, len[j]);
+          }
+          const double* __restrict__ vt = EF == 4 ? s_vals : vals;
+          // k < maxlen means some column has entries, so the array is not
+          // empty; indices are clamped into each column's own slots, and a
+          // column past its length reads entry 0 (value 0.0)
+          constexpr int kSlots = 4;
+          for (int k0 = 0; k0 < maxlen; k0 += kSlots) {
+            uint32_t e[kSlots][J];
+  #pragma unroll
+            for (int s = 0; s < kSlots; ++s)
+  #pragma unroll
+              for (int j = 0; j < J; ++j) e[s][j] = p.dn_ell[base[j] + min(k0 + s, max(len[j] - 1, 0)) * kSlice];
+  #pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+              if (k0 + s >= maxlen) break;
+  #pragma unroll
+              for (int j = 0; j < J; ++j) {
+                const uint32_t x = k0 + s < len[j] ? e[s][j] : 0u;
+                const int c = (int)(x & kColMask);
+                acc[j] = fma(vt[x >> kColBits], LDS ? s_v[c] : V[rowoff + c] * vscale, acc[j]);
+              }
+            }
+          }
         }
-        for (; q < he; ++q) {
-          const uint32_t e = p.up_packed[q];
-          const double h = vals[e >> kColBits] * vscale;
-          const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
-          double x[J];
-#pragma unroll
-          for (int j = 0; j < J; ++j) x[j] = src[DCOL(j)];
-#pragma unroll
-          for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
+      };
+      auto up_term = [&]() {
+        // up-spin hopping: linear combination of whole rows (coalesced); the
+        // loads of four (then two) neighbour rows are in flight together,
+        // each column's FMAs still in neighbour order.  The quad forms (EF 1-3)
+        // take one row at a time: their slot quads already fill the registers
+        // (two at a time made the 16-site sector 4.05 -> 4.22 ms).
+        constexpr bool kWide = EF == 0 || EF == 4;
+        if (UP) {
+          int q = hb;
+          for (; kWide && q + 3 < he; q += 4) {
+            uint32_t e[4];
+            double h[4];
+            const double* __restrict__ src[4];
+  #pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              e[r] = p.up_packed[q + r];
+              h[r] = vals[e[r] >> kColBits] * vscale;
+              src[r] = V + (size_t)(e[r] & kColMask) * Ddn;
+            }
+            double x[4][J];
+  #pragma unroll
+            for (int r = 0; r < 4; ++r)
+  #pragma unroll
+              for (int j = 0; j < J; ++j) x[r][j] = src[r][DCOL(j)];
+  #pragma unroll
+            for (int j = 0; j < J; ++j)
+              acc[j] = fma(h[3], x[3][j], fma(h[2], x[2][j], fma(h[1], x[1][j], fma(h[0], x[0][j], acc[j]))));
+          }
+          for (; kWide && q + 1 < he; q += 2) {
+            const uint32_t e0 = p.up_packed[q], e1 = p.up_packed[q + 1];
+            const double h0 = vals[e0 >> kColBits] * vscale, h1 = vals[e1 >> kColBits] * vscale;
+            const double* __restrict__ src0 = V + (size_t)(e0 & kColMask) * Ddn;
+            const double* __restrict__ src1 = V + (size_t)(e1 & kColMask) * Ddn;
+            double x0[J], x1[J];
+  #pragma unroll
+            for (int j = 0; j < J; ++j) {
+              x0[j] = src0[DCOL(j)];
+              x1[j] = src1[DCOL(j)];
+            }
+  #pragma unroll
+            for (int j = 0; j < J; ++j) acc[j] = fma(h1, x1[j], fma(h0, x0[j], acc[j]));
+          }
+          for (; q < he; ++q) {
+            const uint32_t e = p.up_packed[q];
+            const double h = vals[e >> kColBits] * vscale;
+            const double* __restrict__ src = V + (size_t)(e & kColMask) * Ddn;
+            double x[J];
+  #pragma unroll
+            for (int j = 0; j < J; ++j) x[j] = src[DCOL(j)];
+  #pragma unroll
+            for (int j = 0; j < J; ++j) acc[j] = fma(h, x[j], acc[j]);
+          }
         }
+      };
+      // wavefronts of odd index take the up-spin rows first: while half of a
+      // work-group's wavefronts gather from the row in LDS, the other half
+      // wait on neighbour rows from L2 and beyond (each column's sum then
+      // runs diagonal, up-spin, down-spin terms instead of diagonal,
+      // down-spin, up-spin)
+      if (UP && ((tid >> 6) & 1)) {
+        up_term();
+        down_term();
+      } else {
+        down_term();
+        up_term();
       }
       if (KAN) {
         const int a = bu >> p.nb;
