@@ -5,7 +5,12 @@
 //   4ring   -- test/HubbardModel_Test.cpp:27-71 (E0, ground sector, observables);
 //   4ring_nomag -- test/StaticObservables_Test.cpp:27-72;
 //   dyson_hubbard0, dyson_hubbard, dyson_siam0 -- the self-energy from the
-//              Dyson equation (GreensFunction.h:202-223), see dyson().
+//              Dyson equation (GreensFunction.h:202-223), see dyson();
+//   storage  -- the storage's corners through its public members: every
+//              one-state sector's pair is (its diagonal element, [1])
+//              (zero_eigenapair, SpinResolvedStorage.h:52-56), and
+//              constant_shift adds shift * v to av until the next fill
+//              (SpinResolvedStorage.h:464-466).
 // Usage: edtest <case> <golden dir>.  Prints one JSON line; exit code 0 on pass.
 #include <unistd.h>
 
@@ -106,6 +111,67 @@ static int ring4(const std::string& dir, bool mag) {
   std::printf("{\"case\": \"%s\", \"E0\": %.12f, \"nup\": %d, \"ndn\": %d, \"N0\": %.12f, \"M0\": %.3e, \"ok\": %s}\n",
               mag ? "4ring" : "4ring_nomag", gs.eigenvalue(), gs.sector().nup(), gs.sector().ndown(), obs["N"][0],
               obs["M"][0], ok ? "true" : "false");
+  return ok ? 0 : 1;
+}
+
+static int storage_corners(const std::string& dir) {
+  params p;
+  define_parameters(p);
+  p["NSITES"] = 4;
+  p["NSPINS"] = 2;
+  p["INPUT_FILE"] = dir + "/4ring_input.json";
+  p["arpack.SECTOR"] = false;
+  p["storage.MAX_DIM"] = 36;
+  p["storage.EIGENVALUES_ONLY"] = 0;
+  p["arpack.NEV"] = 1;
+  HIPHubbardHamiltonian ham(p);
+  ham.diag(false);
+  auto& m = ham.model();
+  auto& st = ham.storage();
+  bool ok = true;
+  int ones = 0;
+  double worst = 0.0;
+  std::string pairs_json;
+  // one-state sectors: (0,0), (0,4), (4,0), (4,4); the state's bits are the
+  // up half over the down half, all set or all clear
+  for (const auto& pr : ham.eigenpairs()) {
+    const int a = pr.sector().nup(), b = pr.sector().ndown();
+    if (pr.sector().size() != 1) continue;
+    ++ones;
+    const long long s = ((long long)(a ? 0xF : 0) << 4) | (b ? 0xF : 0);
+    worst = std::max(worst, std::abs(pr.eigenvalue() - m.diagonal(s)));
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "%s[%d, %d, %.15g]", pairs_json.empty() ? "" : ", ", a, b, pr.eigenvalue());
+    pairs_json += buf;
+    ok = ok && pr.eigenvector().size() == 1 && pr.eigenvector()[0] == 1.0;
+  }
+  ok = ok && ones == 4 && worst < 1e-12;
+  // constant_shift on sector (2,2): av(v) + shift * v, discarded by the next fill
+  m.symmetry().set_sector(Symmetry::SzSymmetry::Sector(2, 2, 36));
+  st.fill();
+  std::vector<double> v(36), w0(36), w1(36), w2(36);
+  for (int i = 0; i < 36; ++i) v[i] = std::sin(1.0 + i);
+  st.av(v.data(), w0.data(), 36, true);
+  st.constant_shift(0.75);
+  st.av(v.data(), w1.data(), 36, true);
+  st.fill();
+  st.av(v.data(), w2.data(), 36, true);
+  double dshift = 0.0, dfill = 0.0;
+  for (int i = 0; i < 36; ++i) {
+    dshift = std::max(dshift, std::abs(w1[i] - w0[i] - 0.75 * v[i]));
+    dfill = std::max(dfill, std::abs(w2[i] - w0[i]));
+  }
+  ok = ok && dshift < 1e-13 && dfill == 0.0;
+  // the shift reaches a one-state sector's pair too
+  m.symmetry().set_sector(Symmetry::SzSymmetry::Sector(4, 4, 1));
+  st.fill();
+  st.constant_shift(0.5);
+  st.zero_eigenapair();
+  const double zshift = std::abs(st.eigenvalues()[0] - (m.diagonal(0xFF) + 0.5));
+  ok = ok && zshift < 1e-12;
+  std::printf("{\"case\": \"storage\", \"one_state_pairs\": [%s], \"max_dE\": %.3e, \"shift_err\": %.3e, "
+              "\"refill_err\": %.3e, \"zero_pair_shift_err\": %.3e, \"ok\": %s}\n",
+              pairs_json.c_str(), worst, dshift, dfill, zshift, ok ? "true" : "false");
   return ok ? 0 : 1;
 }
 
@@ -220,6 +286,7 @@ int main(int argc, char** argv) {
     if (c == "gfchi") return gfchi(dir);
     if (c == "4ring") return ring4(dir, true);
     if (c == "4ring_nomag") return ring4(dir, false);
+    if (c == "storage") return storage_corners(dir);
     if (c.rfind("dyson_", 0) == 0) return dyson(c.substr(6));
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

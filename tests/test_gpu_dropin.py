@@ -18,7 +18,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("case", ["gfchi", "4ring", "4ring_nomag", "dyson_hubbard0", "dyson_hubbard", "dyson_siam0"])
+@pytest.mark.parametrize("case", ["gfchi", "4ring", "4ring_nomag", "dyson_hubbard0", "dyson_hubbard", "dyson_siam0",
+                                  "storage"])
 def test_dropin_reference_tests(case):
     assert os.path.exists(BIN), "edtest driver not built (run __graft_entry__.build())"
     r = subprocess.run([BIN, case, GOLDEN], capture_output=True, text=True, timeout=120)
@@ -85,3 +86,22 @@ def test_hdf5_driver_gf_chi(tmp_path):
             q = ((w * norb + 0) * ns + s) * 2
             err = max(err, abs(complex(data[q], data[q + 1]) - complex(ref[w][1], ref[w][2])))
     assert err < 1e-10, err
+
+
+def test_dropin_one_state_pairs_match_oracle_diagonal():
+    """The storage corner case's one-state pairs (sectors (0,0), (0,4),
+    (4,0), (4,4) of the 4-site ring with field and chemical potential) equal
+    the oracle's HubbardModel diagonal of the single state (HubbardModel.h:
+    116-132, pinned by the 4-ring E0 and the brute-force tests)."""
+    from oracle import edoracle as eo
+    from tests.fixtures import hubbard_arrays, load_input
+    r = subprocess.run([BIN, "storage", GOLDEN], capture_output=True, text=True, timeout=120)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert r.returncode == 0 and res["ok"], res
+    Ns, t, U, xmu, H, J = hubbard_arrays(load_input("4ring_input.json"))
+    om = eo.hubbard(Ns, t, U, xmu, H, J)
+    got = {(a, b): e for a, b, e in res["one_state_pairs"]}
+    assert set(got) == {(0, 0), (0, 4), (4, 0), (4, 4)}
+    for (a, b), e in got.items():
+        st = ((0xF if a else 0) << Ns) | (0xF if b else 0)
+        assert abs(e - om.diagonal(st)) < 1e-12, ((a, b), e, om.diagonal(st))
