@@ -9,9 +9,10 @@
 //   UD[a][b] = sum_u bu_a D_u[b]            D_u[b] = sum_d w bd_b
 //   DD[a][b] = sum_d C_d bd_a bd_b          C_d = sum_u w      (column sums)
 // and N_eff = 1 / sum w^2.  The device makes one pass over the rows (R_u,
-// D_u, sum w^2 per row) and one over the columns (C_d, by row tiles), the
-// host forms UU, UD, DD from those D_up (N_s + 2) + D_dn numbers and
-// assembles the observables with the reference's orbital conventions.
+// D_u, sum w^2 per row) and one over the columns (C_d, by row tiles), then
+// forms UU, UD, DD and sum w^2 in two fixed-order reduction stages; the host
+// reads those 3 N_s^2 + 1 numbers and assembles the observables with the
+// reference's orbital conventions.
 #include <algorithm>
 #include <cstring>
 
@@ -24,6 +25,7 @@ namespace {
 constexpr int kObsBlock = 256;
 constexpr int kObsMaxSites = 32;
 constexpr int kColTile = 128;  // rows per column-sum tile
+constexpr int kMomBlocks = 128;  // first-stage blocks of the moment reduction
 
 // One block per row u (grid-stride): out[u * (Ns + 2) + b] = sum_d w [bit b
 // of bits_d] for b < Ns, then sum_d w, then sum_d w^2.
@@ -99,6 +101,54 @@ __global__ __launch_bounds__(kObsBlock) void obs_cols_sum_kernel(const double* _
   col[d] = s;
 }
 
+// First stage of the moments: block k takes rows [k R, (k + 1) R) and
+// columns [k C, (k + 1) C); thread t = a Ns + b (a, b < Ns) sums
+// UU[a][b], UD[a][b] over its rows and DD[a][b] over its columns; thread 0
+// also sum w^2.  part[k * (3 Ns^2 + 1) + ...] = (UU, UD, DD, sumsq).
+__global__ __launch_bounds__(1024) void obs_moments_kernel(const double* __restrict__ rows, const double* __restrict__ col,
+                                                           const int32_t* __restrict__ up_bits,
+                                                           const int32_t* __restrict__ dn_bits, int64_t Dup,
+                                                           int64_t Ddn, int Ns, double* __restrict__ part) {
+  const int t = threadIdx.x;
+  const int nout = Ns + 2, nm = 3 * Ns * Ns + 1;
+  const int k = blockIdx.x;
+  const int64_t R = (Dup + gridDim.x - 1) / gridDim.x, C = (Ddn + gridDim.x - 1) / gridDim.x;
+  double* o = part + (size_t)k * nm;
+  if (t < Ns * Ns) {
+    const int a = t / Ns, b = t - a * Ns;
+    double uu = 0.0, ud = 0.0, dd = 0.0;
+    for (int64_t u = k * R; u < min(Dup, (k + 1) * R); ++u) {
+      const unsigned bu = (unsigned)up_bits[u];
+      if (!((bu >> a) & 1u)) continue;
+      const double* r = rows + u * nout;
+      if ((bu >> b) & 1u) uu += r[Ns];
+      ud += r[b];
+    }
+    for (int64_t d = k * C; d < min(Ddn, (k + 1) * C); ++d) {
+      const unsigned bd = (unsigned)dn_bits[d];
+      if (((bd >> a) & 1u) && ((bd >> b) & 1u)) dd += col[d];
+    }
+    o[t] = uu;
+    o[Ns * Ns + t] = ud;
+    o[2 * Ns * Ns + t] = dd;
+  }
+  if (t == 0) {
+    double s = 0.0;
+    for (int64_t u = k * R; u < min(Dup, (k + 1) * R); ++u) s += rows[u * nout + Ns + 1];
+    o[3 * Ns * Ns] = s;
+  }
+}
+
+// Second stage: out[i] = sum over the first-stage blocks in order.
+__global__ __launch_bounds__(256) void obs_moments_sum_kernel(const double* __restrict__ part, int nblk, int nm,
+                                                              double* __restrict__ out) {
+  for (int i = threadIdx.x; i < nm; i += blockDim.x) {
+    double s = 0.0;
+    for (int k = 0; k < nblk; ++k) s += part[(size_t)k * nm + i];
+    out[i] = s;
+  }
+}
+
 }  // namespace
 
 void static_obs_run(Ctx& c, const double* dv, int norb, double* out) {
@@ -108,10 +158,11 @@ void static_obs_run(Ctx& c, const double* dv, int norb, double* out) {
 size_t static_obs_scratch(const SectorTables& st, int Ns) {
   const int64_t Dup = st.up.D, Ddn = st.dn.D;
   const int64_t ntile = (Dup + kColTile - 1) / kColTile;
-  return (size_t)Dup * (Ns + 2) + (size_t)ntile * Ddn + (size_t)Ddn;
+  const size_t nm = 3 * (size_t)Ns * Ns + 1;
+  return (size_t)Dup * (Ns + 2) + (size_t)ntile * Ddn + (size_t)Ddn + (size_t)kMomBlocks * nm + nm;
 }
 
-size_t static_obs_host(const SectorTables& st, int Ns) { return (size_t)st.up.D * (Ns + 2) + (size_t)st.dn.D; }
+size_t static_obs_host(const SectorTables&, int Ns) { return 3 * (size_t)Ns * Ns + 1; }
 
 void static_obs_launch(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, double* scratch,
                        hipStream_t stream, double* host) {
@@ -123,15 +174,20 @@ void static_obs_launch(const SectorTables& st, const DevSector& ds, int Ns, cons
   double* rows = scratch;
   double* part = rows + rows_n;
   double* col = part + part_n;
+  const int nm = 3 * Ns * Ns + 1;
+  double* mpart = col + Ddn;
+  double* mom = mpart + (size_t)kMomBlocks * nm;
   const int grid_rows = (int)std::min<int64_t>(Dup, 4096);
   hipLaunchKernelGGL(obs_rows_kernel, dim3(grid_rows), dim3(kObsBlock), 0, stream, dv, Dup, Ddn, ds.dn.bits, Ns, rows);
   hipLaunchKernelGGL(obs_cols_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock), (unsigned)ntile),
                      dim3(kObsBlock), 0, stream, dv, Dup, Ddn, part);
   hipLaunchKernelGGL(obs_cols_sum_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0,
                      stream, part, ntile, Ddn, col);
+  hipLaunchKernelGGL(obs_moments_kernel, dim3(kMomBlocks), dim3((unsigned)std::max(64, (Ns * Ns + 63) / 64 * 64)), 0,
+                     stream, rows, col, ds.up.bits, ds.dn.bits, Dup, Ddn, Ns, mpart);
+  hipLaunchKernelGGL(obs_moments_sum_kernel, dim3(1), dim3(256), 0, stream, mpart, kMomBlocks, nm, mom);
   EDH_HIP(hipGetLastError());
-  EDH_HIP(hipMemcpyAsync(host, rows, rows_n * sizeof(double), hipMemcpyDeviceToHost, stream));
-  EDH_HIP(hipMemcpyAsync(host + rows_n, col, (size_t)Ddn * sizeof(double), hipMemcpyDeviceToHost, stream));
+  EDH_HIP(hipMemcpyAsync(host, mom, (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, stream));
 }
 
 void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, const double* dv, int norb,
@@ -146,35 +202,12 @@ void static_obs_sector(const SectorTables& st, const DevSector& ds, int Ns, cons
 
 void static_obs_reduce(const SectorTables& st, int Ns, int norb, const double* host, double* out) {
   EDH_CHECK(norb >= 1 && norb <= Ns, EDH_ERR_ARG, "static observables: interacting orbitals out of range");
-  const int64_t Dup = st.up.D, Ddn = st.dn.D;
-  const int nout = Ns + 2;
-  const double* hr = host;
-  const double* hc = host + (size_t)Dup * nout;
   // second moments of the occupation bits (bit b of a per-spin pattern is
-  // site Ns - 1 - b, FermionicModel.h:35-68)
-  std::vector<double> UU((size_t)Ns * Ns, 0.0), UD((size_t)Ns * Ns, 0.0), DD((size_t)Ns * Ns, 0.0);
-  double sumsq = 0.0;
-  for (int64_t u = 0; u < Dup; ++u) {
-    const unsigned bu = (unsigned)st.up.bits[u];
-    const double* r = hr + u * nout;
-    const double R = r[Ns];
-    sumsq += r[Ns + 1];
-    for (int a = 0; a < Ns; ++a) {
-      if (!((bu >> a) & 1u)) continue;
-      for (int b = 0; b < Ns; ++b) {
-        if ((bu >> b) & 1u) UU[(size_t)a * Ns + b] += R;
-        UD[(size_t)a * Ns + b] += r[b];
-      }
-    }
-  }
-  for (int64_t d = 0; d < Ddn; ++d) {
-    const unsigned bd = (unsigned)st.dn.bits[d];
-    for (int a = 0; a < Ns; ++a) {
-      if (!((bd >> a) & 1u)) continue;
-      for (int b = 0; b < Ns; ++b)
-        if ((bd >> b) & 1u) DD[(size_t)a * Ns + b] += hc[d];
-    }
-  }
+  // site Ns - 1 - b, FermionicModel.h:35-68), from the device
+  const double* UU = host;
+  const double* UD = host + (size_t)Ns * Ns;
+  const double* DD = host + 2 * (size_t)Ns * Ns;
+  const double sumsq = host[3 * (size_t)Ns * Ns];
   // orbital o < Ns is the up spin of site o, o >= Ns the down spin of site
   // o - Ns; E[x y] for two spin-orbitals from the three blocks
   auto moment = [&](int o1, int o2) -> double {
