@@ -25,13 +25,16 @@ namespace {
 constexpr int kObsBlock = 256;
 constexpr int kObsMaxSites = 32;
 constexpr int kColTile = 128;  // rows per column-sum tile
-constexpr int kMomBlocks = 128;  // first-stage blocks of the moment reduction
+constexpr int kMomBlocks = 512;  // first-stage blocks of the moment reduction
 
 // One block per row u (grid-stride): out[u * (Ns + 2) + b] = sum_d w [bit b
-// of bits_d] for b < Ns, then sum_d w, then sum_d w^2.
+// of bits_d] for b < Ns, then sum_d w, then sum_d w^2.  MS >= Ns bit slots
+// per element (16 for up to 16 sites: half the bit tests of 32).
+template <int MS>
 __global__ __launch_bounds__(kObsBlock) void obs_rows_kernel(const double* __restrict__ v, int64_t Dup, int64_t Ddn,
                                                              const int32_t* __restrict__ dn_bits, int Ns,
                                                              double* __restrict__ out) {
+  constexpr int kObsMaxSites = MS;
   __shared__ double s_part[kObsBlock / 64][kObsMaxSites + 2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int64_t u = blockIdx.x; u < Dup; u += gridDim.x) {
@@ -139,14 +142,19 @@ __global__ __launch_bounds__(1024) void obs_moments_kernel(const double* __restr
   }
 }
 
-// Second stage: out[i] = sum over the first-stage blocks in order.
+// Second stage, one block per moment i: thread t sums the first-stage
+// blocks t, t + 256, ... in order, then a fixed shuffle tree and the four
+// wavefronts in order give out[i].
 __global__ __launch_bounds__(256) void obs_moments_sum_kernel(const double* __restrict__ part, int nblk, int nm,
                                                               double* __restrict__ out) {
-  for (int i = threadIdx.x; i < nm; i += blockDim.x) {
-    double s = 0.0;
-    for (int k = 0; k < nblk; ++k) s += part[(size_t)k * nm + i];
-    out[i] = s;
-  }
+  __shared__ double s_w[4];
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 256) s += part[(size_t)k * nm + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
 }
 
 }  // namespace
@@ -178,14 +186,19 @@ void static_obs_launch(const SectorTables& st, const DevSector& ds, int Ns, cons
   double* mpart = col + Ddn;
   double* mom = mpart + (size_t)kMomBlocks * nm;
   const int grid_rows = (int)std::min<int64_t>(Dup, 4096);
-  hipLaunchKernelGGL(obs_rows_kernel, dim3(grid_rows), dim3(kObsBlock), 0, stream, dv, Dup, Ddn, ds.dn.bits, Ns, rows);
+  if (Ns <= 16)
+    hipLaunchKernelGGL(obs_rows_kernel<16>, dim3(grid_rows), dim3(kObsBlock), 0, stream, dv, Dup, Ddn, ds.dn.bits, Ns,
+                       rows);
+  else
+    hipLaunchKernelGGL(obs_rows_kernel<kObsMaxSites>, dim3(grid_rows), dim3(kObsBlock), 0, stream, dv, Dup, Ddn,
+                       ds.dn.bits, Ns, rows);
   hipLaunchKernelGGL(obs_cols_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock), (unsigned)ntile),
                      dim3(kObsBlock), 0, stream, dv, Dup, Ddn, part);
   hipLaunchKernelGGL(obs_cols_sum_kernel, dim3((unsigned)((Ddn + kObsBlock - 1) / kObsBlock)), dim3(kObsBlock), 0,
                      stream, part, ntile, Ddn, col);
   hipLaunchKernelGGL(obs_moments_kernel, dim3(kMomBlocks), dim3((unsigned)std::max(64, (Ns * Ns + 63) / 64 * 64)), 0,
                      stream, rows, col, ds.up.bits, ds.dn.bits, Dup, Ddn, Ns, mpart);
-  hipLaunchKernelGGL(obs_moments_sum_kernel, dim3(1), dim3(256), 0, stream, mpart, kMomBlocks, nm, mom);
+  hipLaunchKernelGGL(obs_moments_sum_kernel, dim3(nm), dim3(256), 0, stream, mpart, kMomBlocks, nm, mom);
   EDH_HIP(hipGetLastError());
   EDH_HIP(hipMemcpyAsync(host, mom, (size_t)nm * sizeof(double), hipMemcpyDeviceToHost, stream));
 }
