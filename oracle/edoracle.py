@@ -84,6 +84,8 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(ctypes.c_int8)]
         L.orc_lanczos.restype = ctypes.c_int
         L.orc_lanczos.argtypes = [vp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_lanczos_omp.restype = ctypes.c_int
+        L.orc_lanczos_omp.argtypes = [vp, _dp, ctypes.c_int, _dp, _dp]
         L.orc_frac_points.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_double, _dp, _dp, _dp, _dp]
         L.orc_eigenpair_order.argtypes = [ctypes.c_int, _dp, _ip]
@@ -247,12 +249,14 @@ class SRS:
     def int_start(self):
         return int(lib().orc_srs_int_start(self._h))
 
-    def lanczos(self, v, nlanc_max):
-        """Lanczos::lanczos (Lanczos.h:36-77): returns (nlanc, alfa, beta)."""
+    def lanczos(self, v, nlanc_max, omp=False):
+        """Lanczos::lanczos (Lanczos.h:36-77): returns (nlanc, alfa, beta)
+        (omp: the OpenMP av and vector loops, for large sectors)."""
         v = np.array(v, dtype=np.float64, copy=True)
         a = np.zeros(nlanc_max)
         b = np.zeros(nlanc_max + 1)
-        nl = lib().orc_lanczos(self._h, _d(v), nlanc_max, _d(a), _d(b))
+        fn = lib().orc_lanczos_omp if omp else lib().orc_lanczos
+        nl = fn(self._h, _d(v), nlanc_max, _d(a), _d(b))
         return nl, a, b
 
     def dense(self):

@@ -767,6 +767,45 @@ void orc_occupations(void* mp, int nup, int ndn, int norb, int8_t* nu, int8_t* n
 // Lanczos::lanczos recurrence (Lanczos.h:36-77) with SRS::av in accumulate
 // mode and the serial SRS::vv dot (SpinResolvedStorage.h:390-400).
 // v is overwritten (as in the reference).  Returns nlanc.
+// Lanczos::lanczos (Lanczos.h:36-77) on the OpenMP av, the vector loops and
+// reductions split over threads (same recurrence as orc_lanczos; the sums
+// differ from the serial ones by rounding only).
+int orc_lanczos_omp(void* sp, double* v, int Nl, double* alfa, double* beta) {
+  SRS* s = (SRS*)sp;
+  int64_t size = s->N;
+  std::vector<double> w(size, 0.0);
+  double bet = 0;
+  int nlanc = 0;
+  for (int q = 0; q < Nl; ++q) alfa[q] = 0.0;
+  for (int q = 0; q <= Nl; ++q) beta[q] = 0.0;
+  for (int iter = 1; iter <= Nl; ++iter) {
+    ++nlanc;
+    if (iter != 1) {
+#pragma omp parallel for schedule(static)
+      for (int64_t j = 0; j < size; ++j) {
+        double d = v[j];
+        v[j] = w[j] / bet;
+        w[j] = -bet * d;
+      }
+    }
+    srs_av_omp(s, v, w.data(), false);
+    double alf = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : alf)
+    for (int64_t k = 0; k < size; ++k) alf += w[k] * v[k];
+    alfa[iter - 1] = alf;
+    double b2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : b2)
+    for (int64_t j = 0; j < size; ++j) {
+      w[j] -= alf * v[j];
+      b2 += w[j] * w[j];
+    }
+    bet = std::sqrt(b2);
+    if (iter != Nl) beta[iter] = bet;
+    if (std::abs(bet) < 1e-10) break;
+  }
+  return nlanc;
+}
+
 int orc_lanczos(void* sp, double* v, int Nl, double* alfa, double* beta) {
   SRS* s = (SRS*)sp;
   int64_t size = s->N;

@@ -376,3 +376,42 @@ def test_av_parity_16site_long_rows(ctx, nup):
         ctx.hv_variant = var
         assert _rel(ctx.av(v, w0, clear=False), ref_acc) < 1e-13, var
         assert _rel(ctx.av(v), ref) < 1e-13, var
+
+
+def test_16site_gf_start_vector_and_lanczos_against_oracle():
+    """A Green's-function run of the 16-site 4x4 Hubbard sector (8,8) at
+    its stated size: the ground state (device eigensolver, NEV = 1,
+    NCV = 20, tol 1e-10), its start vector c+_{0 up}|psi> in sector (9,8)
+    (147,232,800 states) bit-exact against the oracle's
+    SpinResolvedStorage::a_adag restatement, its weight <n> = 1/2 (half
+    filling of a bipartite lattice at mu = U/2: particle-hole symmetry), and
+    the first 8 Lanczos coefficients against the oracle's Lanczos::lanczos
+    (OpenMP form) from the same normalised vector (1e-10)."""
+    Ns = 16
+    t = models.torus_hopping(4, 4)
+    U, mu = np.full(Ns, 6.0), np.full(Ns, 3.0)
+    om = eo.hubbard(Ns, t, U, mu)
+    ctx = ea.Context(0)
+    try:
+        ctx.set_model(ea.HubbardModel(Ns, t, U, mu))
+        ctx.set_sector(8, 8)
+        e, vecs, _ = ctx.eigsh(1, 20, 1e-10, 1000, 20261202, vectors=True)
+        psi = vecs[:, 0].copy()
+        del vecs
+        nxt, out = ctx.adag((8, 8), 0, False, psi)
+        assert nxt == (9, 8) and out.size == 11440 * 12870
+        nxt_o, out_o = eo.a_adag(om, (8, 8), 0, False, psi)
+        assert nxt_o == nxt and np.array_equal(out, out_o)
+        del out_o, psi
+        norm2 = float(np.dot(out, out))
+        assert abs(norm2 - 0.5) < 1e-8, norm2
+        out /= np.sqrt(norm2)
+        ctx.set_sector(*nxt)
+        nl, a, b = ctx.lanczos(out, 8)
+        srs = eo.SRS(om, *nxt)
+        nl_o, a_o, b_o = srs.lanczos(out, 8, omp=True)
+        assert nl == nl_o == 8
+        assert np.abs(a - a_o).max() < 1e-10 * max(1.0, np.abs(a_o).max())
+        assert np.abs(b[1:8] - b_o[1:8]).max() < 1e-10 * max(1.0, np.abs(b_o).max())
+    finally:
+        ctx.close()

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import edoracle as eo
+from tests import models
 from tests.fixtures import hubbard_arrays, load_dat, load_input
 
 GFCHI_PARAMS = {"NSITES": 4, "arpack.NEV": 100, "lanc.BETA": 20.0, "lanc.NOMEGA": 1000, "lanc.NLANC": 100,
@@ -188,3 +189,17 @@ def test_static_observables_follow_reference_orbital_rule(kind):
         nu, nd = eo.occupations(m, sector, m.Ns)
         mi = (nu[:, :2] - nd[:, :2]).astype(float)
         assert not np.allclose(obs["M_i M_j"], np.einsum("s,si,sj->ij", v * v, mi, mi).reshape(-1))
+
+
+def test_openmp_lanczos_matches_serial():
+    """The oracle's OpenMP Lanczos (for sectors too large for the serial
+    loop) runs the serial recurrence of Lanczos.h:36-77, which the gom1.dat
+    golden pins: coefficients equal to rounding on a random Hubbard model."""
+    p = models.random_hubbard_params(8, np.random.default_rng(1))
+    srs = eo.SRS(models.oracle_model("hubbard", p), 4, 4)
+    v = np.random.default_rng(2).uniform(-1, 1, srs.n)
+    v /= np.linalg.norm(v)
+    n1, a1, b1 = srs.lanczos(v, 30)
+    n2, a2, b2 = srs.lanczos(v, 30, omp=True)
+    assert n1 == n2 == 30
+    assert np.abs(a1 - a2).max() < 1e-12 and np.abs(b1 - b2).max() < 1e-12
