@@ -100,6 +100,7 @@ _SIGS = {
     "edh_batch_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
     "edh_batch_sweeps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "edh_batch_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+    "edh_batch_time_hv": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_uint64, _dp, _i64p, _i64p]),
     "edh_batch_cf_many": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _ip, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]),
     "edh_batch_cf_many_z": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _ip, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                            _dp]),

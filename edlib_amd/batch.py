@@ -210,6 +210,17 @@ class Batch:
                                             ctypes.byref(blk)))
         return blk.value
 
+    def time_hv(self, sids, reps=20, seed=1):
+        """Grouped H*v of an eigensolver sweep over one operand per sector
+        (w = H v, seeded uniform v): (mean ms per launch, algorithmic bytes
+        per launch = sum of 16 n + table bytes, elements)."""
+        ids = np.ascontiguousarray(sids, dtype=np.int32)
+        ms = ctypes.c_double()
+        nb, ne = ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.lib().edh_batch_time_hv(self._h, len(ids), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           reps, seed, ctypes.byref(ms), ctypes.byref(nb), ctypes.byref(ne)))
+        return ms.value, nb.value, ne.value
+
     def cf_many(self, job_ids, ev, isign, expb, excited, wn):
         """Continued fractions of finished Lanczos runs at z = p + excited*isign
         for each frequency point p: real `wn` are Matsubara frequencies

@@ -1072,6 +1072,16 @@ int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block) {
   API_END
 }
 
+int edh_batch_time_hv(edh_batch* b, int nsec, const int* sec_ids, int reps, uint64_t seed, double* avg_ms,
+                      int64_t* bytes, int64_t* elements) {
+  API_BEGIN
+  edh::Batch& B = use_batch(b);
+  EDH_CHECK(nsec >= 0 && (nsec == 0 || sec_ids) && avg_ms && bytes && elements, EDH_ERR_ARG, "bad argument");
+  std::vector<int> ids(sec_ids, sec_ids + nsec);
+  B.time_hv(ids, reps, seed, avg_ms, bytes, elements);
+  API_END
+}
+
 namespace {
 void batch_cf_many(edh_batch* b, int njobs, const int* job_ids, const double* ev, const int* isign, const double* expb,
                    const double* excited, int nz, const double* wr, const double* wn, double* out_re, double* out_im) {

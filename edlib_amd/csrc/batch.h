@@ -188,6 +188,8 @@ struct Batch {
   void free_job(int id);
   int step();
   int autotune(const std::vector<int>& sec_ids);
+  void time_hv(const std::vector<int>& sec_ids, int reps, uint64_t seed, double* avg_ms, int64_t* bytes,
+               int64_t* elements);
 
  private:
   double* dalloc(BJob& j, size_t count);

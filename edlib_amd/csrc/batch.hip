@@ -1864,4 +1864,79 @@ int Batch::autotune(const std::vector<int>& sec_ids) {
   return choice;
 }
 
+// Times the grouped H*v of an eigensolver sweep, w = H v over one operand
+// per listed sector (seeded uniform v in [-1, 1)), as the steady sweeps run
+// it (replayed from a captured graph when graphs are on): mean milliseconds
+// per launch over `reps` launches between two HIP events on the batch's
+// stream, the launch's algorithmic bytes (16 n + the per-spin table bytes,
+// summed over the sectors: read v, write w, read the tables once) and its
+// element count.
+void Batch::time_hv(const std::vector<int>& sec_ids, int reps, uint64_t seed, double* avg_ms, int64_t* bytes,
+                    int64_t* elements) {
+  EDH_CHECK(reps >= 1, EDH_ERR_ARG, "batch: reps must be >= 1");
+  std::vector<int> ids;
+  int64_t tot = 0, bsum = 0;
+  for (int id : sec_ids) {
+    EDH_CHECK(id >= 0 && id < (int)secs.size() && secs[id], EDH_ERR_ARG, "batch: no such sector");
+    const BSector& s = *secs[id];
+    if (s.tabs.N == 0) continue;
+    ids.push_back(id);
+    tot += s.tabs.N;
+    bsum += 16 * s.tabs.N + s.tabs.table_bytes;
+  }
+  *avg_ms = 0.0;
+  *bytes = bsum;
+  *elements = tot;
+  if (ids.empty()) return;
+  double* buf = nullptr;
+  EDH_HIP(hipMalloc(&buf, (size_t)2 * tot * sizeof(double)));
+  std::vector<std::pair<int, HvArgs>> hv;
+  int64_t off = 0;
+  for (int id : ids) {
+    HvArgs a{};
+    a.v = buf + off;
+    a.w = buf + tot + off;
+    a.wprev = a.w;
+    a.vscale = 1.0;
+    a.mode = 0;
+    hv.push_back({id, a});
+    off += secs[id]->tabs.N;
+  }
+  dev_random_uniform(buf, tot, seed, stream);
+  std::vector<OpGroup> none;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  auto release = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const hipError_t es = hipStreamSynchronize(stream);
+    (void)hipFree(buf);
+    return es;
+  };
+  hipError_t err = hipSuccess;
+  try {
+    err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err == hipSuccess) {
+      issue(none, &hv, none);  // upload
+      issue(none, &hv, none);  // the same sweep again: captured when graphs are on
+      err = hipEventRecord(e0, stream);
+      for (int r = 0; r < reps && err == hipSuccess; ++r) {
+        issue(none, &hv, none);
+        err = hipGetLastError();
+      }
+      if (err == hipSuccess) err = hipEventRecord(e1, stream);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    }
+  } catch (...) {
+    (void)release();
+    throw;
+  }
+  const hipError_t es = release();
+  EDH_HIP(err);
+  EDH_HIP(es);
+  *avg_ms = ms / reps;
+}
+
 }  // namespace edh

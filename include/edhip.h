@@ -332,6 +332,15 @@ int edh_batch_sweeps(edh_batch* b, int64_t* sweeps, int64_t* graph_sweeps);
 /* Autotuner: times the grouped H*v over the given sectors for each block
  * size and keeps the fastest (returned in *block). */
 int edh_batch_autotune(edh_batch* b, int nsec, const int* sec_ids, int* block);
+/* Measurement: mean time of the grouped H*v launch of an eigensolver sweep,
+ * w = H v over one operand per listed sector (seeded uniform v), over `reps`
+ * launches timed by HIP events on the batch's stream (graph-replayed when
+ * graphs are on); *bytes = its algorithmic bytes, the sum over the sectors
+ * of 16 n + the per-spin table bytes; *elements = the sum of n.  No
+ * counterpart in the reference (its sector loop applies av() one sector at a
+ * time, Hamiltonian.h:54-101). */
+int edh_batch_time_hv(edh_batch* b, int nsec, const int* sec_ids, int reps, uint64_t seed, double* avg_ms,
+                      int64_t* bytes, int64_t* elements);
 /* Continued fractions of many finished Lanczos runs in one launch
  * (Lanczos::compute_continued_fraction, Lanczos.h:82-100, for each): run j
  * at z = excited[j]*isign[j] + i*wn[q], q < nz, weight ev[j]*expb[j];

@@ -89,6 +89,54 @@ def test_batch_lanczos_matches_single_sector():
     b.close()
 
 
+def test_grouped_hv_timing_reports_bytes_and_leaves_batch_usable():
+    """edh_batch_time_hv: elements = sum of n, algorithmic bytes = sum of
+    16 n + table bytes (the single-sector context's figure for each sector),
+    a positive time; the batch's later sweeps (whose descriptors differ from
+    the timed launch's captured graph) still give the single-sector Lanczos
+    coefficients."""
+    import edlib_amd as ea
+    from edlib_amd.batch import Batch
+    from models import ring_hopping
+    Ns = 8
+    m = ea.HubbardModel(Ns, ring_hopping(Ns), np.full(Ns, 4.0), np.full(Ns, 2.0))
+    ctx = ea.Context(0)
+    ctx.set_model(m)
+    b = Batch(0)
+    secs = [(4, 4), (3, 5), (2, 2), (0, 0)]
+    sids, want_b, want_n = [], 0, 0
+    for sec in secs:
+        n = ctx.set_sector(*sec)
+        _, _, tb = ctx.sector_info()
+        sid, _ = b.add_sector(m, *sec)
+        sids.append(sid)
+        want_n += n
+        want_b += 16 * n + (tb if n else 0)
+    for graphs in (True, False):
+        b.set_graphs(graphs)
+        ms, nbytes, nel = b.time_hv(sids, 5, SEED)
+        assert nel == want_n and nbytes == want_b
+        assert ms > 0.0 and math.isfinite(ms)
+    b.set_graphs(True)
+    runs = []
+    for sec, sid in zip(secs[:3], sids[:3]):
+        n = ctx.set_sector(*sec)
+        v0 = np.random.default_rng(SEED + 2 + sec[0]).uniform(-1, 1, n)
+        v0 /= np.linalg.norm(v0)
+        runs.append((sec, v0, b.submit_lanczos(sid, v0, 30)))
+    b.run()
+    for sec, v0, jid in runs:
+        ctx.set_sector(*sec)
+        nl_ref, a_ref, b_ref = ctx.lanczos(v0, 30)
+        nl, a, bb = b.lanczos_result(jid, 30)
+        assert nl == nl_ref
+        k = min(nl, 20)
+        np.testing.assert_allclose(a[:k], a_ref[:k], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(bb[1:k], b_ref[1:k], rtol=1e-10, atol=1e-12)
+    ctx.close()
+    b.close()
+
+
 def test_large_operands_on_rows_kernel_match_single_sector():
     """Operands with long rows (D_dn > 4608, N >= 2^22) run on the
     single-sector rows kernel inside a batch sweep: eigenpairs and Lanczos
