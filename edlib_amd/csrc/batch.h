@@ -41,9 +41,12 @@ struct BJob {
   bool pending_start_norm = false;
   double *V = nullptr, *Valt = nullptr, *evecs = nullptr, *ctl = nullptr;
   std::vector<double> evals;
-  // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w; step = next step
+  // Lanczos run: lz[0] = v_k, lz[1] = v_{k-1}, lz[2] = w; step = next step;
+  // lzb = the three buffers in their order at step 1 (the device resolves
+  // the rotation from its own step counter)
   int nl = 0, step = 0, nlanc = 0;
   double* lz[3] = {nullptr, nullptr, nullptr};
+  double* lzb[3] = {nullptr, nullptr, nullptr};
   double *d_alpha = nullptr, *d_beta = nullptr, *tdot = nullptr;
   int64_t ntdot = 0;
   int* d_flags = nullptr;  // [0] stopped (|beta| < 1e-10), [1] steps taken
@@ -68,6 +71,11 @@ struct BOpHost {
   double* hist_b = nullptr;  // Lanczos beta_k output or null
   int* flags = nullptr;      // Lanczos stop flags or null
   int kstep = 0;
+  // Lanczos step resolved on the device: the run's step counter k; X, X1, w
+  // are then its three buffers at step 1, hist_a / hist_b the alpha and
+  // beta histories and m the run's length (kstep, the histories' entries
+  // and the rotation follow k)
+  const double* lzk = nullptr;
   int mode = 0;  // OP_* bits
   double scale = 1.0;
   double* nrm_out = nullptr;  // extra copy of the norm, or null
@@ -153,9 +161,10 @@ struct Batch {
 
   size_t bytes_in_use = 0;
   int hv_block = 256;
-  // Steady sweeps (the same descriptors as the sweep before: eigensolver
-  // runs only, no process started or finished) replay one captured graph
-  // of the sweep's kernels and skip the descriptor upload.
+  // Steady sweeps (the same descriptors as the sweep before: no process
+  // started or finished, no Lanczos run at its last step, no operand on the
+  // rows kernel) replay one captured graph of the sweep's kernels and skip
+  // the descriptor upload.
   bool use_graphs = true;
   std::vector<char> last_desc;     // descriptor bytes of the last sweep
   std::vector<int64_t> last_cfg;   // and its launch configuration

@@ -52,6 +52,7 @@ constexpr int S_NB = kMaxBasis + 3;      // norm before the first projection pas
 constexpr int S_GATE = kMaxBasis + 4;    // 1 when the second pass is needed
 constexpr int S_LZV = kMaxBasis + 5;     // } Lanczos run: scale of the stored current vector (1 / beta_{k-1})
 constexpr int S_LZW = kMaxBasis + 6;     // }   and of the previous one in w = H v_k - beta_{k-1} v_{k-1}
+constexpr int S_LZK = kMaxBasis + 7;     // Lanczos run: its next step k (1-based)
 constexpr int S_T = 2 * kMaxBasis;       // projected-matrix columns / Lanczos history
 
 // Eigensolver control block (after the projected-matrix block of a run's
@@ -93,6 +94,7 @@ struct BOp {
   int m;              // basis size
   const double* tdot;  // Lanczos step: per-task shares of <v, w>
   int64_t ntdot;
+  const double* lzk;  // Lanczos step counter (operands in rotation) or null
 };
 
 // An eigensolver operation follows its run's control block: the operand is
@@ -108,6 +110,20 @@ __device__ __forceinline__ BOp resolve(const BOp& in) {
     o.cnt = (int)col + 1;
     o.w = const_cast<double*>(X) + (col + 1) * o.ldx;
     if (o.hsum) o.hsum = o.sc + S_T + col * (o.m + 1);
+  } else if (o.lzk) {
+    // a Lanczos step from the run's own counter: the current vector and
+    // the H*v output in their rotation, the k-th history entries
+    // (Lanczos.h:47-64; beta_k is not stored at the last step)
+    const int k = (int)*o.lzk;
+    const int r0 = lz_buffer(k, 0), r2 = lz_buffer(k, 2);
+    const double* b0 = o.X;
+    const double* b1 = o.X1;
+    const double* b2 = o.w;
+    o.X = r0 == 0 ? b0 : (r0 == 1 ? b1 : b2);
+    o.w = const_cast<double*>(r2 == 0 ? b0 : (r2 == 1 ? b1 : b2));
+    o.hist_a = o.hist_a + (k - 1);
+    o.hist_b = k != o.m ? o.hist_b + k : nullptr;
+    o.kstep = k;
   }
   return o;
 }
@@ -329,6 +345,7 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
         if (o.hist_b) *o.hist_b = nrm;
         o.flags[1] = o.kstep;
         if (fabs(nrm) < 1e-10) o.flags[0] = 1;
+        if (o.lzk) o.sc[S_LZK] = o.kstep + 1;  // the step is complete (the last read of k in this sweep)
       }
       if (o.mode & OP_LZS) {
         // next step: v_{k+1} = w / beta_k stays stored as w, and the term
@@ -421,7 +438,7 @@ __global__ __launch_bounds__(kOpBlock) void bscale_kernel(const BOp* __restrict_
 // stored (unnormalised) current vector is alpha times its scale.
 __global__ __launch_bounds__(256) void blz_alpha_kernel(const BOp* __restrict__ ops) {
   __shared__ double s_red[8];
-  const BOp o = ops[blockIdx.x];
+  const BOp o = resolve(ops[blockIdx.x]);
   if (op_skip(o)) return;
   double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
   int64_t q = threadIdx.x;
@@ -1171,11 +1188,12 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
   j.ntdot = hv_batch_routed(s.tabs.up.D, s.tabs.dn.D) ? hv_batch_rows_partials(s.tabs.up.D)
                                                        : hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
   double* base = dalloc(j, 3 * nv + al(nsc) + (size_t)j.ntdot);
-  for (int q = 0; q < 3; ++q) j.lz[q] = base + q * nv;
+  for (int q = 0; q < 3; ++q) j.lz[q] = j.lzb[q] = base + q * nv;
   j.sc = base + 3 * nv;
   j.tdot = j.sc + al(nsc);
   fill_dev(j.sc, nsc);
   dev_fill(j.sc + S_LZV, 1, 1.0, stream);  // a given start vector is normalised
+  dev_fill(j.sc + S_LZK, 1, 1.0, stream);  // the first step
   j.d_alpha = j.sc + S_T;
   j.d_beta = j.d_alpha + nl;
   j.d_flags = (int*)(j.d_beta + nl + 1);
@@ -1440,6 +1458,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
         o.m = h.m;
         o.tdot = h.tdot;
         o.ntdot = h.ntdot;
+        o.lzk = h.lzk;
         std::memcpy(hb + p.ops_off + i * sizeof(BOp), &o, sizeof(BOp));
         for (int c = 0; c < o.nch; ++c) cj[c0 + c] = (int)i;
         c0 += o.nch;
@@ -1715,11 +1734,22 @@ int Batch::step() {
   }
   for (BJob* j : lz) {
     HvArgs a{};
-    a.v = j->lz[0];
-    a.w = j->lz[2];
-    a.wprev = j->lz[1];
+    const DevSector& ds = sector(j->sec).dev;
+    if (hv_batch_routed(ds.up.D, ds.dn.D)) {
+      // the rows kernel takes this step's buffers
+      a.v = j->lz[0];
+      a.w = j->lz[2];
+      a.wprev = j->lz[1];
+      a.mode = j->step > 1 ? 1 : 0;
+    } else {
+      // the grouped kernel resolves the rotation from the run's step
+      // counter, so the descriptor is the same at every step
+      a.v = j->lzb[0];
+      a.wprev = j->lzb[1];
+      a.w = j->lzb[2];
+      a.lzk = j->sc + S_LZK;
+    }
     a.vscale = 1.0;
-    a.mode = j->step > 1 ? 1 : 0;
     // {1 / beta_{k-1}, -beta_{k-1} / beta_{k-2}}; at the first step {1 / |v_1|, 0}
     a.dscale = j->sc + S_LZV;
     a.done = j->d_flags;
@@ -1757,17 +1787,22 @@ int Batch::step() {
     }
   }
   for (BJob* j : lz) {
+    // the step resolved on the device from the run's counter (the same
+    // descriptor at every step; the host's step count equals that counter
+    // while the run is live)
     BOpHost o;
-    o.X = j->lz[0];
+    o.X = j->lzb[0];
+    o.X1 = j->lzb[1];
+    o.w = j->lzb[2];
+    o.lzk = j->sc + S_LZK;
+    o.m = j->nl;
     o.ldx = j->n;
     o.cnt = 1;
-    o.w = j->lz[2];
     o.n = j->n;
     o.sc = j->sc;
-    o.hist_a = j->d_alpha + (j->step - 1);
-    o.hist_b = j->step != j->nl ? j->d_beta + j->step : nullptr;  // Lanczos.h:64
+    o.hist_a = j->d_alpha;
+    o.hist_b = j->d_beta;  // Lanczos.h:64: not stored at the last step
     o.flags = j->d_flags;
-    o.kstep = j->step;
     o.mode = OP_LZ | OP_LZS;
     o.tdot = j->tdot;
     o.ntdot = j->ntdot;

@@ -248,7 +248,17 @@ struct HvArgs {
   // optional (grouped H*v only): per-task shares of <v, w> (v scaled), at
   // least hv_batch_tasks(D_up, D_dn) of them
   double* tdot = nullptr;
+  // optional (grouped H*v only): a Lanczos run's step counter k (a double on
+  // the device); v, wprev and w are then given as the run's three buffers
+  // b0, b1, b2 and the launch takes v = b[r], wprev = b[(r + 1) % 3],
+  // w = b[(r + 2) % 3] with r = (3 - (k - 1) % 3) % 3, accumulating wprev
+  // from the second step on (the buffer rotation of Lanczos.h:47-51)
+  const double* lzk = nullptr;
 };
+
+// Buffer of a Lanczos run's three in rotation at step k >= 1: role 0 = the
+// current vector, 1 = the previous one, 2 = the H*v output.
+__host__ __device__ inline int lz_buffer(int k, int role) { return ((3 - (k - 1) % 3) % 3 + role) % 3; }
 
 // Control block of a batched eigensolver run (doubles, on the device): the
 // basis column being extended, which of the two basis buffers holds the

@@ -576,6 +576,7 @@ struct HvBatchJob {
   const double* ctl;    // eigensolver control block or null
   const double* v_alt;  // second basis buffer (with ctl)
   double* tdot;         // per-task shares of <v, w>, or null
+  const double* lzk;    // Lanczos run's step counter (buffers in rotation) or null
 };
 
 // One element of the grouped H*v: w[u][d] (written to W) from the operand
@@ -585,8 +586,9 @@ struct HvBatchJob {
 // order) is the same either way, so the result does not depend on it.
 template <bool STAGED>
 __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const HvParams& p, const double* __restrict__ V,
-                                                   double* __restrict__ W, int u, int d, double vscale, double wscale,
-                                                   int na, const double* R) {
+                                                   double* __restrict__ W, const double* __restrict__ Wprev,
+                                                   bool accum, int u, int d, double vscale, double wscale, int na,
+                                                   const double* R) {
   const int Ddn = p.Ddn;
   const double* __restrict__ vals = p.values;
   const int cross = J.cross;
@@ -601,7 +603,7 @@ __device__ __forceinline__ double hv_batch_element(const HvBatchJob& J, const Hv
   else if (cross == CROSS_IMP) x = cross_term<CROSS_IMP>(p, u, bu, bd, na);
   else if (cross == CROSS_GENERAL) x = cross_term<CROSS_GENERAL>(p, u, bu, bd, na);
   else x = 0.0;
-  double acc = p.accumulate ? wscale * p.wprev[rowoff + d] : 0.0;
+  double acc = accum ? wscale * Wprev[rowoff + d] : 0.0;
   const double vself = R[d];
   acc = fma(p.up_e[u] + p.dn_e[d] + x, vself * vscale, acc);
   // down-spin hopping (sliced ELL, slice = d / 64)
@@ -680,6 +682,8 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
     if (p.done && *p.done) continue;  // a stopped Lanczos run (uniform over the block)
     const double* __restrict__ V = p.v;
     double* __restrict__ W = p.w;
+    const double* __restrict__ Wprev = p.wprev;
+    bool accum = p.accumulate != 0;
     if (J.ctl) {  // eigensolver run: its basis column and buffer
       if (J.ctl[kCtlState] != 0.0) continue;
       const int64_t N = (int64_t)p.Dup * p.Ddn;
@@ -687,6 +691,14 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
       const int64_t col = (int64_t)J.ctl[kCtlCol];
       V = base + col * N;
       W = const_cast<double*>(base) + (col + 1) * N;
+    } else if (J.lzk) {  // Lanczos run: its buffers in rotation by the step counter
+      const int k = (int)*J.lzk;
+      const double* b[3] = {p.v, p.wprev, p.w};
+      const int r0 = lz_buffer(k, 0), r1 = lz_buffer(k, 1), r2 = lz_buffer(k, 2);
+      V = r0 == 0 ? b[0] : (r0 == 1 ? b[1] : b[2]);
+      Wprev = r1 == 0 ? b[0] : (r1 == 1 ? b[1] : b[2]);
+      W = const_cast<double*>(r2 == 0 ? b[0] : (r2 == 1 ? b[1] : b[2]));
+      accum = k > 1;
     }
     const int Ddn = p.Ddn;
     const int lg = Ddn >= 64 ? 6 : (Ddn <= 1 ? 0 : 32 - __clz(Ddn - 1));  // log2 Dp
@@ -721,9 +733,9 @@ __global__ __launch_bounds__(512) void hv_batch_wave_kernel(const HvBatchJob* __
       const bool live = u < U.row0 + U.nrows && d < Ddn;
       double vw = 0.0;
       if (live) {
-        vw = staged ? hv_batch_element<true>(J, p, V, W, u, d, vscale, wscale, na,
+        vw = staged ? hv_batch_element<true>(J, p, V, W, Wprev, accum, u, d, vscale, wscale, na,
                                              s_unit + (size_t)(u - U.row0) * Ddn)
-                    : hv_batch_element<false>(J, p, V, W, u, d, vscale, wscale, na, nullptr);
+                    : hv_batch_element<false>(J, p, V, W, Wprev, accum, u, d, vscale, wscale, na, nullptr);
       }
       if (J.tdot) {
         // Lanczos runs: this task's share of <v, w>, one wavefront tree,
@@ -982,6 +994,7 @@ void hv_batch_fill(const DevSector& ds, const HvArgs& a, int64_t row0, void* out
   j.ctl = a.ctl;
   j.v_alt = a.v_alt;
   j.tdot = a.tdot;
+  j.lzk = a.lzk;
   std::memcpy(out, &j, sizeof(j));
 }
 
@@ -1039,6 +1052,7 @@ int64_t hv_batch_rows_partials(int64_t dup) { return std::min<int64_t>(dup, 1 <<
 void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s) {
   EDH_CHECK(hv_batch_routed(ds.up.D, ds.dn.D), EDH_ERR_INTERNAL, "grouped H*v: operand not routed to the rows kernel");
   EDH_CHECK(plan(ds).grid == hv_batch_rows_partials(ds.up.D), EDH_ERR_INTERNAL, "grouped H*v: rows grid mismatch");
+  EDH_CHECK(!a.lzk, EDH_ERR_INTERNAL, "grouped H*v: rows-kernel operands take resolved buffers");
   HvArgs b = a;
   b.dot_partial = a.tdot;  // one share of <v, w> per work-group (row slot)
   launch_hv(ds, b, s, HV_SINGLE);

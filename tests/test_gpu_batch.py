@@ -310,6 +310,55 @@ def test_graph_replayed_sweeps_match_eager():
     assert np.array_equal(tight["G"], rg["G"])
 
 
+def test_lanczos_sweeps_replay_graphs_bit_identical():
+    """Lanczos runs resolve their buffer rotation and history entries from a
+    device step counter, so their sweeps are steady from step to step and
+    replay the captured graph; the coefficients are bit-identical to eager
+    launches and equal the single-sector recurrence.  The run on the
+    8-state sector (0, 1) stops early (|beta| < 1e-10 once the Krylov space
+    is the whole sector) while the others go on."""
+    import edlib_amd as ea
+    from edlib_amd.batch import Batch
+    from models import ring_hopping
+    Ns = 8
+    m = ea.HubbardModel(Ns, ring_hopping(Ns), np.full(Ns, 4.0), np.full(Ns, 2.0))
+    ctx = ea.Context(0)
+    ctx.set_model(m)
+    secs = [(4, 4), (3, 5), (2, 3), (0, 1)]
+    v0s = {}
+    for sec in secs:
+        n = ctx.set_sector(*sec)
+        v = np.random.default_rng(SEED + 5 + sec[0]).uniform(-1, 1, n)
+        v0s[sec] = v / np.linalg.norm(v)
+    nl = 45
+    res = {}
+    for graphs in (True, False):
+        b = Batch(0)
+        b.set_graphs(graphs)
+        jobs = []
+        for sec in secs:
+            sid, _ = b.add_sector(m, *sec)
+            jobs.append(b.submit_lanczos(sid, v0s[sec], nl))
+        b.run()
+        res[graphs] = ([b.lanczos_result(j, nl) for j in jobs], b.sweep_counts())
+        b.close()
+    (rg, (sw_g, rep_g)), (re_, (sw_e, rep_e)) = res[True], res[False]
+    assert sw_g == sw_e and rep_e == 0
+    assert rep_g >= nl - 4, (sw_g, rep_g)  # every step but the first two and the last
+    for (nl_g, a_g, b_g), (nl_e, a_e, b_e) in zip(rg, re_):
+        assert nl_g == nl_e
+        assert np.array_equal(a_g, a_e) and np.array_equal(b_g, b_e)
+    assert rg[3][0] <= 8 and all(r[0] == nl for r in rg[:3])
+    for sec, (nl_g, a_g, b_g) in zip(secs, rg):
+        ctx.set_sector(*sec)
+        nl_ref, a_ref, b_ref = ctx.lanczos(v0s[sec], nl)
+        assert nl_g == nl_ref
+        k = min(nl_g, 20)
+        np.testing.assert_allclose(a_g[:k], a_ref[:k], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(b_g[1:k], b_ref[1:k], rtol=1e-10, atol=1e-12)
+    ctx.close()
+
+
 def test_aim12_all_sectors_batched_against_oracle():
     """The batched solve at its stated size: single-orbital Anderson impurity
     + 11 bath sites, all 169 (N_up, N_dn) sectors (dimensions 1..853,776),
