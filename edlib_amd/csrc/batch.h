@@ -169,10 +169,18 @@ struct Batch {
   std::vector<char> last_desc;     // descriptor bytes of the last sweep
   std::vector<int64_t> last_cfg;   // and its launch configuration
   const void* last_arena = nullptr;
+  // The host runs at most kAhead sweeps ahead of the device (an event per
+  // sweep in a ring): it learns of finished processes, and so starts the
+  // work that waits on them, within that many sweeps.
+  static constexpr int kAhead = 4;
+  hipEvent_t ahead_ev[kAhead] = {};
+  bool ahead_used[kAhead] = {};
+  int ahead_i = 0;
   hipGraphExec_t sweep_graph = nullptr;
   hipEvent_t graph_ev = nullptr;  // recorded after every launch of sweep_graph
   bool graph_launched = false;
   size_t graph_sweeps = 0;  // sweeps run as graph replays
+  size_t graph_failures = 0;  // captures whose instantiation failed (those sweeps ran eagerly)
   // Executable graphs that were launched are destroyed only once their last
   // launch has completed (HIP does not defer the destruction of an exec that
   // is still running): retired ones wait here with the event of that launch.

@@ -26,6 +26,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -876,6 +877,8 @@ Batch::~Batch() {
   drop_graph();
   reap_graphs(true);
   if (graph_ev) (void)hipEventDestroy(graph_ev);
+  for (hipEvent_t& e : ahead_ev)
+    if (e) (void)hipEventDestroy(e);
   uploads.release();
   if (h_obs) (void)hipHostFree(h_obs);
   for (DevBuf* b : {&d_arena, &d_partial, &d_scal, &d_gather, &d_obs}) b->release();
@@ -1327,6 +1330,15 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   };
   std::vector<Placed> pl_pre, pl_post;
   size_t off = 0;
+  // sections start 16-byte aligned; the alignment gaps are zeroed below, so
+  // that two sweeps with the same descriptors have the same bytes (the
+  // staging buffers are reused and hold older contents)
+  std::vector<std::pair<size_t, size_t>> gaps;
+  auto pad = [&gaps](size_t end) {
+    const size_t a = align16(end);
+    if (a > end) gaps.push_back({end, a - end});
+    return a;
+  };
   // eigensolver cycle ends and the combinations they plan: one cycle
   // descriptor per run, one combination descriptor per run with its chunks
   const int ncyc = cycles ? (int)cycles->size() : 0;
@@ -1336,11 +1348,11 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     max_m = std::max(max_m, (*cycles)[q]->m);
   }
   const size_t cyc_off = off;
-  off = align16(off + (size_t)ncyc * sizeof(BCyc));
+  off = pad(off + (size_t)ncyc * sizeof(BCyc));
   const size_t comb_ops_off = off;
-  off = align16(off + (size_t)ncyc * sizeof(BComb));
+  off = pad(off + (size_t)ncyc * sizeof(BComb));
   const size_t comb_cj_off = off;
-  off = align16(off + (size_t)comb_nch * sizeof(int));
+  off = pad(off + (size_t)comb_nch * sizeof(int));
   auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
     for (OpGroup& g : gs) {
       if (g.ops.empty()) continue;
@@ -1351,9 +1363,9 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       }
       Placed p{g.kind, (int)g.ops.size(), nch, 0, 0, maxcnt};
       p.ops_off = off;
-      off = align16(off + g.ops.size() * sizeof(BOp));
+      off = pad(off + g.ops.size() * sizeof(BOp));
       p.cj_off = off;
-      off = align16(off + (size_t)nch * sizeof(int));
+      off = pad(off + (size_t)nch * sizeof(int));
       out.push_back(p);
     }
   };
@@ -1383,7 +1395,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       for (int64_t r = 0; r < ds.up.D; r += ru)
         units.push_back({(int)q, (int)r, (int)std::min<int64_t>(ru, ds.up.D - r), 0});
     }
-    off = align16(off + hv->size() * jb);
+    off = pad(off + hv->size() * jb);
   }
   int hv_lds = 0;
   if (has_hv) {
@@ -1393,12 +1405,13 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     hv_lds = hv_batch_lds_elems(spans);
   }
   const size_t units_off = off;
-  off = align16(off + units.size() * sizeof(HvBatchUnit));
+  off = pad(off + units.size() * sizeof(HvBatchUnit));
   place(post, pl_post);
   // grouped read of the processes reporting this sweep
   const int nread = reads ? (int)reads->size() : 0;
   const size_t gat_off = off;
-  off = align16(off + (size_t)nread * sizeof(BGather));
+  // (the gather kernel's descriptors: the reads after the first ncyc)
+  off = pad(off + (size_t)std::max(nread - ncyc, 0) * sizeof(BGather));
   size_t read_total = 0;
   for (int q = 0; q < nread; ++q) read_total += (*reads)[q].count;
   if (off == 0) return;
@@ -1517,6 +1530,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       roff += (int64_t)(*reads)[q].count;
     }
   }
+  for (const auto& g : gaps) std::memset(hb + g.first, 0, g.second);
   int max_ch = 1;
   for (auto* v : {&pl_pre, &pl_post})
     for (const Placed& p : *v) max_ch = std::max(max_ch, p.nch);
@@ -1545,6 +1559,14 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     last_cfg = cfg;
     last_arena = d_arena.p;
   }
+  // at most kAhead sweeps in flight: wait for the one issued kAhead sweeps ago
+  const int ai = ahead_i;
+  ahead_i = (ahead_i + 1) % kAhead;
+  if (ahead_used[ai]) {
+    issue_frees_until(ahead_ev[ai]);
+    EDH_HIP(hipEventSynchronize(ahead_ev[ai]));
+  }
+  if (!ahead_ev[ai]) EDH_HIP(hipEventCreateWithFlags(&ahead_ev[ai], hipEventDisableTiming));
   char* db = d_arena.as<char>();
   double* part = d_partial.as<double>();
   auto launch = [&](const std::vector<Placed>& pls) {
@@ -1615,8 +1637,14 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const hipError_t e = hipGraphInstantiate(&sweep_graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) {
+        // this sweep runs eagerly; capture is tried again at the next
+        // steady stretch (the failure is counted and reported once)
         sweep_graph = nullptr;
-        use_graphs = false;
+        (void)hipGetLastError();
+        if (graph_failures++ == 0)
+          fprintf(stderr, "edhip batch: hipGraphInstantiate failed (%s); the sweep runs eagerly\n",
+                  hipGetErrorString(e));
+        last_cfg.clear();  // not steady next sweep: the upload and capture start over
         kernels();
       }
     }
@@ -1656,6 +1684,8 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     }
     slab_fifo.push_back(si2);
   }
+  EDH_HIP(hipEventRecord(ahead_ev[ai], stream));
+  ahead_used[ai] = true;
   EDH_HIP(hipGetLastError());
 }
 
