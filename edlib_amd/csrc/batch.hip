@@ -470,8 +470,10 @@ __global__ void bfill_kernel(double* p, int64_t n, double x) {
 // carries that plan out.  The host only reads the run's state back.
 // ---------------------------------------------------------------------------
 
+// (the single-sector eigensolver's restart_size: ARPACK dsaup2's rule)
 __host__ __device__ inline int restart_keep(int nev, int m, int nconv) {
   int kk = nev + (nconv < (m - nev) / 2 ? nconv : (m - nev) / 2);
+  if (kk == 1) kk = m >= 6 ? m / 2 : (m > 2 ? 2 : 1);
   if (kk > m - 2) kk = m - 2;
   return kk < 1 ? 1 : kk;
 }

@@ -159,10 +159,13 @@ void dense_sector_matrix(const Model& m, const SectorTables& st, std::vector<dou
 }
 
 // ARPACK's dsaup2 rule: keep the nev wanted Ritz vectors plus one more per
-// converged pair, at most half of the new space (this is what its implicit
+// converged pair, at most half of the new space, and when that leaves a
+// single vector, half the basis (m >= 6) or two (m > 2) -- dsaup2's guard
+// against stagnation for one wanted eigenvalue (this is what its implicit
 // restart with exact shifts retains).
 int restart_size(int nev, int m, int nconv) {
-  const int kk = nev + std::min(nconv, (m - nev) / 2);
+  int kk = nev + std::min(nconv, (m - nev) / 2);
+  if (kk == 1) kk = m >= 6 ? m / 2 : (m > 2 ? 2 : 1);
   return std::max(1, std::min(kk, m - 2));
 }
 
