@@ -1640,12 +1640,14 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       (void)hipGraphDestroy(g);
       if (e != hipSuccess) {
         // this sweep runs eagerly; capture is tried again at the next
-        // steady stretch (the failure is counted and reported once)
+        // steady stretch, up to eight failures per batch (the first is
+        // reported), then the batch launches eagerly
         sweep_graph = nullptr;
         (void)hipGetLastError();
         if (graph_failures++ == 0)
           fprintf(stderr, "edhip batch: hipGraphInstantiate failed (%s); the sweep runs eagerly\n",
                   hipGetErrorString(e));
+        if (graph_failures >= 8) use_graphs = false;
         last_cfg.clear();  // not steady next sweep: the upload and capture start over
         kernels();
       }
