@@ -428,3 +428,45 @@ def test_kanamori_stream_to_14_sites_served():
         alone = solve_batched(models[r], sp)
         assert np.array_equal(alone["energies"], done[r]["energies"])
         assert np.array_equal(alone["G"], done[r]["G"])
+
+
+def test_one_wanted_eigenvalue_and_one_step_lanczos():
+    """NEV = 1: the restart keeps half the basis (ARPACK dsaup2's guard for a
+    single wanted eigenvalue), so after the first NCV steps every cycle adds
+    NCV/2 H*v, in the single-sector and the batched eigensolver alike; both
+    find the oracle's ARPACK ground energy.  A one-step Lanczos run (the
+    first step is also the last: beta_1 is not stored, Lanczos.h:64) gives
+    alpha_1 = <v|H|v> as the single-sector recurrence does."""
+    import edlib_amd as ea
+    from edlib_amd.batch import Batch
+    from oracle import edoracle as eo
+    from models import ring_hopping
+    Ns, sec, ncv = 10, (5, 5), 10
+    U = np.full(Ns, 4.0)
+    m = ea.HubbardModel(Ns, ring_hopping(Ns), U, U / 2)
+    ctx = ea.Context(0)
+    ctx.set_model(m)
+    n = ctx.set_sector(*sec)
+    ev_p, _, nav_p = ctx.eigsh(1, ncv, 1e-12, 1000, SEED + 31, vectors=False)
+    b = Batch(0)
+    sid, _ = b.add_sector(m, *sec)
+    jid = b.submit_eig(sid, 1, ncv, 1e-12, 1000, SEED + 31)
+    v0 = np.random.default_rng(SEED + 32).uniform(-1, 1, n)
+    v0 /= np.linalg.norm(v0)
+    lid = b.submit_lanczos(sid, v0, 1)
+    b.run()
+    ev_b, _ = b.eig_result(jid)
+    nav_b = b.job_info(jid)["n_av"]
+    om = eo.hubbard(Ns, ring_hopping(Ns), U, U / 2)
+    e_ar, _ = eo.storage_diag(eo.SRS(om, *sec), 1, ncv)
+    for ev, nav in ((ev_p, nav_p), (ev_b, nav_b)):
+        assert len(ev) == 1
+        assert abs(ev[0] - e_ar[0]) <= 1e-10 * abs(e_ar[0])
+        assert nav > ncv and (nav - ncv) % (ncv // 2) == 0, (nav, ncv)
+    nl, a, bb = b.lanczos_result(lid, 1)
+    nl_ref, a_ref, b_ref = ctx.lanczos(v0, 1)
+    assert nl == nl_ref == 1
+    np.testing.assert_allclose(a[0], a_ref[0], rtol=1e-12)
+    assert bb[1] == 0.0
+    ctx.close()
+    b.close()
