@@ -41,6 +41,7 @@ namespace {
 
 constexpr int kChunkMax = 2048;  // elements per reduction chunk (large operands)
 constexpr int kChunkMin = 256;   // ... (small operands: more blocks in flight)
+constexpr int kChunkHuge = 65536;  // ... (operands above 4 M elements)
 constexpr int kOpBlock = 256;  // threads per chunk kernel block
 constexpr int kHGroup = 8;     // projections accumulated per pass over a chunk
 constexpr double kDgks = 0.717;
@@ -253,22 +254,46 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_dots_kernel(const BOp* __res
 // l+64, ...; then a fixed shuffle tree), into hsum too, alpha history, norm
 // before.  mode 1: norm of the updated vector, DGKS gate, Lanczos beta
 // history and stop test.
+// Lane l takes chunks l, l + 64, ...: eight at a time into eight running
+// sums (their loads in flight together; an operand of 11.8 M elements has
+// 5,751 chunks), the rest into the first, then the eight in a fixed tree
+// (below 512 chunks only the first is used: the plain running sum).
 __device__ __forceinline__ double wave_chunk_sum(const BOp& o, const double* partial, int i, int lane) {
-  double t = 0.0;
-  for (int lc = lane; lc < o.nch; lc += 64) t += partial[(size_t)(o.chunk0 + lc) * kMaxBasis + i];
+  const double* __restrict__ p = partial + (size_t)o.chunk0 * kMaxBasis + i;
+  double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0, t5 = 0.0, t6 = 0.0, t7 = 0.0;
+  int lc = lane;
+  for (; lc + 7 * 64 < o.nch; lc += 8 * 64) {
+    t0 += p[(size_t)lc * kMaxBasis];
+    t1 += p[(size_t)(lc + 64) * kMaxBasis];
+    t2 += p[(size_t)(lc + 128) * kMaxBasis];
+    t3 += p[(size_t)(lc + 192) * kMaxBasis];
+    t4 += p[(size_t)(lc + 256) * kMaxBasis];
+    t5 += p[(size_t)(lc + 320) * kMaxBasis];
+    t6 += p[(size_t)(lc + 384) * kMaxBasis];
+    t7 += p[(size_t)(lc + 448) * kMaxBasis];
+  }
+  for (; lc < o.nch; lc += 64) t0 += p[(size_t)lc * kMaxBasis];
+  double t = ((t0 + t1) + (t2 + t3)) + ((t4 + t5) + (t6 + t7));
   for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
   return t;  // valid on lane 0
 }
 
-__global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ ops, const double* __restrict__ partial,
-                                                      int mode, int gated) {
+constexpr int kReduceBlock = 1024;  // breduce_kernel: one wavefront per coefficient of an operation
+__global__ __launch_bounds__(kReduceBlock) void breduce_kernel(const BOp* __restrict__ ops,
+                                                               const double* __restrict__ partial, int mode,
+                                                               int gated) {
   const BOp o = resolve(ops[blockIdx.x]);
   if (op_skip(o) || (gated && o.sc[S_GATE] == 0.0)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (mode == 2) {
     // after the fused update: norm in slot cnt; DGKS gate; when the gate is
     // set, the second pass's projections go to h and into hsum
+    // (wave 0 sums the norm while waves 1.. sum the projections, which are
+    // stored only when the gate is set)
     __shared__ int s_gate;
+    const int nw = (int)(blockDim.x >> 6);
+    const int ip = wave - 1;
+    double tp = 0.0;
     if (wave == 0) {
       const double t = wave_chunk_sum(o, partial, o.cnt, lane);
       if (lane == 0) {
@@ -280,14 +305,18 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
         o.sc[S_GATE] = g;
         s_gate = g;
       }
+    } else if (ip < o.cnt) {
+      tp = wave_chunk_sum(o, partial, ip, lane);
     }
     __syncthreads();
     if (!s_gate) return;
-    for (int i = wave; i < o.cnt; i += 4) {
-      const double t = wave_chunk_sum(o, partial, i, lane);
-      if (lane == 0) {
-        o.sc[i] = t;
-        if (o.hsum) o.hsum[i] += t;
+    if (wave > 0) {
+      for (int i = ip; i < o.cnt; i += nw - 1) {
+        const double t = i == ip ? tp : wave_chunk_sum(o, partial, i, lane);
+        if (lane == 0) {
+          o.sc[i] = t;
+          if (o.hsum) o.hsum[i] += t;
+        }
       }
     }
     if (o.mode & OP_SCALE_NRM) {
@@ -308,7 +337,7 @@ __global__ __launch_bounds__(256) void breduce_kernel(const BOp* __restrict__ op
   }
   if (mode == 0) {
     const int tot = o.cnt + ((o.mode & OP_NB) ? 1 : 0);
-    for (int i = wave; i < tot; i += 4) {
+    for (int i = wave; i < tot; i += (int)(blockDim.x >> 6)) {
       const double t = wave_chunk_sum(o, partial, i, lane);
       if (lane == 0) {
         if (i < o.cnt) {
@@ -816,11 +845,15 @@ __global__ void bgather_kernel(const BGather* __restrict__ ops, double* __restri
 
 
 // Chunk size of an operand of n elements: about n / 256, a power of two in
-// [kChunkMin, kChunkMax] -- it depends on n only, so a process reduces the
+// [kChunkMin, kChunkMax] (up to kChunkHuge above 4 M elements, so that
+// there are at most ~2048 chunks) -- it depends on n only, so a process reduces the
 // same way whatever else is in the batch.
 int chunk_of(int64_t n) {
   int c = kChunkMin;
   while (c < kChunkMax && (int64_t)c * 256 < n) c *= 2;
+  // very large operands: longer chunks, at most ~2048 of them (their
+  // partial sums are what the reductions read)
+  while (c < kChunkHuge && (int64_t)c * 4096 < n) c *= 2;
   return c;
 }
 int nchunks(int64_t n) { return (int)((n + chunk_of(n) - 1) / chunk_of(n)); }
@@ -1329,6 +1362,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     int kind, nops, nch;
     size_t ops_off, cj_off;
     int maxcnt;  // largest projection count of the group's operations
+    int maxnch;  // most chunks of one operation
   };
   std::vector<Placed> pl_pre, pl_post;
   size_t off = 0;
@@ -1358,12 +1392,13 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
   auto place = [&](std::vector<OpGroup>& gs, std::vector<Placed>& out) {
     for (OpGroup& g : gs) {
       if (g.ops.empty()) continue;
-      int nch = 0, maxcnt = 0;
+      int nch = 0, maxcnt = 0, maxnch = 0;
       for (const BOpHost& h : g.ops) {
         nch += nchunks(h.n);
+        maxnch = std::max(maxnch, nchunks(h.n));
         maxcnt = std::max(maxcnt, h.ctl ? h.m : h.cnt);
       }
-      Placed p{g.kind, (int)g.ops.size(), nch, 0, 0, maxcnt};
+      Placed p{g.kind, (int)g.ops.size(), nch, 0, 0, maxcnt, maxnch};
       p.ops_off = off;
       off = pad(off + g.ops.size() * sizeof(BOp));
       p.cj_off = off;
@@ -1577,31 +1612,34 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const int* dcj = (const int*)(db + p.cj_off);
       // projections (and the norm slot) in one pass over a chunk when they fit twelve registers
       const bool small = p.maxcnt + 1 <= kFuseSmall;
+      // one wavefront per coefficient; sixteen of them when the operations
+      // have many chunks (the reduction's loads are then its cost)
+      const int rblock = p.maxnch >= 512 ? kReduceBlock : 256;
       auto dots = small ? bdots_kernel<kFuseSmall> : bdots_kernel<kHGroup>;
       auto update = small ? bupdate_kernel<kFuseSmall> : bupdate_kernel<kHGroup>;
       if (p.kind == G_PROJECT) {
         hipLaunchKernelGGL(dots, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 0, 0);
         hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_REORTH) {
         // pass 1 projections (with the norm before) -> fused pass 1 update and
         // pass 2 projections (with the DGKS gate) -> pass 2 update where gated
         hipLaunchKernelGGL(dots, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 0, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 0, 0);
         auto fused = p.maxcnt <= kFuseSmall ? bupdate_dots_kernel<kFuseSmall> : bupdate_dots_kernel<kFuse>;
         hipLaunchKernelGGL(fused, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 2, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 2, 0);
         auto finish = small ? bupdate_scale_kernel<kFuseSmall> : bupdate_scale_kernel<kHGroup>;
         hipLaunchKernelGGL(finish, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
       } else if (p.kind == G_LZ) {
         // alpha from the H*v's shares, w -= alpha v_k with |w|^2, beta_k and scales
         hipLaunchKernelGGL(blz_alpha_kernel, dim3(p.nops), dim3(256), 0, stream, dops);
         hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 1, 0);
       } else if (p.kind == G_NORM) {
         hipLaunchKernelGGL(update, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj, part, 0);
-        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(256), 0, stream, dops, part, 1, 0);
+        hipLaunchKernelGGL(breduce_kernel, dim3(p.nops), dim3(rblock), 0, stream, dops, part, 1, 0);
       } else {
         hipLaunchKernelGGL(bscale_kernel, dim3(p.nch), dim3(kOpBlock), 0, stream, dops, dcj);
       }
