@@ -1650,7 +1650,34 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     if (has_hv)
       launch_hv_batch(db + hv_off, (const HvBatchUnit*)(db + units_off), (int)units.size(), hv_block, hv_lds,
                       stream);
-    for (int q : routed) launch_hv_batch_rows(sector((*hv)[q].first).dev, (*hv)[q].second, stream);
+    {
+      // Lanczos operands of one sector share a multi-operand launch (up to
+      // kMultiMax at a time, the same arithmetic per operand), the others
+      // take one launch each
+      std::vector<std::pair<int, std::vector<HvArgs>>> groups;  // (sector, operands)
+      for (int q : routed) {
+        const int sid = (*hv)[q].first;
+        const HvArgs& a = (*hv)[q].second;
+        const DevSector& ds = sector(sid).dev;
+        if (a.ctl || !a.dscale || !a.tdot || hv_batch_rows_multi_fit(ds) < 2) {
+          launch_hv_batch_rows(ds, a, stream);
+          continue;
+        }
+        auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) {
+          return g.first == sid && (int)g.second.size() < hv_batch_rows_multi_fit(ds);
+        });
+        if (it == groups.end()) {
+          groups.push_back({sid, {}});
+          it = groups.end() - 1;
+        }
+        it->second.push_back(a);
+      }
+      for (const auto& g : groups) {
+        const DevSector& ds = sector(g.first).dev;
+        if (g.second.size() == 1) launch_hv_batch_rows(ds, g.second[0], stream);
+        else launch_hv_batch_rows_multi(ds, g.second.data(), (int)g.second.size(), stream);
+      }
+    }
     launch(pl_post);
     if (ncyc) {
       const size_t smem = (size_t)(2 * max_m * max_m + 3 * max_m) * sizeof(double) + (size_t)max_m * sizeof(int);

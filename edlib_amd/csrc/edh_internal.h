@@ -295,5 +295,12 @@ void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, i
 bool hv_batch_routed(int64_t dup, int64_t ddn);
 int64_t hv_batch_rows_partials(int64_t dup);
 void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s);
+// Several Lanczos operands of one rows-kernel sector in one launch, each with
+// the single-operand launch's arithmetic: hv_batch_rows_multi_fit gives how
+// many fit one launch (LDS), 0 when the sector's format has no such kernel;
+// the operands carry device scales (dscale) and their <v, w> shares (tdot).
+constexpr int kMultiMax = 4;
+int hv_batch_rows_multi_fit(const DevSector& ds);
+void launch_hv_batch_rows_multi(const DevSector& ds, const HvArgs* a, int count, hipStream_t s);
 
 }  // namespace edh

@@ -466,6 +466,207 @@ __global__ __launch_bounds__(1024) void hv_rows_kernel(HvParams p) {
   if (UP && p.dot_partial) block_reduce_store(dsum, s_red, p.dot_partial + blockIdx.x);
 }
 
+// Several Lanczos operands of one sector in one launch (the batch's
+// Green's-function runs on sectors routed to the rows kernel).  Each operand
+// gets exactly the arithmetic of hv_rows_kernel<4, CROSS, KAN, true, EF,
+// true, false> -- the same column ownership, the same order of its terms
+// (diagonal, then down- and up-spin hopping in the wavefront's order, then
+// the impurity terms) and the same share of <v, w> per work-group -- so its
+// result does not depend on which operands share the launch; the sector
+// tables, the diagonal and the down-spin entries are read once for all of
+// them, and their rows sit side by side in LDS.  Operands nk..K-1 and
+// stopped runs are skipped (uniform over the grid).
+struct HvMulti {
+  const double* v[kMultiMax];
+  double* w[kMultiMax];
+  const double* wprev[kMultiMax];
+  const double* dscale[kMultiMax];  // {vscale, wscale} on the device
+  const int* done[kMultiMax];
+  double* dot[kMultiMax];
+  int accumulate[kMultiMax];
+  int nk;
+};
+
+template <int CROSS, bool KAN, int EF>
+__global__ __launch_bounds__(1024) void hv_rows_multi_kernel(HvParams p, HvMulti m) {
+  constexpr int J = 4, K = kMultiMax;
+  extern __shared__ double s_v[];  // K rows of Ddn + 1, then (EF 4) the value table
+  __shared__ double s_red[16];
+  const int T = blockDim.x;
+  const int tid = threadIdx.x;
+  const int Ddn = p.Ddn;
+  const int ld = Ddn + 1;
+  const int na = 1 << (p.Ns - p.nb);
+  bool live[K];
+  double vs[K], ws[K], dsum[K];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    live[k] = k < m.nk && !(m.done[k] && *m.done[k]);
+    vs[k] = live[k] ? m.dscale[k][0] : 0.0;
+    ws[k] = live[k] ? m.dscale[k][1] : 0.0;
+    dsum[k] = 0.0;
+    any = any || live[k];
+  }
+  if (!any) return;
+  const double* __restrict__ vals = p.values;
+  double* s_vals = s_v + K * ld;
+  if (EF == 4)
+    for (int i = tid; i < p.nvals; i += T) s_vals[i] = vals[i];
+
+  for (int slot0 = blockIdx.x; slot0 < p.Dup; slot0 += gridDim.x) {
+    int slot = slot0;
+    if (p.xcd_rows && slot0 < (p.Dup & ~255))
+      slot = (slot0 & ~255) + ((slot0 & 7) << 5) + ((slot0 >> 3) & 31);
+    const int u = __builtin_amdgcn_readfirstlane(p.row_order[slot]);
+    const size_t rowoff = (size_t)u * Ddn;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!live[k]) continue;
+      const double* __restrict__ V = m.v[k];
+      for (int d0 = 0; d0 < Ddn; d0 += kStage * T) {
+        double x[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int d = d0 + tid + q * T;
+          x[q] = d < Ddn ? V[rowoff + d] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+          const int d = d0 + tid + q * T;
+          if (d < Ddn) s_v[k * ld + d] = x[q] * vs[k];
+        }
+      }
+      if (EF && tid == 0) s_v[k * ld + Ddn] = 0.0;
+    }
+    __syncthreads();
+    const int bu = p.up_bits[u];
+    const double eu = p.up_e[u];
+    const int hb = p.up_rowp[u], he = p.up_rowp[u + 1];
+    for (int c0 = 0; c0 < Ddn; c0 += T * J) {
+#define DCOL(j) min(c0 + tid + (j) * T, Ddn - 1)
+      double acc[K][J];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double win[J];
+        if (live[k] && m.accumulate[k]) {
+#pragma unroll
+          for (int j = 0; j < J; ++j) win[j] = __builtin_nontemporal_load(m.wprev[k] + rowoff + DCOL(j));
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) acc[k][j] = (live[k] && m.accumulate[k]) ? ws[k] * win[j] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = DCOL(j);
+        const int bd = p.dn_bits[d];
+        const double dg = eu + p.dn_e[d] + cross_term<CROSS>(p, u, bu, bd, na);
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (live[k]) acc[k][j] = fma(dg, s_v[k * ld + d], acc[k][j]);
+      }
+      auto down_term = [&]() {
+        int len[J], base[J];
+        int maxlen = 0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int sl = DCOL(j) >> 6;
+          len[j] = p.dn_sllen[sl];
+          base[j] = len[j] > 0 ? p.dn_slptr[sl] + (DCOL(j) & 63) : 0;
+          maxlen = max(maxlen, len[j]);
+        }
+        const double* __restrict__ vt = EF == 4 ? s_vals : vals;
+        constexpr int kSlots = 4;
+        for (int k0 = 0; k0 < maxlen; k0 += kSlots) {
+          uint32_t e[kSlots][J];
+#pragma unroll
+          for (int sq = 0; sq < kSlots; ++sq)
+#pragma unroll
+            for (int j = 0; j < J; ++j) e[sq][j] = p.dn_ell[base[j] + min(k0 + sq, max(len[j] - 1, 0)) * kSlice];
+#pragma unroll
+          for (int sq = 0; sq < kSlots; ++sq) {
+            if (k0 + sq >= maxlen) break;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+              const uint32_t x = k0 + sq < len[j] ? e[sq][j] : 0u;
+              const int c = (int)(x & kColMask);
+              const double val = vt[x >> kColBits];
+#pragma unroll
+              for (int k = 0; k < K; ++k)
+                if (live[k]) acc[k][j] = fma(val, s_v[k * ld + c], acc[k][j]);
+            }
+          }
+        }
+      };
+      auto up_term = [&]() {
+        // one neighbour row at a time for all operands (each operand's FMAs
+        // still in neighbour order)
+        for (int q = hb; q < he; ++q) {
+          const uint32_t e = p.up_packed[q];
+          const double hv = vals[e >> kColBits];
+          const size_t nrow = (size_t)(e & kColMask) * Ddn;
+          double x[K][J];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (live[k])
+#pragma unroll
+              for (int j = 0; j < J; ++j) x[k][j] = m.v[k][nrow + DCOL(j)];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (live[k]) {
+              const double h = hv * vs[k];
+#pragma unroll
+              for (int j = 0; j < J; ++j) acc[k][j] = fma(h, x[k][j], acc[k][j]);
+            }
+        }
+      };
+      if ((tid >> 6) & 1) {
+        up_term();
+        down_term();
+      } else {
+        down_term();
+        up_term();
+      }
+      if (KAN) {
+        const int a = bu >> p.nb;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int bd = p.dn_bits[DCOL(j)];
+          const int b = bd >> p.nb;
+          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+          for (int q = kb; q < ke; ++q) {
+            const double c = p.kan_c[q];
+            const int64_t off = (int64_t)rowoff + DCOL(j) + p.kan_off[q];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              if (live[k]) acc[k][j] = fma(c, m.v[k][off] * vs[k], acc[k][j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int d = c0 + tid + j * T;
+        if (d < Ddn) {
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            if (live[k]) {
+              __builtin_nontemporal_store(acc[k][j], m.w[k] + rowoff + d);
+              dsum[k] = fma(s_v[k * ld + d], acc[k][j], dsum[k]);
+            }
+        }
+      }
+#undef DCOL
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (!live[k]) continue;
+    block_reduce_store(dsum[k], s_red, m.dot[k] + blockIdx.x);
+    __syncthreads();  // s_red is reused by the next operand's reduction
+  }
+}
+
 // Up-spin term over column slabs of 16 (one 128-byte line per row segment):
 // w[u][c] += sum_k Hup[u][k] v[k][c].  Persistent work-groups read the XCD
 // they run on (HW_REG_XCC_ID) and claim (slab, row block) items from that
@@ -1048,6 +1249,67 @@ bool hv_batch_routed(int64_t dup, int64_t ddn) {
 }
 
 int64_t hv_batch_rows_partials(int64_t dup) { return std::min<int64_t>(dup, 1 << 20); }
+
+int hv_batch_rows_multi_fit(const DevSector& ds) {
+  if (!hv_batch_routed(ds.up.D, ds.dn.D)) return 0;
+  const Launch L = plan(ds);
+  if (!L.lds || L.J != 4 || (L.ef != 0 && L.ef != 4)) return 0;
+  const size_t rowb = (size_t)(ds.dn.D + 1) * sizeof(double);
+  const size_t valb = L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0;
+  return (size_t)kMultiMax * rowb + valb <= lds_budget() ? kMultiMax : 0;
+}
+
+template <int CROSS, bool KAN, int EF>
+void launch_multi_t(const HvParams& p, const HvMulti& m, const Launch& L, size_t smem, hipStream_t s) {
+  auto k = hv_rows_multi_kernel<CROSS, KAN, EF>;
+  if (smem > 64 * 1024)
+    EDH_HIP(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+  hipLaunchKernelGGL(k, dim3(L.grid), dim3(L.block), smem, s, p, m);
+  EDH_HIP(hipGetLastError());
+}
+
+template <int CROSS>
+void launch_multi_c(const HvParams& p, const HvMulti& m, const Launch& L, size_t smem, bool kan, hipStream_t s) {
+  if (kan) {
+    if (L.ef == 4) launch_multi_t<CROSS, true, 4>(p, m, L, smem, s);
+    else launch_multi_t<CROSS, true, 0>(p, m, L, smem, s);
+  } else {
+    if (L.ef == 4) launch_multi_t<CROSS, false, 4>(p, m, L, smem, s);
+    else launch_multi_t<CROSS, false, 0>(p, m, L, smem, s);
+  }
+}
+
+void launch_hv_batch_rows_multi(const DevSector& ds, const HvArgs* a, int count, hipStream_t s) {
+  const int fit = hv_batch_rows_multi_fit(ds);
+  EDH_CHECK(count >= 2 && count <= fit, EDH_ERR_INTERNAL, "grouped H*v: operand count does not fit one launch");
+  const Launch L = plan(ds);
+  EDH_CHECK(L.grid == hv_batch_rows_partials(ds.up.D), EDH_ERR_INTERNAL, "grouped H*v: rows grid mismatch");
+  HvMulti m;
+  std::memset(&m, 0, sizeof(m));
+  for (int k = 0; k < count; ++k) {
+    EDH_CHECK(a[k].dscale && a[k].tdot && !a[k].lzk && !a[k].ctl, EDH_ERR_INTERNAL,
+              "grouped H*v: multi-operand launch takes Lanczos operands with device scales");
+    m.v[k] = a[k].v;
+    m.w[k] = a[k].w;
+    m.wprev[k] = a[k].wprev;
+    m.dscale[k] = a[k].dscale;
+    m.done[k] = a[k].done;
+    m.dot[k] = a[k].tdot;
+    m.accumulate[k] = a[k].mode;
+  }
+  m.nk = count;
+  const HvParams p = make_params(ds, a[0]);
+  const size_t smem = (size_t)kMultiMax * (ds.dn.D + 1) * sizeof(double) +
+                      (L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0);
+  EDH_CHECK(smem <= lds_budget(), EDH_ERR_INTERNAL, "grouped H*v: multi-operand rows do not fit LDS");
+  const bool kan = ds.kan_ml > 0;
+  switch (ds.cross_mode) {
+    case CROSS_NONE: launch_multi_c<CROSS_NONE>(p, m, L, smem, kan, s); break;
+    case CROSS_POPC: launch_multi_c<CROSS_POPC>(p, m, L, smem, kan, s); break;
+    case CROSS_IMP: launch_multi_c<CROSS_IMP>(p, m, L, smem, kan, s); break;
+    default: launch_multi_c<CROSS_GENERAL>(p, m, L, smem, kan, s); break;
+  }
+}
 
 void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s) {
   EDH_CHECK(hv_batch_routed(ds.up.D, ds.dn.D), EDH_ERR_INTERNAL, "grouped H*v: operand not routed to the rows kernel");

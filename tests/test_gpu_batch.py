@@ -470,3 +470,47 @@ def test_one_wanted_eigenvalue_and_one_step_lanczos():
     assert bb[1] == 0.0
     ctx.close()
     b.close()
+
+
+def test_lanczos_runs_sharing_a_rows_kernel_launch_are_bit_identical():
+    """Lanczos runs on one sector routed to the rows kernel (Kanamori 3 + 11,
+    sector (5,7): 2002 x 3432 = 6,870,864 states) share multi-operand
+    launches when several are active: each run's coefficients are
+    bit-identical to the run alone (single-operand launch) whatever the
+    group (2, 4, 4 + 1 operands), and equal the single-sector recurrence."""
+    import edlib_amd as ea
+    from edlib_amd.batch import Batch
+    from edlib_amd.workloads import siam_model
+    m = siam_model(14, 3, SEED + 1001)
+    sec, nl = (5, 7), 12
+    ctx = ea.Context(0)
+    ctx.set_model(m)
+    n = ctx.set_sector(*sec)
+    assert n == 2002 * 3432
+    rng = np.random.default_rng(SEED + 41)
+    v0s = []
+    for _ in range(5):
+        v = rng.uniform(-1, 1, n)
+        v0s.append(v / np.linalg.norm(v))
+
+    def run(vs):
+        b = Batch(0)
+        sid, _ = b.add_sector(m, *sec)
+        jobs = [b.submit_lanczos(sid, v, nl) for v in vs]
+        b.run()
+        out = [b.lanczos_result(j, nl) for j in jobs]
+        b.close()
+        return out
+
+    alone = [run([v])[0] for v in v0s]
+    for group in ([0, 1], [0, 1, 2, 3], [0, 1, 2, 3, 4]):
+        res = run([v0s[i] for i in group])
+        for i, (nl_g, a_g, b_g) in zip(group, res):
+            nl_a, a_a, b_a = alone[i]
+            assert nl_g == nl_a
+            assert np.array_equal(a_g, a_a) and np.array_equal(b_g, b_a), (group, i)
+    nl_ref, a_ref, b_ref = ctx.lanczos(v0s[0], nl)
+    assert alone[0][0] == nl_ref
+    np.testing.assert_allclose(alone[0][1][:nl], a_ref[:nl], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(alone[0][2][1:nl], b_ref[1:nl], rtol=1e-10, atol=1e-12)
+    ctx.close()
