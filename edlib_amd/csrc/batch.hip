@@ -404,17 +404,52 @@ __global__ __launch_bounds__(kOpBlock) void bupdate_kernel(const BOp* __restrict
   const int64_t e0 = (int64_t)lc * o.chunk;
   const int64_t e1 = min(o.n, e0 + o.chunk);
   double nsq = 0.0;
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
-    double x = o.w[e];
-    for (int i0 = 0; i0 < o.cnt; i0 += KG) {
-      double xv[KG];
+  if (o.cnt <= 1) {
+    // a Lanczos update (one projection) or a norm: four elements per
+    // iteration with their loads in flight together, the same operations
+    // in the same order as the general loop (whose remaining FMAs with
+    // zero coefficients leave x unchanged)
+    const double h0 = s_h[0];
+    const bool upd = o.cnt == 1;
+    int64_t e = e0 + threadIdx.x;
+    for (; e + 3 * kOpBlock < e1; e += 4 * kOpBlock) {
+      double x[4], xv[4];
 #pragma unroll
-      for (int g = 0; g < KG; ++g) xv[g] = i0 + g < o.cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
+      for (int u = 0; u < 4; ++u) x[u] = o.w[e + u * kOpBlock];
+      if (upd) {
 #pragma unroll
-      for (int g = 0; g < KG; ++g) x = fma(-s_h[i0 + g], xv[g], x);
+        for (int u = 0; u < 4; ++u) xv[u] = o.X[e + u * kOpBlock];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (upd) {
+          x[u] = fma(-h0, xv[u], x[u]);
+          o.w[e + u * kOpBlock] = x[u];
+        }
+        nsq = fma(x[u], x[u], nsq);
+      }
     }
-    if (o.cnt) o.w[e] = x;
-    nsq = fma(x, x, nsq);
+    for (; e < e1; e += kOpBlock) {
+      double x = o.w[e];
+      if (upd) {
+        x = fma(-h0, o.X[e], x);
+        o.w[e] = x;
+      }
+      nsq = fma(x, x, nsq);
+    }
+  } else {
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kOpBlock) {
+      double x = o.w[e];
+      for (int i0 = 0; i0 < o.cnt; i0 += KG) {
+        double xv[KG];
+#pragma unroll
+        for (int g = 0; g < KG; ++g) xv[g] = i0 + g < o.cnt ? o.X[(int64_t)(i0 + g) * o.ldx + e] : 0.0;
+#pragma unroll
+        for (int g = 0; g < KG; ++g) x = fma(-s_h[i0 + g], xv[g], x);
+      }
+      if (o.cnt) o.w[e] = x;
+      nsq = fma(x, x, nsq);
+    }
   }
   const double t = block_sum256(nsq, s_red);
   if (threadIdx.x == 0) partial[(size_t)c * kMaxBasis] = t;
