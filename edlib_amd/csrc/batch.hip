@@ -1258,7 +1258,7 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
   // <v, w>, each 256-byte aligned
   auto al = [](size_t c) { return (c + 31) & ~size_t(31); };
   const size_t nv = al((size_t)j.n), nsc = S_T + 2 * (size_t)nl + 3;
-  j.ntdot = hv_batch_routed(s.tabs.up.D, s.tabs.dn.D) ? hv_batch_rows_partials(s.tabs.up.D)
+  j.ntdot = hv_batch_routed_lanczos(s.tabs.up.D, s.tabs.dn.D) ? hv_batch_rows_partials(s.tabs.up.D)
                                                        : hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
   double* base = dalloc(j, 3 * nv + al(nsc) + (size_t)j.ntdot);
   for (int q = 0; q < 3; ++q) j.lz[q] = j.lzb[q] = base + q * nv;
@@ -1451,14 +1451,22 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
     // rows per unit: one while the rows of the sweep are few (parallelism),
     // up to ~4K elements per unit when there are many (less scheduling)
     int64_t total_rows = 0;
+    // Lanczos operands (device scales and <v, w> shares, no control
+    // block) follow hv_batch_routed_lanczos, the others hv_batch_routed
+    auto routed_op = [&](size_t q) {
+      const DevSector& ds = sector((*hv)[q].first).dev;
+      const HvArgs& a = (*hv)[q].second;
+      const bool lanczos = !a.ctl && a.dscale && a.tdot;
+      return lanczos ? hv_batch_routed_lanczos(ds.up.D, ds.dn.D) : hv_batch_routed(ds.up.D, ds.dn.D);
+    };
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
-      if (!hv_batch_routed(ds.up.D, ds.dn.D)) total_rows += ds.up.D;
+      if (!routed_op(q)) total_rows += ds.up.D;
     }
     const int64_t spread = std::max<int64_t>(1, total_rows / 8192);
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
-      if (hv_batch_routed(ds.up.D, ds.dn.D)) {
+      if (routed_op(q)) {
         routed.push_back((int)q);
         continue;
       }
@@ -1869,7 +1877,7 @@ int Batch::step() {
   for (BJob* j : lz) {
     HvArgs a{};
     const DevSector& ds = sector(j->sec).dev;
-    if (hv_batch_routed(ds.up.D, ds.dn.D)) {
+    if (hv_batch_routed_lanczos(ds.up.D, ds.dn.D)) {
       // the rows kernel takes this step's buffers
       a.v = j->lz[0];
       a.w = j->lz[2];
