@@ -6,7 +6,10 @@ function (4 start vectors per thermally relevant pair, beta=10, 256
 Matsubara points, 60 Lanczos steps), everything through the batched engine.
 A first solve warms up the code paths; the second is the one reported, with
 the wall time of its eigensolver and Green's-function phases.
-Usage: python -m edlib_amd.profile_solve [--plain] [--no-graphs]
+With --kanamori14: the served stream's 14-site request instead (Kanamori
+3 orbitals + 11 bath sites, U=4, J=0.7, half filling, all 225 sectors, G of
+the three impurity orbitals), one solve.
+Usage: python -m edlib_amd.profile_solve [--plain] [--no-graphs] [--kanamori14]
 (--no-graphs launches every sweep's kernels one by one, for counter
 collection, whose per-dispatch sampling does not follow replayed graphs)
 """
@@ -25,16 +28,21 @@ def main():
     ap.add_argument("--plain", action="store_true", help="also time the plain per-sector solve")
     ap.add_argument("--sweeps", type=int, default=16, help="sweeps per Batch.step call")
     ap.add_argument("--no-graphs", action="store_true", help="no hipGraph replay of steady sweeps")
+    ap.add_argument("--kanamori14", action="store_true", help="the 14-site Kanamori request, one solve")
     a = ap.parse_args()
     from edlib_amd.batch import Batch
     from edlib_amd.solver import BatchedSolve, SolveParams, solve_plain
     from edlib_amd.workloads import siam_model
-    sp = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0,), seed=SEED)
-    m = siam_model(12, 1, SEED + 4)
+    if a.kanamori14:
+        sp = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0, 1, 2), seed=SEED)
+        m = siam_model(14, 3, SEED + 1000 + 1)
+    else:
+        sp = SolveParams(nev=4, beta=10.0, nomega=256, nlanc=60, orbitals=(0,), seed=SEED)
+        m = siam_model(12, 1, SEED + 4)
     b = Batch(0)
     if a.no_graphs:
         b.set_graphs(False)
-    for rep in range(2):
+    for rep in range(1 if a.kanamori14 else 2):
         t0 = time.perf_counter()
         s = BatchedSolve(b, m, sp)
         s.start()
