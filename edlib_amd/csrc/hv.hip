@@ -520,25 +520,28 @@ __global__ __launch_bounds__(1024) void hv_rows_multi_kernel(HvParams p, HvMulti
       slot = (slot0 & ~255) + ((slot0 & 7) << 5) + ((slot0 >> 3) & 31);
     const int u = __builtin_amdgcn_readfirstlane(p.row_order[slot]);
     const size_t rowoff = (size_t)u * Ddn;
+    // the rows of all operands: four elements of each per thread and round,
+    // every load of a round in flight together
+    constexpr int kStageM = 4;
+    for (int d0 = 0; d0 < Ddn; d0 += kStageM * T) {
+      double x[K][kStageM];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (!live[k]) continue;
-      const double* __restrict__ V = m.v[k];
-      for (int d0 = 0; d0 < Ddn; d0 += kStage * T) {
-        double x[kStage];
+      for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int q = 0; q < kStage; ++q) {
+        for (int q = 0; q < kStageM; ++q) {
           const int d = d0 + tid + q * T;
-          x[q] = d < Ddn ? V[rowoff + d] : 0.0;
+          x[k][q] = (live[k] && d < Ddn) ? m.v[k][rowoff + d] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kStage; ++q) {
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int q = 0; q < kStageM; ++q) {
           const int d = d0 + tid + q * T;
-          if (d < Ddn) s_v[k * ld + d] = x[q] * vs[k];
+          if (live[k] && d < Ddn) s_v[k * ld + d] = x[k][q] * vs[k];
         }
-      }
-      if (EF && tid == 0) s_v[k * ld + Ddn] = 0.0;
     }
+    if (EF && tid == 0)
+      for (int k = 0; k < K; ++k) s_v[k * ld + Ddn] = 0.0;
     __syncthreads();
     const int bu = p.up_bits[u];
     const double eu = p.up_e[u];
