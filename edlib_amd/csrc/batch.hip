@@ -1258,7 +1258,7 @@ int Batch::submit_lanczos(int sec, const double* v0_dev, int nl) {
   // <v, w>, each 256-byte aligned
   auto al = [](size_t c) { return (c + 31) & ~size_t(31); };
   const size_t nv = al((size_t)j.n), nsc = S_T + 2 * (size_t)nl + 3;
-  j.ntdot = hv_batch_routed_lanczos(s.tabs.up.D, s.tabs.dn.D) ? hv_batch_rows_partials(s.tabs.up.D)
+  j.ntdot = hv_batch_routed_lanczos(s.dev) ? hv_batch_rows_partials(s.tabs.up.D)
                                                        : hv_batch_tasks(s.tabs.up.D, s.tabs.dn.D);
   double* base = dalloc(j, 3 * nv + al(nsc) + (size_t)j.ntdot);
   for (int q = 0; q < 3; ++q) j.lz[q] = j.lzb[q] = base + q * nv;
@@ -1457,7 +1457,7 @@ void Batch::issue(std::vector<OpGroup>& pre, const std::vector<std::pair<int, Hv
       const DevSector& ds = sector((*hv)[q].first).dev;
       const HvArgs& a = (*hv)[q].second;
       const bool lanczos = !a.ctl && a.dscale && a.tdot;
-      return lanczos ? hv_batch_routed_lanczos(ds.up.D, ds.dn.D) : hv_batch_routed(ds.up.D, ds.dn.D);
+      return lanczos ? hv_batch_routed_lanczos(ds) : hv_batch_routed(ds.up.D, ds.dn.D);
     };
     for (size_t q = 0; q < hv->size(); ++q) {
       const DevSector& ds = sector((*hv)[q].first).dev;
@@ -1877,7 +1877,7 @@ int Batch::step() {
   for (BJob* j : lz) {
     HvArgs a{};
     const DevSector& ds = sector(j->sec).dev;
-    if (hv_batch_routed_lanczos(ds.up.D, ds.dn.D)) {
+    if (hv_batch_routed_lanczos(ds)) {
       // the rows kernel takes this step's buffers
       a.v = j->lz[0];
       a.w = j->lz[2];

@@ -294,10 +294,10 @@ void launch_hv_batch(const void* djobs, const HvBatchUnit* dunits, int nunits, i
 // Lanczos <v, w> shares are one per work-group (hv_batch_rows_partials).
 bool hv_batch_routed(int64_t dup, int64_t ddn);
 // The same for Lanczos operands (Green's-function runs): also sectors of at
-// least 512 rows of 1024..4608 elements, whose runs share multi-operand
-// launches (the decision depends on the sector only, so a run's arithmetic
-// does not depend on the other runs).
-bool hv_batch_routed_lanczos(int64_t dup, int64_t ddn);
+// least 256 rows of 512..4608 elements whose format has the multi-operand
+// kernel, so that their runs share launches (the decision depends on the
+// sector only, so a run's arithmetic does not depend on the other runs).
+bool hv_batch_routed_lanczos(const DevSector& ds);
 int64_t hv_batch_rows_partials(int64_t dup);
 void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s);
 // Several Lanczos operands of one rows-kernel sector in one launch, each with

@@ -1251,19 +1251,27 @@ bool hv_batch_routed(int64_t dup, int64_t ddn) {
   return dup * ddn >= (int64_t(1) << 22) && (ddn > kStageMax || (ddn >= 1024 && dup >= 1024));
 }
 
-bool hv_batch_routed_lanczos(int64_t dup, int64_t ddn) {
-  return hv_batch_routed(dup, ddn) || (dup >= 256 && ddn >= 512 && ddn <= kStageMax);
+namespace {
+// The sector's entry format and row length allow the multi-operand kernel.
+bool multi_ok(const DevSector& ds) {
+  const Launch L = plan(ds);
+  if (!L.lds || L.J != 4 || (L.ef != 0 && L.ef != 4)) return false;
+  const size_t rowb = (size_t)(ds.dn.D + 1) * sizeof(double);
+  const size_t valb = L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0;
+  return (size_t)kMultiMax * rowb + valb <= lds_budget();
+}
+}  // namespace
+
+bool hv_batch_routed_lanczos(const DevSector& ds) {
+  const int64_t dup = ds.up.D, ddn = ds.dn.D;
+  if (hv_batch_routed(dup, ddn)) return true;
+  return dup >= 256 && ddn >= 512 && ddn <= kStageMax && multi_ok(ds);
 }
 
 int64_t hv_batch_rows_partials(int64_t dup) { return std::min<int64_t>(dup, 1 << 20); }
 
 int hv_batch_rows_multi_fit(const DevSector& ds) {
-  if (!hv_batch_routed_lanczos(ds.up.D, ds.dn.D)) return 0;
-  const Launch L = plan(ds);
-  if (!L.lds || L.J != 4 || (L.ef != 0 && L.ef != 4)) return 0;
-  const size_t rowb = (size_t)(ds.dn.D + 1) * sizeof(double);
-  const size_t valb = L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0;
-  return (size_t)kMultiMax * rowb + valb <= lds_budget() ? kMultiMax : 0;
+  return hv_batch_routed_lanczos(ds) && multi_ok(ds) ? kMultiMax : 0;
 }
 
 template <int CROSS, bool KAN, int EF>
@@ -1319,7 +1327,7 @@ void launch_hv_batch_rows_multi(const DevSector& ds, const HvArgs* a, int count,
 }
 
 void launch_hv_batch_rows(const DevSector& ds, const HvArgs& a, hipStream_t s) {
-  EDH_CHECK(hv_batch_routed_lanczos(ds.up.D, ds.dn.D), EDH_ERR_INTERNAL,
+  EDH_CHECK(hv_batch_routed_lanczos(ds), EDH_ERR_INTERNAL,
             "grouped H*v: operand not routed to the rows kernel");
   EDH_CHECK(plan(ds).grid == hv_batch_rows_partials(ds.up.D), EDH_ERR_INTERNAL, "grouped H*v: rows grid mismatch");
   EDH_CHECK(!a.lzk, EDH_ERR_INTERNAL, "grouped H*v: rows-kernel operands take resolved buffers");
