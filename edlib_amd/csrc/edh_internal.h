@@ -184,6 +184,7 @@ struct DevSector {
   DevSpin up, dn;
   double* values = nullptr;
   int nvalues = 0;  // entries of values
+  int nkan = 0;     // impurity terms (entries of kan_off and kan_c)
   int imp_ml = 0;
   int cross_mode = 0;
   double cross_u = 0.0;

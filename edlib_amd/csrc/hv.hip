@@ -55,6 +55,7 @@ struct HvParams {
   double dn_s4_scale;
   const double* values;
   int nvals;  // entries of values (staged in LDS after the row with EF == 4)
+  int nkan;   // impurity terms (staged in LDS by the multi-operand kernel)
   // cross term
   double cross_u;
   const double* cross_imp;
@@ -513,6 +514,19 @@ __global__ __launch_bounds__(1024) void hv_rows_multi_kernel(HvParams p, HvMulti
   double* s_vals = s_v + K * ld;
   if (EF == 4)
     for (int i = tid; i < p.nvals; i += T) s_vals[i] = vals[i];
+  // the impurity terms' tables after the values (the first row's barrier
+  // orders these writes before any read): coefficients, element offsets,
+  // then the (a, b) pointers
+  double* s_kc = s_vals + (EF == 4 ? p.nvals : 0);
+  int64_t* s_koff = reinterpret_cast<int64_t*>(s_kc + (KAN ? p.nkan : 0));
+  int* s_kptr = reinterpret_cast<int*>(s_koff + (KAN ? p.nkan : 0));
+  if (KAN) {
+    for (int i = tid; i < p.nkan; i += T) {
+      s_kc[i] = p.kan_c[i];
+      s_koff[i] = p.kan_off[i];
+    }
+    for (int i = tid; i <= na * na; i += T) s_kptr[i] = p.kan_ptr[i];
+  }
 
   for (int slot0 = blockIdx.x; slot0 < p.Dup; slot0 += gridDim.x) {
     int slot = slot0;
@@ -636,10 +650,10 @@ __global__ __launch_bounds__(1024) void hv_rows_multi_kernel(HvParams p, HvMulti
         for (int j = 0; j < J; ++j) {
           const int bd = p.dn_bits[DCOL(j)];
           const int b = bd >> p.nb;
-          const int kb = p.kan_ptr[a * na + b], ke = p.kan_ptr[a * na + b + 1];
+          const int kb = s_kptr[a * na + b], ke = s_kptr[a * na + b + 1];
           for (int q = kb; q < ke; ++q) {
-            const double c = p.kan_c[q];
-            const int64_t off = (int64_t)rowoff + DCOL(j) + p.kan_off[q];
+            const double c = s_kc[q];
+            const int64_t off = (int64_t)rowoff + DCOL(j) + s_koff[q];
 #pragma unroll
             for (int k = 0; k < K; ++k)
               if (live[k]) acc[k][j] = fma(c, m.v[k][off] * vs[k], acc[k][j]);
@@ -1080,6 +1094,7 @@ HvParams make_params(const DevSector& ds, const HvArgs& a) {
   for (int q = 0; q < 4; ++q) p.dn_vt[q] = ds.dn.vt16[q];
   p.values = ds.values;
   p.nvals = ds.nvalues;
+  p.nkan = ds.nkan;
   p.cross_u = ds.cross_u;
   p.cross_imp = ds.cross_imp;
   p.cross_rowtab = ds.cross_rowtab;
@@ -1252,13 +1267,23 @@ bool hv_batch_routed(int64_t dup, int64_t ddn) {
 }
 
 namespace {
+// LDS of the multi-operand kernel: the operands' rows, the value table (EF
+// 4) and the impurity terms' tables.
+size_t multi_smem(const DevSector& ds, int ef) {
+  size_t b = (size_t)kMultiMax * (ds.dn.D + 1) * sizeof(double);
+  if (ef == 4) b += (size_t)ds.nvalues * sizeof(double);
+  if (ds.kan_ml > 0) {
+    const int na = 1 << ds.kan_ml;
+    b += (size_t)ds.nkan * (sizeof(double) + sizeof(int64_t)) + (size_t)(na * na + 1) * sizeof(int);
+  }
+  return b;
+}
+
 // The sector's entry format and row length allow the multi-operand kernel.
 bool multi_ok(const DevSector& ds) {
   const Launch L = plan(ds);
   if (!L.lds || L.J != 4 || (L.ef != 0 && L.ef != 4)) return false;
-  const size_t rowb = (size_t)(ds.dn.D + 1) * sizeof(double);
-  const size_t valb = L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0;
-  return (size_t)kMultiMax * rowb + valb <= lds_budget();
+  return multi_smem(ds, L.ef) <= lds_budget();
 }
 }  // namespace
 
@@ -1314,8 +1339,7 @@ void launch_hv_batch_rows_multi(const DevSector& ds, const HvArgs* a, int count,
   }
   m.nk = count;
   const HvParams p = make_params(ds, a[0]);
-  const size_t smem = (size_t)kMultiMax * (ds.dn.D + 1) * sizeof(double) +
-                      (L.ef == 4 ? (size_t)ds.nvalues * sizeof(double) : 0);
+  const size_t smem = multi_smem(ds, L.ef);
   EDH_CHECK(smem <= lds_budget(), EDH_ERR_INTERNAL, "grouped H*v: multi-operand rows do not fit LDS");
   const bool kan = ds.kan_ml > 0;
   switch (ds.cross_mode) {

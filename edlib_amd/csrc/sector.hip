@@ -205,6 +205,7 @@ void upload_sector(const SectorTables& st, int Ns, DevSector& ds, hipStream_t s,
     pk.add(ds.kan_ptr, st.kan_ptr);
     pk.add(ds.kan_off, koff);
     pk.add(ds.kan_c, kc);
+    ds.nkan = (int)kc.size();
   }
   pk.add(ds.counters, std::vector<int32_t>(64, 0));
   pk.add(ds.row_order, rcm_order(st.up.rowp, st.up.col, st.up.D));
