@@ -472,21 +472,27 @@ def test_one_wanted_eigenvalue_and_one_step_lanczos():
     b.close()
 
 
-def test_lanczos_runs_sharing_a_rows_kernel_launch_are_bit_identical():
-    """Lanczos runs on one sector routed to the rows kernel (Kanamori 3 + 11,
-    sector (5,7): 2002 x 3432 = 6,870,864 states) share multi-operand
-    launches when several are active: each run's coefficients are
-    bit-identical to the run alone (single-operand launch) whatever the
-    group (2, 4, 4 + 1 operands), and equal the single-sector recurrence."""
+@pytest.mark.parametrize("case", ["kanamori14", "aim12"])
+def test_lanczos_runs_sharing_a_rows_kernel_launch_are_bit_identical(case):
+    """Lanczos runs on one sector routed to the rows kernel share
+    multi-operand launches when several are active: each run's coefficients
+    are bit-identical to the run alone (single-operand launch) whatever the
+    group (2, 4, 4 + 1 operands), and equal the single-sector recurrence.
+    Kanamori 3 + 11, sector (5,7): 2002 x 3432 = 6,870,864 states (with the
+    impurity terms); Anderson 1 + 11, sector (6,6): 924 x 924 (a mid-size
+    sector that Green's-function runs route to the rows kernel)."""
     import edlib_amd as ea
     from edlib_amd.batch import Batch
     from edlib_amd.workloads import siam_model
-    m = siam_model(14, 3, SEED + 1001)
-    sec, nl = (5, 7), 12
+    if case == "kanamori14":
+        m, sec, dims = siam_model(14, 3, SEED + 1001), (5, 7), (2002, 3432)
+    else:
+        m, sec, dims = siam_model(12, 1, SEED + 4), (6, 6), (924, 924)
+    nl = 12
     ctx = ea.Context(0)
     ctx.set_model(m)
     n = ctx.set_sector(*sec)
-    assert n == 2002 * 3432
+    assert n == dims[0] * dims[1]
     rng = np.random.default_rng(SEED + 41)
     v0s = []
     for _ in range(5):
