@@ -16,7 +16,9 @@ Also reported (same JSON line):
   PCIe both ways, as the reference's host-pointer av() contract does);
 * solves: impurity solves/s of the batched engine and of the continuous-
   batching server against the plain per-sector GPU solve on the same
-  problems, with p50/p99 latency (see edlib_amd/solver.py, serve.py).
+  problems, with p50/p99 latency (see edlib_amd/solver.py, serve.py), and
+  the roofline of the batched engine's grouped H*v launch over all 169
+  sectors of the Anderson 1 + 11 model (solves.batched.grouped_hv).
 
 Prints ONE JSON line.  --gpus N>1 exits: this build runs on one GPU.
 """
